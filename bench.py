@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the state-vector hot path on one MI355X.
+
+Workload (one step): the 30-qubit random grid circuit on a 5x6 grid, 40 cycles of
+{sqrt(X), sqrt(Y), sqrt(W)} single-qubit layers and CZ layers A,B,C,D (1,200 single-qubit gates +
+490 CZ), complex64, started from |0...0>, then 10^6 samples of all qubits (bit-exact NumPy-choice
+sampling in Cirq's factor order) and <Z_i Z_j> for the 49 nearest-neighbour pairs.
+
+Metric: layers (cycles) per second of the whole step; HBM GB/s of the gate-pass kernel.
+Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--rows", type=int, default=5)
+    ap.add_argument("--cols", type=int, default=6)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--reps", type=int, default=1_000_000)
+    ap.add_argument("--tile-bits", type=int, default=None)
+    ap.add_argument("--coalesce-bits", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-qubits", type=int, default=26)
+    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(args, circuit_ops_fn, n_gpu: int, layers: int, ops_per_layer: float):
+    """The oracle (Cirq's algorithm restated in NumPy, complex64) on a bounded sample."""
+    from oracle import cirq_np as O
+
+    n = args.cpu_qubits
+    ops = circuit_ops_fn(n)
+    psi = np.zeros(2**n, dtype=np.complex64)
+    psi[0] = 1
+    t = psi.reshape((2,) * n)
+    # warm-up op
+    O.apply_unitary(t, O.gate_unitary(ops[0][1], ops[0][2]), list(ops[0][3]))
+    done = 0
+    t0 = time.perf_counter()
+    for op in ops:
+        t = O.apply_unitary(t, O.gate_unitary(op[1], op[2]), list(op[3]))
+        done += 1
+        if time.perf_counter() - t0 > args.cpu_budget_s:
+            break
+    dt = time.perf_counter() - t0
+    ops_per_s = done / dt
+    layers_per_s_n = ops_per_s / ops_per_layer
+    # per-op cost scales with the state size: 2^(n_gpu - n) more work per op at n_gpu qubits
+    layers_per_s = layers_per_s_n / (2 ** (n_gpu - n))
+    cores = len(os.sched_getaffinity(0))
+    return {
+        "value": layers_per_s,
+        "unit": "layers/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (
+            f"first {done} ops ({dt:.1f} s) of the same circuit at {n} qubits through the NumPy restatement "
+            f"of Cirq's apply_unitary (complex64, single-threaded NumPy; {cores} cores visible, "
+            f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}): {layers_per_s_n:.4f} layers/s at "
+            f"{n} qubits, scaled by 2^-{n_gpu - n} to {n_gpu} qubits"
+        ),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        if int(os.environ.get("RANK", "0")) == 0:
+            print(
+                "bench.py runs on one GPU only: this simulator has no multi-GPU path "
+                "(sharding is described in DESIGN.md)", file=sys.stderr
+            )
+        sys.exit(2)
+
+    import torch
+
+    import cirq_amd as cirq
+    from cirq_amd import engine, generators
+    from cirq_amd.paulis import pauli_masks
+    from cirq_amd.simulator import product_state_factors
+
+    n = args.rows * args.cols
+    circuit = generators.grid_circuit(args.rows, args.cols, args.depth, seed=0)
+    qubits = sorted(circuit.all_qubits())
+    n_ops = sum(1 for _ in circuit.all_operations())
+    zz = generators.nearest_neighbour_zz(qubits)
+    qmap = {q: i for i, q in enumerate(qubits)}
+    terms = [pauli_masks(t, qmap, n) for z in zz for t in cirq.PauliSum.wrap(z)]
+
+    sim = cirq.Simulator(seed=1234, tile_bits=args.tile_bits, coalesce_bits=args.coalesce_bits)
+    comp = sim._compile(circuit, qubits)
+    factors = product_state_factors(circuit, qubits)
+    assert len(factors) == 1 and len(factors[0]) == n
+    axes = [qmap[q] for q in factors[0]]
+    dev = sim._dev()
+    st = engine.new_state(n, dev)
+    stream = torch.cuda.current_stream()
+    pass_events = []
+
+    def step(record: bool):
+        engine.check(engine.lib().cqs_init_basis(engine.ptr(st), n, 0, engine.stream_handle()), "init")
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        comp.dev.run(st)
+        e1.record(stream)
+        if record:
+            pass_events.append((e0, e1))
+        u = sim._prng.random_sample(args.reps)
+        idx = sim._sampler.draw(st, n, False, u, axes=axes)
+        bits = sim._sampler.bits(idx, n)
+        vals = engine.expect_terms(st, n, False, terms, sim._ws).cpu().numpy()
+        return bits, vals
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bits, vals = step(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    pass_ms = sum(a.elapsed_time(b) for a, b in pass_events) / len(pass_events)
+    n_pass = int(comp.packed.n_passes)
+    avg_launch_s = pass_ms / 1e3 / n_pass
+    bytes_per_launch = 2 * 8 * (2**n)
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    ms_per_step = wall / args.steps * 1e3
+    layers_per_s = args.depth * args.steps / wall
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_k_pass_{n}q.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": "30q layers/s & HBM GB/s; adjoint vs param-shift grads/s at grad err<=1e-5",
+        "value": layers_per_s,
+        "unit": "layers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64 (fp32 pairs); fp64 reductions",
+        "data": "synthetic: seeded random grid circuit (Cirq's generator restated)",
+        "config": {
+            "workload": f"random grid circuit {args.rows}x{args.cols} ({n} qubits), depth {args.depth}, "
+            f"{n_ops} gates, {args.reps} samples, {len(zz)} <ZZ>",
+            "qubits": n,
+            "depth": args.depth,
+            "gates": n_ops,
+            "samples": args.reps,
+            "zz_pairs": len(zz),
+            "passes": n_pass,
+            "tile_bits": int(comp.packed.pass_arr["L"].max()) if n_pass else 0,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "extra": {
+            "pass_kernel_avg_ms": avg_launch_s * 1e3,
+            "passes_ms_per_step": pass_ms,
+            "gate_applications_per_s": n_ops * args.steps / wall,
+            "hbm_gbs_whole_step": n_pass * bytes_per_launch / (ms_per_step / 1e3) / 1e9,
+            "sample_check": int(bits.shape[0]),
+        },
+    }
+    if not args.no_cpu_baseline:
+        from tests.helpers import to_oracle_ops
+
+        def ops_fn(nq):
+            c = generators.grid_circuit(args.rows, args.cols, args.depth, seed=0, n_qubits=nq)
+            return to_oracle_ops(c, sorted(c.all_qubits()))
+
+        out["cpu_baseline"] = cpu_baseline(args, ops_fn, n, args.depth, n_ops / args.depth)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

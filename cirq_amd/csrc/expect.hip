@@ -1,0 +1,226 @@
+// Pauli-string expectations, PauliSum application and norms (FP64 accumulation).
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cqs {
+
+constexpr int kTermsPerLaunch = 16;
+constexpr int kRedBlocks = 512;  // fixed grid => deterministic reduction order
+
+struct TermBatch {
+  uint64_t x;                    // shared flip mask
+  int count;
+  int pad;
+  uint64_t z[kTermsPerLaunch];
+};
+
+// partials[(blockIdx * 16 + k) * 2 + {0,1}] = this block's share of term k.
+__global__ __launch_bounds__(256) void k_expect(const float2* __restrict__ amps, int n, int is_dm,
+                                                TermBatch tb, double* __restrict__ partials) {
+  __shared__ double red[4][kTermsPerLaunch][2];
+  const uint64_t N = 1ull << n;
+  double ar[kTermsPerLaunch], ai[kTermsPerLaunch];
+#pragma unroll
+  for (int k = 0; k < kTermsPerLaunch; ++k) ar[k] = ai[k] = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+    double pr, pi;
+    if (is_dm) {
+      const float2 v = amps[i * N + (i ^ tb.x)];
+      pr = v.x;
+      pi = v.y;
+    } else {
+      const float2 a = amps[i];
+      const float2 b = amps[i ^ tb.x];
+      const double a_r = a.x, a_i = a.y, b_r = b.x, b_i = b.y;
+      pr = b_r * a_r + b_i * a_i;  // conj(b) * a
+      pi = b_r * a_i - b_i * a_r;
+    }
+#pragma unroll
+    for (int k = 0; k < kTermsPerLaunch; ++k) {
+      if (k < tb.count) {
+        const bool neg = __popcll(i & tb.z[k]) & 1;
+        ar[k] += neg ? -pr : pr;
+        ai[k] += neg ? -pi : pi;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kTermsPerLaunch; ++k) {
+    if (k < tb.count) {
+      const double sr = wave_sum(ar[k]), si = wave_sum(ai[k]);
+      if (lane == 0) {
+        red[w][k][0] = sr;
+        red[w][k][1] = si;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < tb.count * 2) {
+    const int k = threadIdx.x >> 1, c = threadIdx.x & 1;
+    const double s = (red[0][k][c] + red[1][k][c]) + (red[2][k][c] + red[3][k][c]);
+    partials[((size_t)blockIdx.x * kTermsPerLaunch + k) * 2 + c] = s;
+  }
+}
+
+struct FinalBatch {
+  int count;
+  int nblocks;
+  int out_idx[kTermsPerLaunch];
+  double cr[kTermsPerLaunch], ci[kTermsPerLaunch];
+};
+
+__global__ __launch_bounds__(64) void k_expect_final(const double* __restrict__ partials, FinalBatch fb,
+                                                     double* __restrict__ out) {
+  const int lane = threadIdx.x;
+  for (int k = 0; k < fb.count; ++k) {
+    double sr = 0.0, si = 0.0;
+    for (int b = lane; b < fb.nblocks; b += 64) {
+      sr += partials[((size_t)b * kTermsPerLaunch + k) * 2 + 0];
+      si += partials[((size_t)b * kTermsPerLaunch + k) * 2 + 1];
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) {
+      out[2 * fb.out_idx[k] + 0] = fb.cr[k] * sr - fb.ci[k] * si;
+      out[2 * fb.out_idx[k] + 1] = fb.cr[k] * si + fb.ci[k] * sr;
+    }
+  }
+}
+
+constexpr int kApplyTerms = 32;
+struct ApplyBatch {
+  int count;
+  int accumulate;
+  uint64_t x[kApplyTerms];
+  uint64_t z[kApplyTerms];
+  double cr[kApplyTerms], ci[kApplyTerms];
+};
+
+// phi[j] (+)= sum_t c_t (-1)^popcount((j ^ x_t) & z_t) psi[j ^ x_t]
+__global__ __launch_bounds__(256) void k_apply_paulisum(const float2* __restrict__ psi, float2* __restrict__ phi,
+                                                        uint64_t N, ApplyBatch ab) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride) {
+    double sr = 0.0, si = 0.0;
+    if (ab.accumulate) {
+      const float2 o = phi[j];
+      sr = o.x;
+      si = o.y;
+    }
+    for (int t = 0; t < ab.count; ++t) {
+      const uint64_t src = j ^ ab.x[t];
+      const float2 v = psi[src];
+      const double s = (__popcll(src & ab.z[t]) & 1) ? -1.0 : 1.0;
+      const double vr = s * v.x, vi = s * v.y;
+      sr += ab.cr[t] * vr - ab.ci[t] * vi;
+      si += ab.cr[t] * vi + ab.ci[t] * vr;
+    }
+    phi[j] = make_float2((float)sr, (float)si);
+  }
+}
+
+}  // namespace cqs
+
+using namespace cqs;
+
+extern "C" {
+
+size_t cqs_expect_workspace_bytes(int32_t n_terms) {
+  (void)n_terms;
+  return sizeof(double) * 2 * kTermsPerLaunch * kRedBlocks;
+}
+
+int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms,
+                      const uint64_t* xmask, const uint64_t* zmask, const double* coef, double* out,
+                      void* workspace, void* stream) {
+  CQS_REQUIRE(amps != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_qubits >= 0 && n_qubits <= (is_dm ? 20 : CQS_MAX_BITS), "n_qubits=%d out of range",
+              n_qubits);
+  CQS_REQUIRE(n_terms >= 0, "n_terms < 0");
+  if (n_terms == 0) return CQS_OK;
+  CQS_REQUIRE(xmask && zmask && coef, "NULL term arrays");
+  const uint64_t valid = n_qubits >= 64 ? ~0ull : ((1ull << n_qubits) - 1);
+  std::vector<int> order(n_terms);
+  std::iota(order.begin(), order.end(), 0);
+  for (int t = 0; t < n_terms; ++t)
+    CQS_REQUIRE(!(xmask[t] & ~valid) && !(zmask[t] & ~valid), "term %d has masks beyond %d qubits", t,
+                n_qubits);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return xmask[a] < xmask[b]; });
+  hipStream_t s = as_stream(stream);
+  double* partials = (double*)workspace;
+  const uint64_t N = 1ull << n_qubits;
+  int grid = (int)std::min<uint64_t>(kRedBlocks, (N + 255) / 256);
+  if (grid < 1) grid = 1;
+  for (int i = 0; i < n_terms;) {
+    TermBatch tb;
+    FinalBatch fb;
+    tb.x = xmask[order[i]];
+    tb.count = 0;
+    tb.pad = 0;
+    while (i < n_terms && tb.count < kTermsPerLaunch && xmask[order[i]] == tb.x) {
+      const int t = order[i];
+      tb.z[tb.count] = zmask[t];
+      fb.out_idx[tb.count] = t;
+      fb.cr[tb.count] = coef[2 * t];
+      fb.ci[tb.count] = coef[2 * t + 1];
+      ++tb.count;
+      ++i;
+    }
+    for (int k = tb.count; k < kTermsPerLaunch; ++k) tb.z[k] = 0;
+    fb.count = tb.count;
+    fb.nblocks = grid;
+    hipLaunchKernelGGL(k_expect, dim3(grid), dim3(256), 0, s, (const float2*)amps, (int)n_qubits, (int)is_dm,
+                       tb, partials);
+    hipLaunchKernelGGL(k_expect_final, dim3(1), dim3(64), 0, s, (const double*)partials, fb, out);
+  }
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+int cqs_norm2(const void* amps, int32_t n_bits, double* out, void* workspace, void* stream) {
+  // A 1-term "expectation" of the identity over all 2^n_bits entries, real part in out[0].
+  CQS_REQUIRE(amps != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_bits >= 0 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
+  const uint64_t x = 0, z = 0;
+  const double c[2] = {1.0, 0.0};
+  return cqs_expect_paulis(amps, n_bits, 0, 1, &x, &z, c, out, workspace, stream);
+}
+
+int cqs_apply_paulisum(const void* psi, void* phi, int32_t n_qubits, int32_t n_terms, const uint64_t* xmask,
+                       const uint64_t* zmask, const double* coef, void* stream) {
+  CQS_REQUIRE(psi != nullptr && phi != nullptr && psi != phi, "psi/phi must be distinct non-NULL");
+  CQS_REQUIRE(n_qubits >= 0 && n_qubits <= CQS_MAX_BITS, "n_qubits=%d out of range", n_qubits);
+  CQS_REQUIRE(n_terms >= 0, "n_terms < 0");
+  CQS_REQUIRE(n_terms == 0 || (xmask && zmask && coef), "NULL term arrays");
+  const uint64_t valid = (1ull << n_qubits) - 1;
+  for (int t = 0; t < n_terms; ++t)
+    CQS_REQUIRE(!(xmask[t] & ~valid) && !(zmask[t] & ~valid), "term %d has masks beyond %d qubits", t, n_qubits);
+  const uint64_t N = 1ull << n_qubits;
+  hipStream_t s = as_stream(stream);
+  int grid = (int)std::min<uint64_t>(65536, (N + 255) / 256);
+  if (grid < 1) grid = 1;
+  int done = 0;
+  do {
+    ApplyBatch ab;
+    ab.count = std::min(kApplyTerms, n_terms - done);
+    ab.accumulate = done > 0;
+    for (int k = 0; k < kApplyTerms; ++k) {
+      const bool v = k < ab.count;
+      ab.x[k] = v ? xmask[done + k] : 0;
+      ab.z[k] = v ? zmask[done + k] : 0;
+      ab.cr[k] = v ? coef[2 * (done + k)] : 0.0;
+      ab.ci[k] = v ? coef[2 * (done + k) + 1] : 0.0;
+    }
+    hipLaunchKernelGGL(k_apply_paulisum, dim3(grid), dim3(256), 0, s, (const float2*)psi, (float2*)phi, N, ab);
+    done += ab.count;
+  } while (done < n_terms);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,726 @@
+// Measurement sampling, bit-exact with NumPy's RandomState.choice as Cirq calls it.
+//
+// Cirq (cirq-core/cirq/sim/state_vector.py:219-226, cirq-core/cirq/sim/simulation_utils.py:62-64):
+//   q = (psi * conj(psi)).real                  float32; NumPy evaluates fma(a, a, b*b)
+//   q = clip(q, 0, None); p = q / q.sum()       float32; q.sum() = sequential float32 sum over
+//                                               8192-element buffers of pairwise sums
+//   choice: cdf = cumsum(float64(p)); cdf /= cdf[-1]; idx = searchsorted(cdf, u, 'right')
+//
+// The float64 cumsum is sequential.  It is reproduced exactly in parallel: inside one binade
+// [2^E, 2^(E+1)) every partial sum is a multiple of u = 2^(E-52), and rounding s + x depends on s
+// only through the parity of s/u (ties to even).  So a chunk of additions that stays inside one
+// binade maps s to s + d[parity(s/u)]*u; such maps compose associatively
+// (compose(f, g)[p] = f[p] + g[p ^ (f[p] & 1)]).  Chunks that cross a binade edge (a few dozen as
+// the sum climbs from 0 to 1) are summed element by element.  Every map is verified against the
+// exact incoming value before use, so a misclassified chunk only costs time.
+#include "common.hpp"
+
+namespace cqs {
+
+constexpr int kChunk = 256;         // elements per chunk
+constexpr int kSuper = 1024;        // chunks per superchunk
+constexpr int kNpBuf = 8192;        // NumPy's reduction buffer (elements)
+constexpr int kChunksPerWG = 32;    // chunks staged per workgroup in the chunk kernels
+
+struct ChunkFn {
+  long long d0, d1;  // increments in ulps for incoming parity 0 / 1
+  int E;             // binade exponent
+  int ok;            // 1 = usable map
+  long long pad;
+};
+
+struct SampleHdr {
+  float T32;        // NumPy's float32 normalisation sum
+  float pad0;
+  double total;     // cdf[-1] before normalisation (exact NumPy value)
+  int n_seq_chunks; // chunks summed element by element (diagnostic)
+  int pad1[59];
+};
+
+struct Layout {
+  uint64_t N, NC, Cl, NSC, SCl, NB;
+  size_t o_hdr, o_csum32, o_capx, o_cin, o_cfn, o_scsum, o_scpre, o_scfn, o_scstart, o_scmode,
+      o_cstart, bytes;
+};
+
+static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+
+static Layout layout_for(int n) {
+  Layout l;
+  l.N = 1ull << n;
+  l.NC = l.N >= (uint64_t)kChunk ? l.N / kChunk : 1;
+  l.Cl = l.N / l.NC;
+  l.NSC = l.NC >= (uint64_t)kSuper ? l.NC / kSuper : 1;
+  l.SCl = l.NC / l.NSC;
+  l.NB = l.N >= (uint64_t)kNpBuf ? l.N / kNpBuf : 1;
+  size_t o = 0;
+  l.o_hdr = o; o += al(sizeof(SampleHdr));
+  l.o_csum32 = o; o += al(l.NB * sizeof(float));
+  l.o_capx = o; o += al(l.NC * sizeof(double));
+  l.o_cin = o; o += al(l.NC * sizeof(double));
+  l.o_cfn = o; o += al(l.NC * sizeof(ChunkFn));
+  l.o_scsum = o; o += al(l.NSC * sizeof(double));
+  l.o_scpre = o; o += al(l.NSC * sizeof(double));
+  l.o_scfn = o; o += al(l.NSC * sizeof(ChunkFn));
+  l.o_scstart = o; o += al(l.NSC * sizeof(double));
+  l.o_scmode = o; o += al(l.NSC * sizeof(int));
+  l.o_cstart = o; o += al(l.NC * sizeof(double));
+  l.bytes = o;
+  return l;
+}
+
+// Probability source: |psi_i|^2, Re rho_ii, or a caller buffer; clipped at 0 like np.clip.
+// With permute != 0, sample index i is first mapped to the state index whose bit perm[b] is
+// bit b of i (all qubits measured in a non-natural order).
+struct Loader {
+  const float2* amps;
+  const float* probs;
+  uint64_t N;
+  int mode;  // 0: state vector, 1: density-matrix diagonal, 2: float buffer
+  int nbits;
+  int permute;
+  int8_t perm[CQS_MAX_BITS];
+  __device__ __forceinline__ uint64_t map(uint64_t i) const {
+    if (!permute) return i;
+    uint64_t r = 0;
+    for (int b = 0; b < nbits; ++b) r |= ((i >> b) & 1ull) << perm[b];
+    return r;
+  }
+  __device__ __forceinline__ float q(uint64_t i) const {
+    float v;
+    if (mode == 0) {
+      const float2 a = amps[map(i)];
+      v = fmaf(a.x, a.x, a.y * a.y);
+    } else if (mode == 1) {
+      const uint64_t k = map(i);
+      v = amps[k * (N + 1)].x;
+    } else {
+      v = probs[i];
+    }
+    return v < 0.f ? 0.f : v;
+  }
+};
+
+__device__ __forceinline__ double ldexp_d(double x, int e) { return ldexp(x, e); }
+
+// ---------------------------------------------------------------- NumPy float32 sum
+// Pairwise sum of one buffer of len (power of two <= 8192) exactly as NumPy's
+// FLOAT_pairwise_sum: blocks of 128 with 8 strided accumulators, halves combined recursively.
+__global__ __launch_bounds__(256) void k_sum_buffers(Loader ld, uint64_t len, float* __restrict__ out) {
+  __shared__ float sq[kNpBuf + kNpBuf / 128];
+  const uint64_t off = (uint64_t)blockIdx.x * len;
+  for (uint64_t i = threadIdx.x; i < len; i += 256) sq[i + (i >> 7)] = ld.q(off + i);
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int t = threadIdx.x;
+  if (len < 128) {
+    if (t == 0) {
+      float res;
+      if (len < 8) {
+        res = 0.f;
+        for (uint64_t i = 0; i < len; ++i) res = res + sq[i];
+      } else {
+        float r[8];
+        for (int j = 0; j < 8; ++j) r[j] = sq[j];
+        for (uint64_t i = 8; i < len; i += 8)
+          for (int j = 0; j < 8; ++j) r[j] = r[j] + sq[i + j];
+        res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+      }
+      out[blockIdx.x] = res;
+    }
+    return;
+  }
+  const int nblk = (int)(len / 128);
+  float s = 0.f;
+  if (t < nblk) {
+    const float* b = sq + t * 129;
+    float r[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = b[j];
+    for (int i = 8; i < 128; i += 8)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) r[j] = r[j] + b[i + j];
+    s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  }
+  // adjacent-pair tree == NumPy's recursive halving for power-of-two lengths
+  for (int o = 1; o < nblk; o <<= 1) s = s + __shfl_xor(s, o, 64);
+  if (t == 0) out[blockIdx.x] = s;
+}
+
+// Sequential float32 accumulation of the buffer sums (NumPy's reduce loop).
+__global__ void k_total32(const float* __restrict__ bsum, uint64_t nb, SampleHdr* hdr) {
+  if (threadIdx.x != 0) return;
+  float S = 0.f;
+  uint64_t c = 0;
+  for (; c + 8 <= nb; c += 8) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = bsum[c + j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) S = S + v[j];
+  }
+  for (; c < nb; ++c) S = S + bsum[c];
+  hdr->T32 = S;
+}
+
+__device__ __forceinline__ double px(const Loader& ld, float T32, uint64_t i) {
+  return (double)__fdiv_rn(ld.q(i), T32);
+}
+
+// ---------------------------------------------------------------- approximate chunk sums
+__global__ __launch_bounds__(256) void k_chunk_approx(Loader ld, const SampleHdr* hdr, uint64_t NC,
+                                                      uint64_t Cl, double* __restrict__ capx) {
+  __shared__ double part[256];
+  const float T32 = hdr->T32;
+  const uint64_t c0 = (uint64_t)blockIdx.x * kChunksPerWG;
+  // 8 threads per chunk, each summing a strided slice (order is irrelevant here)
+  const int lc = threadIdx.x >> 3, sub = threadIdx.x & 7;
+  const uint64_t c = c0 + lc;
+  double s = 0.0;
+  if (c < NC)
+    for (uint64_t i = sub; i < Cl; i += 8) s += px(ld, T32, c * Cl + i);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (sub == 0 && c < NC) {
+    double t = 0.0;
+    for (int k = 0; k < 8; ++k) t += part[threadIdx.x + k];
+    capx[c] = t;
+  }
+}
+
+// Exclusive scan of approximate chunk sums inside each superchunk; superchunk totals out.
+__global__ __launch_bounds__(256) void k_scan_local(const double* __restrict__ capx, uint64_t SCl,
+                                                    double* __restrict__ cin, double* __restrict__ scsum) {
+  __shared__ double sh[256];
+  const uint64_t base = (uint64_t)blockIdx.x * SCl;
+  const int per = (int)((SCl + 255) / 256);
+  const int t = threadIdx.x;
+  double loc = 0.0;
+  for (int k = 0; k < per; ++k) {
+    const uint64_t i = (uint64_t)t * per + k;
+    if (i < SCl) loc += capx[base + i];
+  }
+  sh[t] = loc;
+  __syncthreads();
+  if (t == 0) {
+    double run = 0.0;
+    for (int k = 0; k < 256; ++k) {
+      const double v = sh[k];
+      sh[k] = run;
+      run += v;
+    }
+    scsum[blockIdx.x] = run;
+  }
+  __syncthreads();
+  double run = sh[t];
+  for (int k = 0; k < per; ++k) {
+    const uint64_t i = (uint64_t)t * per + k;
+    if (i < SCl) {
+      cin[base + i] = run;
+      run += capx[base + i];
+    }
+  }
+}
+
+__global__ void k_scan_top(const double* __restrict__ scsum, uint64_t NSC, double* __restrict__ scpre) {
+  if (threadIdx.x != 0) return;
+  double run = 0.0;
+  for (uint64_t i = 0; i < NSC; ++i) {
+    scpre[i] = run;
+    run += scsum[i];
+  }
+}
+
+// ---------------------------------------------------------------- chunk maps
+__device__ __forceinline__ int binade_of(double x) {
+  int e;
+  frexp(x, &e);
+  return e - 1;  // x in [2^E, 2^(E+1))
+}
+
+__global__ __launch_bounds__(256) void k_chunk_fn(Loader ld, const SampleHdr* hdr, uint64_t NC,
+                                                  uint64_t Cl, uint64_t SCl, const double* __restrict__ capx,
+                                                  const double* __restrict__ cin, const double* __restrict__ scpre,
+                                                  ChunkFn* __restrict__ cfn) {
+  __shared__ float sp[kChunksPerWG * (kChunk + 1)];
+  __shared__ int ovf[2 * kChunksPerWG];
+  const float T32 = hdr->T32;
+  const uint64_t c0 = (uint64_t)blockIdx.x * kChunksPerWG;
+  const uint64_t nloc = (NC - c0) < (uint64_t)kChunksPerWG ? (NC - c0) : (uint64_t)kChunksPerWG;
+  for (uint64_t i = threadIdx.x; i < nloc * Cl; i += 256) {
+    const uint64_t lc = i / Cl, k = i % Cl;
+    sp[lc * (kChunk + 1) + k] = __fdiv_rn(ld.q((c0 + lc) * Cl + k), T32);
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int lc = t & (kChunksPerWG - 1);
+  const int par = t / kChunksPerWG;  // 0 or 1 for t < 64
+  const uint64_t c = c0 + lc;
+  int E = 0, clean = 0;
+  long long d = 0;
+  int over = 1;
+  if (t < 2 * kChunksPerWG && c < NC) {
+    const double a = cin[c] + scpre[c / SCl];
+    const double b = a + capx[c];
+    const double eps = 0x1p-40;
+    if (a > 0.0) {
+      E = binade_of(a);
+      const double lo = ldexp_d(1.0, E), hi = ldexp_d(1.0, E + 1);
+      clean = (a * (1.0 - eps) >= lo) && (b * (1.0 + eps) < hi);
+    }
+    if (clean) {
+      const double u = ldexp_d(1.0, E - 52);
+      const double s0 = ldexp_d(1.0, E) + (par ? u : 0.0);
+      double s = s0;
+      const float* row = sp + lc * (kChunk + 1);
+      for (uint64_t k = 0; k < Cl; ++k) s = s + (double)row[k];
+      over = !(s < ldexp_d(1.0, E + 1));
+      d = (long long)ldexp_d(s - s0, 52 - E);
+    }
+    ovf[t] = over;
+  }
+  __syncthreads();
+  if (t < 2 * kChunksPerWG && c < NC) {
+    if (par == 0) {
+      cfn[c].d0 = d;
+      cfn[c].E = E;
+      cfn[c].ok = clean && !ovf[lc] && !ovf[lc + kChunksPerWG];
+    } else {
+      cfn[c].d1 = d;
+    }
+  }
+}
+
+__device__ __forceinline__ void compose(long long& a0, long long& a1, long long b0, long long b1) {
+  // (apply a, then b)
+  const long long r0 = a0 + ((a0 & 1) ? b1 : b0);
+  const long long r1 = a1 + ((a1 & 1) ? b0 : b1);
+  a0 = r0;
+  a1 = r1;
+}
+
+// Compose the chunk maps of each superchunk when all share one binade.
+__global__ __launch_bounds__(256) void k_super_fn(const ChunkFn* __restrict__ cfn, uint64_t SCl,
+                                                  ChunkFn* __restrict__ scfn) {
+  __shared__ long long s0[256], s1[256];
+  __shared__ int okall, E0;
+  const uint64_t base = (uint64_t)blockIdx.x * SCl;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    okall = 1;
+    E0 = cfn[base].E;
+  }
+  __syncthreads();
+  const int per = (int)((SCl + 255) / 256);
+  long long a0 = 0, a1 = 0;
+  int ok = 1;
+  for (int k = 0; k < per; ++k) {
+    const uint64_t i = (uint64_t)t * per + k;
+    if (i < SCl) {
+      const ChunkFn f = cfn[base + i];
+      if (!f.ok || f.E != E0) ok = 0;
+      compose(a0, a1, f.d0, f.d1);
+    }
+  }
+  if (!ok) okall = 0;  // benign race: every writer stores 0
+  s0[t] = a0;
+  s1[t] = a1;
+  __syncthreads();
+  if (t == 0) {
+    long long r0 = 0, r1 = 0;
+    for (int k = 0; k < 256; ++k) compose(r0, r1, s0[k], s1[k]);
+    ChunkFn out;
+    out.d0 = r0;
+    out.d1 = r1;
+    out.E = E0;
+    out.ok = okall;
+    out.pad = 0;
+    scfn[blockIdx.x] = out;
+  }
+}
+
+// Applies map f to exact value S if S lies in binade E and the result stays inside it.
+__device__ __forceinline__ bool try_apply(double& S, int E, long long d0, long long d1) {
+  if (!(S > 0.0)) return false;
+  const double lo = ldexp_d(1.0, E), hi = ldexp_d(1.0, E + 1);
+  if (!(S >= lo && S < hi)) return false;
+  const long long k = (long long)ldexp_d(S, 52 - E);
+  const long long d = (k & 1) ? d1 : d0;
+  const double r = S + ldexp_d((double)d, E - 52);
+  if (!(r < hi)) return false;
+  S = r;
+  return true;
+}
+
+// One workgroup of 64 lanes walks the superchunks in order.  All lanes keep identical copies
+// of S; lanes cooperate only to stage the elements of a chunk that must be summed one by one.
+__global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, uint64_t NSC, uint64_t SCl,
+                                                    uint64_t Cl, const ChunkFn* __restrict__ cfn,
+                                                    const ChunkFn* __restrict__ scfn,
+                                                    double* __restrict__ scstart, int* __restrict__ scmode,
+                                                    double* __restrict__ cstart) {
+  __shared__ double xs[kChunk];
+  const float T32 = hdr->T32;
+  const int t = threadIdx.x;
+  double S = 0.0;
+  int nseq = 0;
+  for (uint64_t sc = 0; sc < NSC; ++sc) {
+    if (t == 0) scstart[sc] = S;
+    const ChunkFn F = scfn[sc];
+    if (F.ok && try_apply(S, F.E, F.d0, F.d1)) {
+      if (t == 0) scmode[sc] = 1;
+      continue;
+    }
+    if (t == 0) scmode[sc] = 0;
+    for (uint64_t k = 0; k < SCl; ++k) {
+      const uint64_t c = sc * SCl + k;
+      if (t == 0) cstart[c] = S;
+      const ChunkFn f = cfn[c];
+      if (f.ok && try_apply(S, f.E, f.d0, f.d1)) continue;
+      __syncthreads();
+      for (uint64_t i = t; i < Cl; i += 64) xs[i] = px(ld, T32, c * Cl + i);
+      __syncthreads();
+      for (uint64_t i = 0; i < Cl; ++i) S = S + xs[i];
+      ++nseq;
+    }
+  }
+  if (t == 0) {
+    hdr->total = S;
+    hdr->n_seq_chunks = nseq;
+  }
+}
+
+// Chunk start values inside superchunks that were advanced by one composed map.
+__global__ __launch_bounds__(256) void k_chunk_starts(const ChunkFn* __restrict__ cfn, uint64_t SCl,
+                                                      const double* __restrict__ scstart,
+                                                      const int* __restrict__ scmode,
+                                                      const ChunkFn* __restrict__ scfn,
+                                                      double* __restrict__ cstart) {
+  if (!scmode[blockIdx.x]) return;
+  __shared__ long long g0[256], g1[256];
+  const uint64_t base = (uint64_t)blockIdx.x * SCl;
+  const int t = threadIdx.x;
+  const int per = (int)((SCl + 255) / 256);
+  long long a0 = 0, a1 = 0;
+  for (int k = 0; k < per; ++k) {
+    const uint64_t i = (uint64_t)t * per + k;
+    if (i < SCl) compose(a0, a1, cfn[base + i].d0, cfn[base + i].d1);
+  }
+  g0[t] = a0;
+  g1[t] = a1;
+  __syncthreads();
+  // inclusive Hillis-Steele scan; element t becomes (x_0 ... x_t) applied in order
+  for (int o = 1; o < 256; o <<= 1) {
+    long long p0 = 0, p1 = 0, c0 = g0[t], c1 = g1[t];
+    const bool has = t >= o;
+    if (has) {
+      p0 = g0[t - o];
+      p1 = g1[t - o];
+    }
+    __syncthreads();
+    if (has) {
+      compose(p0, p1, c0, c1);  // earlier prefix first, then this element
+      g0[t] = p0;
+      g1[t] = p1;
+    }
+    __syncthreads();
+  }
+  const int E = scfn[blockIdx.x].E;
+  const double S0 = scstart[blockIdx.x];
+  const long long k = (long long)ldexp_d(S0, 52 - E);
+  const int p = (int)(k & 1);
+  long long acc = 0;
+  if (t > 0) acc = p ? g1[t - 1] : g0[t - 1];
+  for (int kk = 0; kk < per; ++kk) {
+    const uint64_t i = (uint64_t)t * per + kk;
+    if (i < SCl) {
+      cstart[base + i] = S0 + ldexp_d((double)acc, E - 52);
+      const ChunkFn f = cfn[base + i];
+      const int pp = p ^ (int)(acc & 1);
+      acc += pp ? f.d1 : f.d0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- draws
+__global__ __launch_bounds__(256) void k_draw(Loader ld, const SampleHdr* hdr, uint64_t NC, uint64_t Cl,
+                                              const double* __restrict__ cstart,
+                                              const double* __restrict__ uni, int64_t reps,
+                                              int64_t* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= reps) return;
+  const double T = hdr->total;
+  const float T32 = hdr->T32;
+  const double u = uni[r];
+  // first chunk whose end value, normalised, exceeds u
+  uint64_t lo = 0, hi = NC - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    const double end = (mid + 1 < NC) ? cstart[mid + 1] : T;
+    if (end / T > u)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  const uint64_t c = lo;
+  double s = cstart[c];
+  int64_t idx = (int64_t)(c * Cl + Cl - 1);
+  for (uint64_t i = 0; i < Cl; ++i) {
+    s = s + px(ld, T32, c * Cl + i);
+    if (s / T > u) {
+      idx = (int64_t)(c * Cl + i);
+      break;
+    }
+  }
+  out[r] = idx;
+}
+
+__global__ void k_bits(const int64_t* __restrict__ idx, int64_t reps, int m, uint8_t* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t total = reps * m;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int64_t r = e / m;
+    const int k = (int)(e % m);
+    out[e] = (uint8_t)((idx[r] >> (m - 1 - k)) & 1);
+  }
+}
+
+// Marginal probabilities: measured axes first (in the given order), then the rest; sum over the
+// rest with NumPy's pairwise order per row (rows are contiguous after the transpose).
+struct AxisBits {
+  int b[64];
+};
+
+// Pairwise sum (NumPy order) of a row of length len <= 128 held by one thread.
+template <typename F>
+__device__ __forceinline__ float pairwise_short(uint64_t len, F at) {
+  if (len < 8) {
+    float res = 0.f;
+    for (uint64_t i = 0; i < len; ++i) res = res + at(i);
+    return res;
+  }
+  float r[8];
+  for (int j = 0; j < 8; ++j) r[j] = at(j);
+  for (uint64_t i = 8; i < len; i += 8)
+    for (int j = 0; j < 8; ++j) r[j] = r[j] + at(i + j);
+  return ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+}
+
+// Short rows (<= 128 unmeasured combinations): one thread per output row.
+__global__ __launch_bounds__(256) void k_marginal_short(Loader ld, int n, AxisBits ab, int m, uint64_t rows,
+                                                        float* __restrict__ out) {
+  const uint64_t row = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (row >= rows) return;
+  const int rlen_bits = n - m;
+  uint64_t rbase = 0;
+  for (int b = 0; b < m; ++b) rbase |= ((row >> (m - 1 - b)) & 1ull) << ab.b[b];
+  auto at = [&](uint64_t k) {
+    uint64_t fi = rbase;
+    for (int b = 0; b < rlen_bits; ++b) fi |= ((k >> (rlen_bits - 1 - b)) & 1ull) << ab.b[m + b];
+    return ld.q(fi);
+  };
+  out[row] = pairwise_short(1ull << rlen_bits, at);
+}
+
+__global__ __launch_bounds__(256) void k_marginal(Loader ld, int n, AxisBits ab, int m, float* __restrict__ out) {
+  const int* axes_bits = ab.b;
+  // axes_bits[0..m) : flat bit of measured qubit k (k-th most significant output bit)
+  // axes_bits[m..n) : flat bits of the unmeasured qubits in ascending axis order
+  __shared__ float sq[kNpBuf + kNpBuf / 128];
+  const int rlen_bits = n - m;
+  const uint64_t rlen = 1ull << rlen_bits;
+  const uint64_t row = blockIdx.x;
+  // flat index of (row, k): row bits -> measured positions, k bits -> unmeasured positions
+  uint64_t rbase = 0;
+  for (int b = 0; b < m; ++b) rbase |= ((row >> (m - 1 - b)) & 1ull) << axes_bits[b];
+  float S = 0.f;
+  for (uint64_t bufo = 0; bufo < rlen; bufo += kNpBuf) {
+    const uint64_t len = (rlen - bufo) < (uint64_t)kNpBuf ? (rlen - bufo) : (uint64_t)kNpBuf;
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < len; i += 256) {
+      const uint64_t k = bufo + i;
+      uint64_t fi = rbase;
+      for (int b = 0; b < rlen_bits; ++b) fi |= ((k >> (rlen_bits - 1 - b)) & 1ull) << axes_bits[m + b];
+      sq[i + (i >> 7)] = ld.q(fi);
+    }
+    __syncthreads();
+    float res = 0.f;
+    if (threadIdx.x < 64) {
+      const int t = threadIdx.x;
+      if (len < 128) {
+        if (len < 8) {
+          for (uint64_t i = 0; i < len; ++i) res = res + sq[i];
+        } else {
+          float r[8];
+          for (int j = 0; j < 8; ++j) r[j] = sq[j];
+          for (uint64_t i = 8; i < len; i += 8)
+            for (int j = 0; j < 8; ++j) r[j] = r[j] + sq[i + j];
+          res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        }
+      } else {
+        const int nblk = (int)(len / 128);
+        float s = 0.f;
+        if (t < nblk) {
+          const float* bb = sq + t * 129;
+          float r[8];
+          for (int j = 0; j < 8; ++j) r[j] = bb[j];
+          for (int i = 8; i < 128; i += 8)
+            for (int j = 0; j < 8; ++j) r[j] = r[j] + bb[i + j];
+          s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        }
+        for (int o = 1; o < nblk; o <<= 1) s = s + __shfl_xor(s, o, 64);
+        res = s;
+      }
+    }
+    if (threadIdx.x == 0) S = S + res;
+  }
+  if (threadIdx.x == 0) out[row] = S;
+}
+
+}  // namespace cqs
+
+using namespace cqs;
+
+extern "C" {
+
+size_t cqs_sample_workspace_bytes(int32_t n_qubits) {
+  if (n_qubits < 0 || n_qubits > 40) return 0;
+  return layout_for(n_qubits).bytes;
+}
+
+static Loader make_loader(const void* amps, const float* probs_in, int n, int is_dm, const int32_t* perm = nullptr) {
+  Loader ld;
+  ld.amps = (const float2*)amps;
+  ld.probs = probs_in;
+  ld.N = 1ull << n;
+  ld.mode = probs_in ? 2 : (is_dm ? 1 : 0);
+  ld.nbits = n;
+  ld.permute = 0;
+  for (int b = 0; b < CQS_MAX_BITS; ++b) ld.perm[b] = 0;
+  if (perm != nullptr && !probs_in) {
+    ld.permute = 1;
+    for (int b = 0; b < n; ++b) ld.perm[b] = (int8_t)perm[b];
+  }
+  return ld;
+}
+
+// qubit_axes (HOST, n entries or NULL): sampling order of the qubits.  Returns false if it is not
+// a permutation; fills perm with the flat bit of each sample-index bit.
+static bool axes_to_perm(const int32_t* axes, int n, int32_t* perm) {
+  if (axes == nullptr) return true;
+  uint64_t seen = 0;
+  for (int k = 0; k < n; ++k) {
+    const int a = axes[k];
+    if (a < 0 || a >= n || ((seen >> a) & 1ull)) return false;
+    seen |= 1ull << a;
+    perm[n - 1 - k] = n - 1 - a;  // sample bit (n-1-k) <- qubit axes[k] <- flat bit n-1-a
+  }
+  return true;
+}
+
+int cqs_sample_prepare(const void* amps, const float* probs_in, int32_t n_qubits, int32_t is_dm,
+                       const int32_t* qubit_axes, void* workspace, void* stream) {
+  CQS_REQUIRE(n_qubits >= 0 && n_qubits <= 40, "n_qubits=%d out of range", n_qubits);
+  int32_t perm[64];
+  CQS_REQUIRE(axes_to_perm(qubit_axes, n_qubits, perm), "qubit_axes is not a permutation of 0..%d", n_qubits - 1);
+  CQS_REQUIRE(amps != nullptr || probs_in != nullptr, "no probability source");
+  CQS_REQUIRE(workspace != nullptr, "workspace is NULL");
+  const Layout l = layout_for(n_qubits);
+  char* ws = (char*)workspace;
+  SampleHdr* hdr = (SampleHdr*)(ws + l.o_hdr);
+  float* csum32 = (float*)(ws + l.o_csum32);
+  double* capx = (double*)(ws + l.o_capx);
+  double* cin = (double*)(ws + l.o_cin);
+  ChunkFn* cfn = (ChunkFn*)(ws + l.o_cfn);
+  double* scsum = (double*)(ws + l.o_scsum);
+  double* scpre = (double*)(ws + l.o_scpre);
+  ChunkFn* scfn = (ChunkFn*)(ws + l.o_scfn);
+  double* scstart = (double*)(ws + l.o_scstart);
+  int* scmode = (int*)(ws + l.o_scmode);
+  double* cstart = (double*)(ws + l.o_cstart);
+  const Loader ld = make_loader(amps, probs_in, n_qubits, is_dm, qubit_axes ? perm : nullptr);
+  hipStream_t s = as_stream(stream);
+  const uint64_t buflen = l.N / l.NB;
+  hipLaunchKernelGGL(k_sum_buffers, dim3((unsigned)l.NB), dim3(256), 0, s, ld, buflen, csum32);
+  hipLaunchKernelGGL(k_total32, dim3(1), dim3(64), 0, s, csum32, l.NB, hdr);
+  const unsigned gch = (unsigned)((l.NC + kChunksPerWG - 1) / kChunksPerWG);
+  hipLaunchKernelGGL(k_chunk_approx, dim3(gch), dim3(256), 0, s, ld, (const SampleHdr*)hdr, l.NC, l.Cl, capx);
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)l.NSC), dim3(256), 0, s, (const double*)capx, l.SCl, cin,
+                     scsum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(64), 0, s, (const double*)scsum, l.NSC, scpre);
+  hipLaunchKernelGGL(k_chunk_fn, dim3(gch), dim3(256), 0, s, ld, (const SampleHdr*)hdr, l.NC, l.Cl, l.SCl,
+                     (const double*)capx, (const double*)cin, (const double*)scpre, cfn);
+  hipLaunchKernelGGL(k_super_fn, dim3((unsigned)l.NSC), dim3(256), 0, s, (const ChunkFn*)cfn, l.SCl, scfn);
+  hipLaunchKernelGGL(k_scan_serial, dim3(1), dim3(64), 0, s, ld, hdr, l.NSC, l.SCl, l.Cl,
+                     (const ChunkFn*)cfn, (const ChunkFn*)scfn, scstart, scmode, cstart);
+  hipLaunchKernelGGL(k_chunk_starts, dim3((unsigned)l.NSC), dim3(256), 0, s, (const ChunkFn*)cfn, l.SCl,
+                     (const double*)scstart, (const int*)scmode, (const ChunkFn*)scfn, cstart);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+int cqs_sample_draw(const void* amps, const float* probs_in, int32_t n_qubits, int32_t is_dm,
+                    const int32_t* qubit_axes, const double* uniforms, int64_t reps, int64_t* out_index,
+                    void* workspace, void* stream) {
+  CQS_REQUIRE(n_qubits >= 0 && n_qubits <= 40, "n_qubits=%d out of range", n_qubits);
+  int32_t perm[64];
+  CQS_REQUIRE(axes_to_perm(qubit_axes, n_qubits, perm), "qubit_axes is not a permutation of 0..%d", n_qubits - 1);
+  CQS_REQUIRE(amps != nullptr || probs_in != nullptr, "no probability source");
+  CQS_REQUIRE(workspace != nullptr, "workspace is NULL");
+  CQS_REQUIRE(reps >= 0, "reps < 0");
+  if (reps == 0) return CQS_OK;
+  CQS_REQUIRE(uniforms != nullptr && out_index != nullptr, "uniforms/out are NULL");
+  const Layout l = layout_for(n_qubits);
+  char* ws = (char*)workspace;
+  const Loader ld = make_loader(amps, probs_in, n_qubits, is_dm, qubit_axes ? perm : nullptr);
+  const unsigned g = (unsigned)((reps + 255) / 256);
+  hipLaunchKernelGGL(k_draw, dim3(g), dim3(256), 0, as_stream(stream), ld, (const SampleHdr*)(ws + l.o_hdr),
+                     l.NC, l.Cl, (const double*)(ws + l.o_cstart), uniforms, reps, out_index);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+int cqs_indices_to_bits(const int64_t* index, int64_t reps, int32_t m, uint8_t* out, void* stream) {
+  CQS_REQUIRE(reps >= 0 && m >= 0 && m <= 63, "bad reps/m");
+  if (reps == 0 || m == 0) return CQS_OK;
+  CQS_REQUIRE(index != nullptr && out != nullptr, "NULL pointer");
+  const int64_t total = reps * m;
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_bits, dim3((unsigned)g), dim3(256), 0, as_stream(stream), index, reps, m, out);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+int cqs_marginal_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const int32_t* qubit_axes,
+                       int32_t m, float* out, void* stream) {
+  CQS_REQUIRE(amps != nullptr && out != nullptr && qubit_axes != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_qubits >= 1 && n_qubits <= 40 && m >= 1 && m <= n_qubits && m <= 31, "bad sizes");
+  // The device needs the flat bit of every axis: measured first, then the unmeasured ones.
+  // qubit_axes is a HOST array of m Cirq axes.
+  AxisBits bits;
+  int used[64] = {0};
+  for (int k = 0; k < m; ++k) {
+    const int a = qubit_axes[k];
+    CQS_REQUIRE(a >= 0 && a < n_qubits && !used[a], "bad or repeated axis %d", a);
+    used[a] = 1;
+    bits.b[k] = n_qubits - 1 - a;
+  }
+  int w = m;
+  for (int a = 0; a < n_qubits; ++a)
+    if (!used[a]) bits.b[w++] = n_qubits - 1 - a;
+  const Loader ld = make_loader(amps, nullptr, n_qubits, is_dm);
+  const uint64_t rows = 1ull << m;
+  if (n_qubits - m <= 7) {
+    hipLaunchKernelGGL(k_marginal_short, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, as_stream(stream), ld,
+                       (int)n_qubits, bits, (int)m, rows, out);
+  } else {
+    CQS_REQUIRE(m <= 24, "marginals over more than 2^24 outcomes with long rows are not supported");
+    hipLaunchKernelGGL(k_marginal, dim3((unsigned)rows), dim3(256), 0, as_stream(stream), ld, (int)n_qubits, bits,
+                       (int)m, out);
+  }
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+"""Device buffers and C-ABI calls for one state (state vector or density matrix) on one GPU.
+
+All work is enqueued on torch's current HIP stream; only the methods that return NumPy data
+synchronise.  There is no CPU fallback: without the built library or a GPU these functions raise.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr
+from .tape import PackedProgram
+
+
+def require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("cirq_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    lib()  # raises if the HIP library is not built
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class DeviceProgram:
+    """A packed pass program resident on the device; parameter matrices can be swapped."""
+
+    def __init__(self, packed: PackedProgram, device):
+        self.packed = packed
+        self.n_bits = packed.n_bits
+        self.device = device
+        self.ops = torch.from_numpy(packed.ops_arr.view(np.int32).copy()).to(device)
+        self.mats = torch.from_numpy(packed.mats_arr.view(np.float32).copy()).to(device)
+        self.aux = torch.from_numpy(packed.aux_arr.copy()).to(device)
+        self.pass_arr = np.ascontiguousarray(packed.pass_arr)
+
+    def set_mats(self, mats_c64: np.ndarray):
+        src = torch.from_numpy(np.ascontiguousarray(mats_c64, dtype=np.complex64).view(np.float32))
+        self.mats.copy_(src, non_blocking=False)
+
+    def set_mats_device(self, mats_f32: torch.Tensor):
+        self.mats = mats_f32
+
+    def run(self, state: torch.Tensor):
+        if self.packed.n_passes == 0:
+            return
+        check(
+            lib().cqs_run_passes(
+                ptr(state),
+                self.n_bits,
+                self.pass_arr.ctypes.data,
+                int(self.packed.n_passes),
+                ptr(self.ops),
+                ptr(self.mats),
+                ptr(self.aux),
+                stream_handle(),
+            ),
+            "cqs_run_passes",
+        )
+
+
+def new_state(n_bits: int, device, basis: int = 0) -> torch.Tensor:
+    try:
+        st = torch.empty(1 << n_bits, dtype=torch.complex64, device=device)
+    except torch.OutOfMemoryError as e:  # the reference raises MemoryError
+        raise MemoryError(f"cannot allocate a state of 2^{n_bits} complex64 amplitudes") from e
+    check(lib().cqs_init_basis(ptr(st), n_bits, int(basis), stream_handle()), "cqs_init_basis")
+    return st
+
+
+class Workspace:
+    """Grow-only device scratch shared by reductions and sampling."""
+
+    def __init__(self, device):
+        self.device = device
+        self._buf = None
+
+    def get(self, nbytes: int) -> torch.Tensor:
+        nbytes = max(256, int(nbytes))
+        if self._buf is None or self._buf.numel() < nbytes:
+            self._buf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        return self._buf
+
+
+def norm2(state: torch.Tensor, n_bits: int, ws: Workspace) -> float:
+    out = torch.zeros(2, dtype=torch.float64, device=state.device)
+    w = ws.get(lib().cqs_expect_workspace_bytes(1))
+    check(lib().cqs_norm2(ptr(state), n_bits, ptr(out), ptr(w), stream_handle()), "cqs_norm2")
+    return float(out[0].item())
+
+
+def scale(state: torch.Tensor, n_bits: int, c: complex):
+    check(lib().cqs_scale(ptr(state), n_bits, float(c.real), float(c.imag), stream_handle()), "cqs_scale")
+
+
+def expect_terms(state: torch.Tensor, n_qubits: int, is_dm: bool, terms, ws: Workspace) -> torch.Tensor:
+    """terms: list of (xmask, zmask, complex coef incl. i^nY).  Returns device complex128 [T]."""
+    T = len(terms)
+    out = torch.zeros(2 * max(T, 1), dtype=torch.float64, device=state.device)
+    if T == 0:
+        return out.view(torch.complex128)[:0]
+    xm = np.array([t[0] for t in terms], dtype=np.uint64)
+    zm = np.array([t[1] for t in terms], dtype=np.uint64)
+    cf = np.array([[complex(t[2]).real, complex(t[2]).imag] for t in terms], dtype=np.float64).reshape(-1)
+    w = ws.get(lib().cqs_expect_workspace_bytes(T))
+    check(
+        lib().cqs_expect_paulis(
+            ptr(state), n_qubits, int(is_dm), T, xm.ctypes.data, zm.ctypes.data, cf.ctypes.data, ptr(out), ptr(w),
+            stream_handle(),
+        ),
+        "cqs_expect_paulis",
+    )
+    return out.view(torch.complex128)
+
+
+def apply_paulisum(psi: torch.Tensor, phi: torch.Tensor, n_qubits: int, terms):
+    T = len(terms)
+    xm = np.array([t[0] for t in terms] or [0], dtype=np.uint64)
+    zm = np.array([t[1] for t in terms] or [0], dtype=np.uint64)
+    cf = np.array([[complex(t[2]).real, complex(t[2]).imag] for t in terms] or [[0, 0]], dtype=np.float64)
+    cf = cf.reshape(-1)
+    check(
+        lib().cqs_apply_paulisum(
+            ptr(psi), ptr(phi), n_qubits, T, xm.ctypes.data, zm.ctypes.data, cf.ctypes.data, stream_handle()
+        ),
+        "cqs_apply_paulisum",
+    )
+
+
+def collapse(state: torch.Tensor, n_bits: int, mask: int, value: int, s: float):
+    check(lib().cqs_collapse(ptr(state), n_bits, int(mask), int(value), float(s), 0.0, stream_handle()), "cqs_collapse")
+
+
+class Sampler:
+    """Bit-exact NumPy-choice sampling of (marginal) probabilities of a state."""
+
+    def __init__(self, device):
+        self.device = device
+        self.ws = Workspace(device)
+
+    def marginal(self, state, n_qubits: int, is_dm: bool, axes) -> torch.Tensor:
+        m = len(axes)
+        out = torch.empty(1 << m, dtype=torch.float32, device=state.device)
+        ax = np.array(axes, dtype=np.int32)
+        check(
+            lib().cqs_marginal_probs(ptr(state), n_qubits, int(is_dm), ax.ctypes.data, m, ptr(out), stream_handle()),
+            "cqs_marginal_probs",
+        )
+        return out
+
+    def draw(self, state, n_qubits: int, is_dm: bool, uniforms: np.ndarray, probs=None, axes=None) -> torch.Tensor:
+        """Indices (device int64) of numpy's choice(2^m, p) for the given uniforms.
+
+        probs: optional device float32 marginal (then n_qubits is its number of bits).
+        axes: optional order of all qubits (a permutation) for sampling every qubit."""
+        reps = int(uniforms.shape[0])
+        w = self.ws.get(lib().cqs_sample_workspace_bytes(n_qubits))
+        src = None if probs is not None else state
+        ax = None if axes is None else np.ascontiguousarray(axes, dtype=np.int32)
+        check(
+            lib().cqs_sample_prepare(ptr(src), ptr(probs), n_qubits, int(is_dm), ptr(ax), ptr(w), stream_handle()),
+            "cqs_sample_prepare",
+        )
+        u = torch.from_numpy(np.ascontiguousarray(uniforms, dtype=np.float64)).to(self.device)
+        idx = torch.empty(max(reps, 1), dtype=torch.int64, device=self.device)
+        check(
+            lib().cqs_sample_draw(
+                ptr(src), ptr(probs), n_qubits, int(is_dm), ptr(ax), ptr(u), reps, ptr(idx), ptr(w), stream_handle()
+            ),
+            "cqs_sample_draw",
+        )
+        return idx[:reps]
+
+    def bits(self, idx: torch.Tensor, m: int) -> np.ndarray:
+        reps = int(idx.shape[0])
+        out = torch.empty(max(reps * m, 1), dtype=torch.uint8, device=self.device)
+        check(lib().cqs_indices_to_bits(ptr(idx), reps, m, ptr(out), stream_handle()), "cqs_indices_to_bits")
+        return out[: reps * m].cpu().numpy().reshape(reps, m)

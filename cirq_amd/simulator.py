@@ -1,0 +1,645 @@
+"""Drop-in Simulator / DensityMatrixSimulator running on one MI355X.
+
+Interface and behaviour follow the reference's simulator ABCs:
+  SimulatesSamples.run / run_sweep        cirq-core/cirq/sim/simulator.py:62-116
+  SimulatesFinalState.simulate(_sweep)     cirq-core/cirq/sim/simulator.py:456-537
+  SimulatesExpectationValues              cirq-core/cirq/sim/simulator.py:291-440,
+                                          cirq-core/cirq/sim/sparse_simulator.py:193-218
+  SimulatesAmplitudes                     cirq-core/cirq/sim/simulator.py:188-216
+and SimulatorBase._run's split of a unitary prefix from terminal measurements
+(cirq-core/cirq/sim/simulator_base.py:216-273).  The state lives on the GPU for the whole
+simulation; circuits are compiled once into pass programs and re-used across resolvers.
+"""
+
+from __future__ import annotations
+
+import collections
+import math
+import warnings
+import weakref
+from typing import Any, Iterator, Sequence
+
+import numpy as np
+import torch
+
+from . import engine
+from .channels import NoiseModel
+from .circuits import Circuit, Moment
+from .gates import is_parameterized
+from .paulis import PauliSum, pauli_masks
+from .qubits import QubitOrder
+from .study import ParamResolver, ResultDict, to_resolvers
+from .tape import (
+    Lowering,
+    Measurement,
+    ParamEvaluator,
+    UnsupportedOperation,
+    compile_ops,
+    fill_param_mats,
+)
+
+
+def parse_random_state(seed) -> np.random.RandomState:
+    """cirq.value.parse_random_state (cirq-core/cirq/value/random_state.py:42)."""
+    if seed is None:
+        return np.random.mtrand._rand
+    if isinstance(seed, np.random.RandomState):
+        return seed
+    if isinstance(seed, (int, np.integer)):
+        return np.random.RandomState(int(seed))
+    raise TypeError(f"Argument must be of type cirq.value.RANDOM_STATE_OR_SEED_LIKE, got {type(seed)}.")
+
+
+def _is_measurement_op(op) -> bool:
+    g = getattr(op, "gate", None)
+    return g is not None and type(g).__name__ == "MeasurementGate"
+
+
+def _split_prefix(circuit, predicate):
+    """split_into_matching_protocol_then_general (cirq-core/cirq/sim/simulator.py:952-983)."""
+    blocked_q: set = set()
+    blocked_k: set = set()
+    prefix, suffix = Circuit(), Circuit()
+    for moment in circuit:
+        mp, gp = [], []
+        for op in moment:
+            qs = set(op.qubits)
+            keys = {op.gate.key} if _is_measurement_op(op) else set()
+            if predicate(op) and qs.isdisjoint(blocked_q) and keys.isdisjoint(blocked_k):
+                mp.append(op)
+            else:
+                blocked_q |= qs
+                blocked_k |= keys
+                gp.append(op)
+        if mp:
+            prefix._moments.append(Moment(mp))
+        if gp:
+            suffix._moments.append(Moment(gp))
+    return prefix, suffix
+
+
+class _Factor:
+    __slots__ = ("qubits",)
+
+    def __init__(self, qubits):
+        self.qubits = list(qubits)
+
+
+def product_state_factors(circuit, qubits) -> list[list]:
+    """Factors of Cirq's SimulationProductState after `circuit`, in the order Cirq samples them.
+
+    Follows cirq-core/cirq/sim/simulation_product_state.py:83-130 (merge by Kronecker product in
+    op-qubit order, identity gates skipped, whole SWAPs rename) and the initial map built over
+    reversed(qubits) (cirq-core/cirq/sim/simulator_base.py:330-340); sampling walks unique
+    factors in that map's order (cirq-core/cirq/sim/simulation_product_state.py:154-172).
+    """
+    states: dict = {}
+    for q in reversed(qubits):
+        states[q] = _Factor([q])
+    for op in circuit.all_operations():
+        g = getattr(op, "gate", None)
+        name = type(g).__name__ if g is not None else ""
+        if name == "IdentityGate":
+            continue
+        if name == "SwapPowGate" and not is_parameterized(g) and g.exponent % 2 == 1 and g.global_shift == 0:
+            q0, q1 = op.qubits
+            a0, a1 = states[q0], states[q1]
+            if a0 is not a1:
+                a1.qubits = [q0 if q == q1 else q for q in a1.qubits]
+                a0.qubits = [q1 if q == q0 else q for q in a0.qubits]
+                states[q0], states[q1] = a1, a0
+            continue
+        merged = None
+        for q in op.qubits:
+            if merged is None:
+                merged = states[q]
+            elif q not in merged.qubits:
+                merged.qubits.extend(states[q].qubits)
+        if merged is None:
+            continue
+        for q in merged.qubits:
+            states[q] = merged
+    out = []
+    seen = set()
+    for v in states.values():
+        if id(v) not in seen:
+            seen.add(id(v))
+            out.append(list(v.qubits))
+    return out
+
+
+class _Compiled:
+    """A circuit lowered, scheduled and uploaded for one qubit order."""
+
+    def __init__(self, circuit, qubits, dm: bool, device, L: int, coalesce: int, fuse: bool):
+        self.qubits = tuple(qubits)
+        self.n = len(self.qubits)
+        self.dm = dm
+        self.n_bits = 2 * self.n if dm else self.n
+        low = Lowering(self.qubits, dm)
+        low.lower_circuit(circuit)
+        self.lowering = low
+        self.measurements: list[Measurement] = low.measurements
+        self.packed = compile_ops(low.ops, self.n_bits, L=L, coalesce=coalesce, fuse=fuse)
+        self.evaluator = ParamEvaluator(low.param_refs)
+        self.has_params = bool(self.packed.param_entries)
+        self.dev = engine.DeviceProgram(self.packed, device)
+
+    def mats_for(self, resolvers) -> np.ndarray:
+        symvals = self.evaluator.symbol_values(resolvers)
+        return fill_param_mats(self.packed, self.evaluator, symvals)
+
+
+class _SimulatorBase:
+    _dm = False
+
+    def __init__(
+        self,
+        *,
+        dtype=np.complex64,
+        noise=None,
+        seed=None,
+        split_untangled_states: bool = True,
+        tile_bits: int | None = None,
+        coalesce_bits: int | None = None,
+        fuse: bool = True,
+    ):
+        if np.dtype(dtype).kind != "c":
+            raise ValueError(f"dtype must be a complex type but was {dtype}")
+        if np.dtype(dtype) != np.complex64:
+            raise ValueError("this simulator computes in complex64; dtype must be np.complex64")
+        self._dtype = np.complex64
+        self.noise = NoiseModel.from_noise_model_like(noise)
+        self._prng = parse_random_state(seed)
+        self._split = bool(split_untangled_states)
+        self._tile_bits = tile_bits
+        self._coalesce = coalesce_bits
+        self._fuse = fuse
+        self._cache: "collections.OrderedDict" = collections.OrderedDict()
+        self._device = None
+        self._sampler = None
+        self._ws = None
+
+    # ---------------------------------------------------------------- plumbing
+    def _dev(self):
+        if self._device is None:
+            self._device = engine.require_gpu()
+            self._sampler = engine.Sampler(self._device)
+            self._ws = engine.Workspace(self._device)
+        return self._device
+
+    def _noisy(self, circuit) -> Circuit:
+        if self.noise is None or type(self.noise).__name__ == "_NoNoise":
+            return circuit
+        qubits = sorted(circuit.all_qubits())
+        return Circuit.from_moments(*self.noise.noisy_moments(list(circuit), qubits))
+
+    def _tile_params(self, n_bits: int) -> tuple[int, int]:
+        if self._tile_bits is not None:
+            L = self._tile_bits
+        else:
+            L = 12 if n_bits >= 20 else (10 if n_bits >= 14 else n_bits)
+        c = self._coalesce if self._coalesce is not None else 4
+        return min(L, n_bits), min(c, L, n_bits)
+
+    def _compile(self, circuit, qubits) -> _Compiled:
+        key = (id(circuit), tuple(qubits))
+        hit = self._cache.get(key)
+        if hit is not None and hit[0]() is circuit:
+            self._cache.move_to_end(key)
+            return hit[1]
+        nb = 2 * len(qubits) if self._dm else len(qubits)
+        L, c = self._tile_params(nb)
+        comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse)
+        try:
+            ref = weakref.ref(circuit)
+        except TypeError:
+            ref = lambda c=circuit: c
+        self._cache[key] = (ref, comp)
+        while len(self._cache) > 16:
+            self._cache.popitem(last=False)
+        return comp
+
+    def _initial_index(self, initial_state, n: int) -> int:
+        if initial_state is None:
+            return 0
+        if isinstance(initial_state, (int, np.integer)):
+            v = int(initial_state)
+            if v < 0 or v >= (1 << n):
+                raise ValueError(f"initial_state {v} out of range for {n} qubits")
+            return v
+        raise TypeError("initial_state must be an int or a state array")
+
+    def _prepare(self, comp: _Compiled, initial_state) -> torch.Tensor:
+        dev = self._dev()
+        n = comp.n
+        if initial_state is not None and not isinstance(initial_state, (int, np.integer)):
+            arr = np.asarray(initial_state)
+            if self._dm:
+                if arr.shape == (2**n,):
+                    v = arr.astype(np.complex128)
+                    arr = np.outer(v, np.conj(v))
+                arr = arr.reshape(2**n, 2**n)
+            else:
+                arr = arr.reshape(-1)
+                if arr.size != 2**n:
+                    raise ValueError(f"initial_state has {arr.size} amplitudes, expected {2**n}")
+            return torch.from_numpy(np.ascontiguousarray(arr.reshape(-1), dtype=np.complex64)).to(dev)
+        b = self._initial_index(initial_state, n)
+        if self._dm:
+            b = b * (2**n) + b
+        return engine.new_state(comp.n_bits, dev, b)
+
+    def _run_program(self, comp: _Compiled, resolvers: list, initial_state) -> Iterator[torch.Tensor]:
+        mats = comp.mats_for(resolvers) if comp.has_params else None
+        for i, _ in enumerate(resolvers):
+            if mats is not None:
+                comp.dev.set_mats(mats[i])
+            st = self._prepare(comp, initial_state)
+            comp.dev.run(st)
+            yield st
+
+    @staticmethod
+    def _check_resolved(resolvers, comp: _Compiled):
+        names = {s.name for s in comp.evaluator.symbols}
+        for r in resolvers:
+            for s in comp.evaluator.symbols:
+                v = r.value_of(s)
+                if not isinstance(v, (int, float, complex, np.number)):
+                    raise ValueError(f"Circuit contains ops whose symbols were not specified in parameter sweep: {names}")
+
+    # ---------------------------------------------------------------- sampling
+    def _sample_factors(self, st, comp: _Compiled, factors, measured: list, reps: int) -> np.ndarray:
+        """Samples `measured` qubits factor by factor with the simulator's prng; columns in measured order."""
+        mset = set(measured)
+        columns, order = [], []
+        idx = {q: i for i, q in enumerate(comp.qubits)}
+        for fq in factors:
+            qs = [q for q in fq if q in mset]
+            if not qs:
+                continue
+            u = self._prng.random_sample(reps)
+            axes = [idx[q] for q in qs]
+            if axes == list(range(comp.n)):
+                ind = self._sampler.draw(st, comp.n, self._dm, u)
+            elif len(axes) == comp.n:
+                ind = self._sampler.draw(st, comp.n, self._dm, u, axes=axes)
+            else:
+                probs = self._sampler.marginal(st, comp.n, self._dm, axes)
+                ind = self._sampler.draw(None, len(axes), False, u, probs=probs)
+            columns.append(self._sampler.bits(ind, len(axes)))
+            order += qs
+        stacked = np.column_stack(columns) if columns else np.zeros((reps, 0), dtype=np.uint8)
+        pos = {q: i for i, q in enumerate(order)}
+        return stacked[:, [pos[q] for q in measured]]
+
+    def _records(self, bits: np.ndarray, measured: list, meas_ops) -> dict:
+        qi = {q: i for i, q in enumerate(measured)}
+        results: dict = {}
+        for op in meas_ops:
+            g = op.gate
+            out = np.zeros((bits.shape[0], len(op.qubits)), dtype=np.uint8)
+            inv = g.full_invert_mask() if hasattr(g, "full_invert_mask") else ()
+            for i, q in enumerate(op.qubits):
+                out[:, i] = bits[:, qi[q]]
+                if i < len(inv) and inv[i]:
+                    out[:, i] ^= 1
+            results.setdefault(g.key, []).append(out)
+        return {k: np.stack(v, axis=1) for k, v in results.items()}
+
+    # ---------------------------------------------------------------- SimulatesSamples
+    def run(self, program, param_resolver=None, repetitions: int = 1) -> ResultDict:
+        return self.run_sweep(program, param_resolver, repetitions)[0]
+
+    def run_sweep(self, program, params, repetitions: int = 1) -> list[ResultDict]:
+        return list(self.run_sweep_iter(program, params, repetitions))
+
+    def sample(self, program, *, repetitions: int = 1, params=None):
+        import pandas as pd
+
+        rows = []
+        for r in self.run_sweep(program, params, repetitions):
+            d = {}
+            for k, v in r.measurements.items():
+                w = 1 << np.arange(v.shape[1] - 1, -1, -1, dtype=np.int64)
+                d[k] = (v.astype(np.int64) * w).sum(axis=1)
+            for k, v in r.params.param_dict.items():
+                d[str(k)] = [v] * repetitions
+            rows.append(pd.DataFrame(d))
+        return pd.concat(rows, ignore_index=True)
+
+    def _can_be_in_run_prefix(self, op) -> bool:
+        """Unitary (or still symbolic) ops for Simulator, any non-measurement for the DM simulator."""
+        if _is_measurement_op(op):
+            return False
+        if self._dm or is_parameterized(op):
+            return True
+        g = getattr(op, "gate", None)
+        obj = g if g is not None else op
+        f = getattr(obj, "_has_unitary_", None)
+        return bool(f()) if f is not None else False
+
+    def run_sweep_iter(self, program, params, repetitions: int = 1) -> Iterator[ResultDict]:
+        if not program.has_measurements():
+            raise ValueError("Circuit has no measurements to sample.")
+        if repetitions < 0:
+            raise ValueError(f"Number of repetitions cannot be negative. Was {repetitions}")
+        resolvers = list(to_resolvers(params))
+        circuit = self._noisy(program)
+        qubits = tuple(sorted(circuit.all_qubits()))
+        prefix, suffix = _split_prefix(circuit, self._can_be_in_run_prefix)
+        meas_ops = list(suffix.all_operations())
+        if not all(_is_measurement_op(op) for op in meas_ops):
+            raise NotImplementedError(
+                "mid-circuit measurements or noise after measurements need per-repetition simulation, "
+                "which this simulator does not implement"
+            )
+        if repetitions == 0:
+            for r in resolvers:
+                yield ResultDict(params=r, records={op.gate.key: np.empty([0, 1, 1]) for op in meas_ops})
+            return
+        comp = self._compile(prefix, qubits)
+        self._check_resolved(resolvers, comp)
+        measured: list = []
+        seen = set()
+        for op in meas_ops:
+            for q in op.qubits:
+                if q not in seen:
+                    seen.add(q)
+                    measured.append(q)
+        factors = product_state_factors(prefix, qubits) if self._split else [list(qubits)]
+        for r, st in zip(resolvers, self._run_program(comp, resolvers, None)):
+            bits = self._sample_factors(st, comp, factors, measured, repetitions)
+            yield ResultDict(params=r, records=self._records(bits, measured, meas_ops))
+
+    # ---------------------------------------------------------------- SimulatesFinalState
+    def simulate(self, program, param_resolver=None, qubit_order=QubitOrder.DEFAULT, initial_state=None):
+        return self.simulate_sweep(program, param_resolver, qubit_order, initial_state)[0]
+
+    def simulate_sweep(self, program, params, qubit_order=QubitOrder.DEFAULT, initial_state=None) -> list:
+        return list(self.simulate_sweep_iter(program, params, qubit_order, initial_state))
+
+    def _measure_terminal(self, st, comp: _Compiled, meas: list) -> dict:
+        """Applies measurements at the end of a simulation (collapse + renormalise)."""
+        out = {}
+        idx = {q: i for i, q in enumerate(comp.qubits)}
+        for m in meas:
+            axes = [idx[q] for q in m.qubits]
+            probs = self._sampler.marginal(st, comp.n, self._dm, axes)
+            p = probs.cpu().numpy().astype(np.float32)
+            p = np.clip(p, 0, None)
+            p /= p.sum()
+            r = int(self._prng.choice(len(p), p=p))
+            bits = [(r >> (len(axes) - 1 - k)) & 1 for k in range(len(axes))]
+            mask = value = 0
+            for a, b in zip(axes, bits):
+                if self._dm:
+                    rb, cb = 2 * comp.n - 1 - a, comp.n - 1 - a
+                    mask |= (1 << rb) | (1 << cb)
+                    value |= (b << rb) | (b << cb)
+                else:
+                    bb = comp.n - 1 - a
+                    mask |= 1 << bb
+                    value |= b << bb
+            s = 1.0 / p[r] if self._dm else 1.0 / math.sqrt(p[r])
+            engine.collapse(st, comp.n_bits, mask, value, s)
+            inv = m.invert_mask
+            out.setdefault(m.key, []).append(
+                np.array([b ^ (1 if k < len(inv) and inv[k] else 0) for k, b in enumerate(bits)], dtype=np.uint8)
+            )
+        return {k: (v[0] if len(v) == 1 else np.stack(v)) for k, v in out.items()}
+
+    def simulate_sweep_iter(self, program, params, qubit_order=QubitOrder.DEFAULT, initial_state=None):
+        resolvers = list(to_resolvers(params))
+        circuit = self._noisy(program)
+        qubits = QubitOrder.as_qubit_order(qubit_order).order_for(circuit.all_qubits())
+        body, meas = _split_terminal_measurements(circuit)
+        comp = self._compile(body, qubits)
+        self._check_resolved(resolvers, comp)
+        for r, st in zip(resolvers, self._run_program(comp, resolvers, initial_state)):
+            measurements = self._measure_terminal(st, comp, meas) if meas else {}
+            yield self._trial_result(r, measurements, st, comp)
+
+    # ---------------------------------------------------------------- expectation values
+    def simulate_expectation_values(
+        self,
+        program,
+        observables,
+        param_resolver=None,
+        qubit_order=QubitOrder.DEFAULT,
+        initial_state=None,
+        permit_terminal_measurements: bool = False,
+    ) -> list:
+        return self.simulate_expectation_values_sweep(
+            program, observables, param_resolver, qubit_order, initial_state, permit_terminal_measurements
+        )[0]
+
+    def simulate_expectation_values_sweep(
+        self,
+        program,
+        observables,
+        params,
+        qubit_order=QubitOrder.DEFAULT,
+        initial_state=None,
+        permit_terminal_measurements: bool = False,
+    ) -> list[list]:
+        return list(
+            self.simulate_expectation_values_sweep_iter(
+                program, observables, params, qubit_order, initial_state, permit_terminal_measurements
+            )
+        )
+
+    def simulate_expectation_values_sweep_iter(
+        self,
+        program,
+        observables,
+        params,
+        qubit_order=QubitOrder.DEFAULT,
+        initial_state=None,
+        permit_terminal_measurements: bool = False,
+    ):
+        if not permit_terminal_measurements and program.has_measurements() and program.are_all_measurements_terminal():
+            raise ValueError(
+                "Provided circuit has terminal measurements, which may "
+                "skew expectation values. If this is intentional, set "
+                "permit_terminal_measurements=True."
+            )
+        if not isinstance(observables, list):
+            observables = [observables]
+        pslist = [PauliSum.wrap(o) for o in observables]
+        resolvers = list(to_resolvers(params))
+        circuit = self._noisy(program)
+        qubits = QubitOrder.as_qubit_order(qubit_order).order_for(circuit.all_qubits())
+        qmap = {q: i for i, q in enumerate(qubits)}
+        body, meas = _split_terminal_measurements(circuit)
+        comp = self._compile(body, qubits)
+        self._check_resolved(resolvers, comp)
+        terms, owner = [], []
+        for oi, ps in enumerate(pslist):
+            for t in ps:
+                for q in t.qubits:
+                    if q not in qmap:
+                        raise KeyError(q)
+                terms.append(pauli_masks(t, qmap, comp.n))
+                owner.append(oi)
+        owner = np.array(owner, dtype=np.int64)
+        for r, st in zip(resolvers, self._run_program(comp, resolvers, initial_state)):
+            if meas:
+                self._measure_terminal(st, comp, meas)
+            vals = engine.expect_terms(st, comp.n, self._dm, terms, self._ws).cpu().numpy()
+            sums = np.zeros(len(pslist), dtype=np.complex128)
+            np.add.at(sums, owner, vals)
+            yield [self._ev_value(v) for v in sums]
+
+    def _ev_value(self, v):
+        return v
+
+    # ---------------------------------------------------------------- amplitudes
+    def compute_amplitudes(self, program, bitstrings, param_resolver=None, qubit_order=QubitOrder.DEFAULT):
+        return self.compute_amplitudes_sweep(program, bitstrings, param_resolver, qubit_order)[0]
+
+    def compute_amplitudes_sweep(self, program, bitstrings, params, qubit_order=QubitOrder.DEFAULT) -> list:
+        if self._dm:
+            raise NotImplementedError("amplitudes are defined for state vectors only")
+        out = []
+        idx = torch.tensor([int(b) for b in bitstrings], dtype=torch.int64)
+        for res in self.simulate_sweep_iter(program, params, qubit_order):
+            amps = res._state[idx.to(res._state.device)].cpu().numpy()
+            out.append(list(amps))
+        return out
+
+
+def _split_terminal_measurements(circuit):
+    """(circuit without terminal measurements, terminal measurements in order)."""
+    meas: list[Measurement] = []
+    if not circuit.has_measurements():
+        return circuit, meas
+    if not circuit.are_all_measurements_terminal():
+        raise NotImplementedError("mid-circuit measurements are not supported by this simulator")
+    body = Circuit()
+    for m in circuit:
+        ops = []
+        for op in m:
+            if _is_measurement_op(op):
+                g = op.gate
+                meas.append(Measurement(g.key, tuple(op.qubits), g.full_invert_mask()))
+            else:
+                ops.append(op)
+        body._moments.append(Moment(ops))
+    return body, meas
+
+
+# --------------------------------------------------------------------------------------------
+class StateVectorTrialResult:
+    """Final state of a simulation (cirq-core/cirq/sim/state_vector_simulator.py:106-146)."""
+
+    def __init__(self, params, measurements, state: torch.Tensor, qubits, ws):
+        self.params = params
+        self.measurements = measurements
+        self._state = state
+        self._qubits = tuple(qubits)
+        self.qubit_map = {q: i for i, q in enumerate(self._qubits)}
+        self._ws = ws
+        self._fsv = None
+
+    @property
+    def final_state_vector(self) -> np.ndarray:
+        """Copied to the host; renormalised only when that improves the total probability."""
+        if self._fsv is None:
+            n = len(self._qubits)
+            n2 = engine.norm2(self._state, n, self._ws)
+            norm = math.sqrt(n2)
+            eps = math.sqrt(np.finfo(np.complex64).eps)
+            ret = self._state
+            if abs(norm - 1) > eps:
+                warnings.warn(f"final state vector's norm={norm} is too far from 1, {abs(norm - 1)} > {eps}.")
+            else:
+                t = ret.clone()
+                engine.scale(t, n, 1.0 / norm)
+                n2b = engine.norm2(t, n, self._ws)
+                if abs(n2b - 1) < abs(n2 - 1):
+                    ret = t
+            self._fsv = ret.cpu().numpy()
+        return self._fsv
+
+    def state_vector(self, copy: bool = False) -> np.ndarray:
+        v = self.final_state_vector
+        return v.copy() if copy else v
+
+    def dirac_notation(self, decimals: int = 2) -> str:
+        v = self.final_state_vector
+        n = len(self._qubits)
+        parts = []
+        for i in np.flatnonzero(np.abs(v) > 10 ** (-decimals)):
+            parts.append(f"({np.round(v[i], decimals)})|{i:0{n}b}⟩")
+        return " + ".join(parts)
+
+    def __repr__(self):
+        return f"StateVectorTrialResult(params={self.params!r}, measurements={self.measurements!r})"
+
+
+class DensityMatrixTrialResult:
+    """Final density matrix of a simulation (cirq-core/cirq/sim/density_matrix_simulator.py:368-374)."""
+
+    def __init__(self, params, measurements, state: torch.Tensor, qubits, ws):
+        self.params = params
+        self.measurements = measurements
+        self._state = state
+        self._qubits = tuple(qubits)
+        self.qubit_map = {q: i for i, q in enumerate(self._qubits)}
+        self._fdm = None
+
+    @property
+    def final_density_matrix(self) -> np.ndarray:
+        if self._fdm is None:
+            n = len(self._qubits)
+            self._fdm = self._state.cpu().numpy().reshape(2**n, 2**n)
+        return self._fdm
+
+    def __repr__(self):
+        return f"DensityMatrixTrialResult(params={self.params!r}, measurements={self.measurements!r})"
+
+
+class Simulator(_SimulatorBase):
+    """State-vector simulator (cirq.Simulator drop-in, cirq-core/cirq/sim/sparse_simulator.py:31-218)."""
+
+    _dm = False
+
+    def _trial_result(self, params, measurements, st, comp):
+        return StateVectorTrialResult(params, measurements, st, comp.qubits, self._ws)
+
+
+class DensityMatrixSimulator(_SimulatorBase):
+    """Density-matrix simulator (cirq.DensityMatrixSimulator drop-in, cirq-core/cirq/sim/density_matrix_simulator.py:32-235)."""
+
+    _dm = True
+
+    def _trial_result(self, params, measurements, st, comp):
+        return DensityMatrixTrialResult(params, measurements, st, comp.qubits, self._ws)
+
+    def _ev_value(self, v):
+        # Tr(rho P) is returned as a float (cirq-core/cirq/ops/pauli_string.py:764)
+        return float(np.real(v))
+
+
+# ---- convenience functions (cirq-core/cirq/sim/mux.py) -----------------------------------------
+def final_state_vector(program, *, initial_state=0, param_resolver=None, qubit_order=QubitOrder.DEFAULT,
+                       ignore_terminal_measurements: bool = False, dtype=np.complex64, seed=None) -> np.ndarray:
+    if not ignore_terminal_measurements and program.has_measurements():
+        raise ValueError("Circuit contains measurements; set ignore_terminal_measurements=True")
+    if ignore_terminal_measurements:
+        program, _ = _split_terminal_measurements(program)
+    return Simulator(dtype=dtype, seed=seed).simulate(program, param_resolver, qubit_order, initial_state).final_state_vector
+
+
+def final_density_matrix(program, *, noise=None, initial_state=0, param_resolver=None,
+                         qubit_order=QubitOrder.DEFAULT, dtype=np.complex64, seed=None,
+                         ignore_measurement_results: bool = True) -> np.ndarray:
+    return DensityMatrixSimulator(dtype=dtype, noise=noise, seed=seed).simulate(
+        program, param_resolver, qubit_order, initial_state
+    ).final_density_matrix
+
+
+def sample(program, *, noise=None, param_resolver=None, repetitions: int = 1, dtype=np.complex64, seed=None):
+    sim = (DensityMatrixSimulator if noise is not None else Simulator)(dtype=dtype, noise=noise, seed=seed)
+    return sim.run(program, param_resolver, repetitions)

@@ -1,0 +1,588 @@
+"""Circuit -> op tape -> passes: the host-side compiler of the engine.
+
+Lowering replaces the reference's per-operation dispatch (cirq-core/cirq/sim/simulator_base.py:
+198-212 -> protocols.act_on -> apply_unitary / apply_channel).  Instead of one NumPy pass per
+operation, operations become bit-level ops (dense matrices on flat-index bits, diagonals, pair
+phases, global scales) that the scheduler packs into passes: each pass reads and writes the state
+once and applies every op whose non-diagonal targets fit in that pass's tile of L local bits.
+
+Bit conventions: a state vector over qubit_order has axis i at flat bit n-1-i; a density matrix
+has row axis i at bit 2n-1-i and column axis i at bit n-1-i.  A Cirq matrix on qubits
+(q_0..q_{k-1}) is big-endian, so its kernel target list is (bit(q_{k-1}), ..., bit(q_0)).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from typing import Callable, Sequence
+
+import numpy as np
+import sympy
+
+from . import _lib
+from .gates import is_parameterized
+
+MAX_DENSE = 5
+MAX_DIAG = 6
+
+
+@dataclasses.dataclass
+class ParamRef:
+    """How to compute one lowered op's matrix from resolved parameter values.
+
+    gate: the (unresolved) gate; transform: maps the gate's (B, d, d) matrix batch to this op's
+    payload batch; exprs: the gate's parameter expressions; slot: first gradient slot.
+    """
+
+    gate: object
+    transform: Callable
+    exprs: tuple
+    slot: int = -1
+
+
+@dataclasses.dataclass
+class LOp:
+    kind: str  # 'dense' | 'diag' | 'pair' | 'scale'
+    targets: tuple  # flat bit positions; matrix index bit m <-> targets[m]
+    mat: np.ndarray | None  # dense: (d, d); diag: (d,); pair: scalar phase; scale: scalar
+    controls: tuple = ()  # ((bit, value), ...) for dense ops
+    pref: ParamRef | None = None
+
+    @property
+    def bits(self) -> frozenset:
+        return frozenset(self.targets) | frozenset(b for b, _ in self.controls)
+
+    @property
+    def fixed(self) -> bool:
+        return self.pref is None
+
+
+@dataclasses.dataclass
+class Measurement:
+    key: str
+    qubits: tuple
+    invert_mask: tuple
+
+
+class UnsupportedOperation(TypeError):
+    pass
+
+
+# --------------------------------------------------------------------------------------------
+def _gate_of(op):
+    return getattr(op, "gate", None)
+
+
+def _is_measurement(op) -> bool:
+    g = _gate_of(op)
+    return g is not None and type(g).__name__ == "MeasurementGate"
+
+
+def _is_pauli_string(op) -> bool:
+    return type(op).__name__ in ("PauliString", "SingleQubitPauliStringGateOperation") and hasattr(op, "items")
+
+
+def _has_channel(op) -> bool:
+    g = _gate_of(op)
+    obj = g if g is not None else op
+    if getattr(obj, "_has_unitary_", lambda: False)() or is_parameterized(obj):
+        return False
+    return hasattr(obj, "_kraus_") or hasattr(obj, "_mixture_")
+
+
+def kraus_of(obj) -> list[np.ndarray]:
+    """Kraus operators: _kraus_, else sqrt(p) U of _mixture_ (cirq-core/cirq/protocols/kraus_protocol.py:138-230)."""
+    f = getattr(obj, "_kraus_", None)
+    if f is not None:
+        r = f()
+        if r is not NotImplemented and r is not None:
+            return [np.asarray(k, dtype=np.complex128) for k in r]
+    f = getattr(obj, "_mixture_", None)
+    if f is not None:
+        r = f()
+        if r is not NotImplemented and r is not None:
+            return [np.sqrt(p) * np.asarray(u, dtype=np.complex128) for p, u in r]
+    u = getattr(obj, "_unitary_", lambda: NotImplemented)()
+    if u is not NotImplemented and u is not None:
+        return [np.asarray(u, dtype=np.complex128)]
+    raise UnsupportedOperation(f"{obj!r} has no unitary, mixture or Kraus representation")
+
+
+def _unitary_batch_fn(gate):
+    """Returns (exprs, fn(vals)->(B,d,d)) for a gate, or raises UnsupportedOperation."""
+    exprs = tuple(gate._param_exprs()) if hasattr(gate, "_param_exprs") else ()
+    if hasattr(gate, "_matrix_batch"):
+        return exprs, gate._matrix_batch
+    u = getattr(gate, "_unitary_", lambda: NotImplemented)()
+    if u is NotImplemented or u is None:
+        raise UnsupportedOperation(f"{gate!r} has no unitary")
+    u = np.asarray(u, dtype=np.complex128)
+    return (), (lambda vals, u=u: u[None])
+
+
+class Lowering:
+    """Lowers a circuit into LOps for a state vector (dm=False) or density matrix (dm=True)."""
+
+    def __init__(self, qubits: Sequence, dm: bool):
+        self.qubits = tuple(qubits)
+        self.index = {q: i for i, q in enumerate(self.qubits)}
+        self.n = len(self.qubits)
+        self.dm = dm
+        self.ops: list[LOp] = []
+        self.measurements: list[Measurement] = []
+        self.param_refs: list[ParamRef] = []
+        self.n_slots = 0
+
+    # bit helpers
+    def row_bit(self, q) -> int:
+        i = self.index[q]
+        return (2 * self.n - 1 - i) if self.dm else (self.n - 1 - i)
+
+    def col_bit(self, q) -> int:
+        return self.n - 1 - self.index[q]
+
+    def lower_circuit(self, circuit, *, measurements_terminal_only=True):
+        moments = list(circuit)
+        for mi, moment in enumerate(moments):
+            for op in moment:
+                self.lower_op(op)
+
+    def lower_op(self, op):
+        if _is_measurement(op):
+            g = _gate_of(op)
+            self.measurements.append(
+                Measurement(g.key, tuple(op.qubits), tuple(getattr(g, "full_invert_mask", lambda: ())()))
+            )
+            return
+        if _is_pauli_string(op) and _gate_of(op) is None:
+            self._lower_pauli_string(op)
+            return
+        g = _gate_of(op)
+        if g is None:
+            raise UnsupportedOperation(f"cannot simulate {op!r}: not a gate operation")
+        if _has_channel(op):
+            if not self.dm:
+                raise UnsupportedOperation(
+                    f"{op!r} is a channel; use DensityMatrixSimulator (trajectory sampling of channels on "
+                    "state vectors is not supported)"
+                )
+            self._lower_channel(g, tuple(op.qubits))
+            return
+        self._lower_gate(g, tuple(op.qubits))
+
+    def _lower_pauli_string(self, ps):
+        from .paulis import PauliString
+
+        coef = ps.coefficient
+        if isinstance(coef, sympy.Basic):
+            raise UnsupportedOperation("PauliString operations with symbolic coefficients are not supported")
+        for q, p in ps.items():
+            self._lower_gate(p, (q,))
+        if complex(coef) != 1:
+            self._add_scale(complex(coef))
+
+    def _add_scale(self, c: complex):
+        if self.dm:
+            c = abs(c) ** 2
+        self.ops.append(LOp("scale", (), np.complex128(c)))
+
+    def _lower_gate(self, gate, qubits):
+        nq = len(qubits)
+        if nq == 0:
+            exprs, fn = _unitary_batch_fn(gate)
+            if any(is_parameterized(e) for e in exprs):
+                raise UnsupportedOperation("parameterised global phases are not supported")
+            self._add_scale(complex(fn(tuple(np.array([float(e)]) for e in exprs))[0, 0, 0]))
+            return
+        controls = ()
+        sub = gate
+        tq = qubits
+        if type(gate).__name__ == "ControlledGate":
+            nc = gate.num_controls
+            cv = tuple(gate.control_values)
+            controls = tuple(zip(qubits[:nc], cv))
+            sub = gate.sub_gate
+            tq = qubits[nc:]
+        exprs, fn = _unitary_batch_fn(sub)
+        param = any(is_parameterized(e) for e in exprs)
+        diagonal = bool(getattr(sub, "_diagonal", False))
+        if not param:
+            m = fn(tuple(np.array([float(e)]) for e in exprs))[0]
+            if not diagonal and np.count_nonzero(m - np.diag(np.diag(m))) == 0:
+                diagonal = True
+        else:
+            m = None
+        ref = None
+        if param:
+            ref = ParamRef(sub, lambda b: b, exprs, slot=self.n_slots)
+            self.n_slots += len(exprs)
+            self.param_refs.append(ref)
+        if diagonal and len(controls) + len(tq) <= (MAX_DIAG // 2 if self.dm else MAX_DIAG):
+            # a controlled diagonal is a diagonal on controls + targets
+            self._emit_diag(tq, controls, m, ref)
+            return
+        if len(tq) > MAX_DENSE:
+            raise UnsupportedOperation(f"{gate!r}: dense gates on more than {MAX_DENSE} qubits are not supported")
+        rows = tuple(self.row_bit(q) for q in reversed(tq))
+        rctl = tuple((self.row_bit(q), v) for q, v in controls)
+        self.ops.append(LOp("dense", rows, m, rctl, _sub_ref(ref, lambda b: b)))
+        if self.dm:
+            cols = tuple(self.col_bit(q) for q in reversed(tq))
+            cctl = tuple((self.col_bit(q), v) for q, v in controls)
+            self.ops.append(
+                LOp("dense", cols, None if m is None else np.conj(m), cctl, _sub_ref(ref, np.conj))
+            )
+
+    def _emit_diag(self, tq, controls, m, ref):
+        """Diagonal of (controls + targets); controls not satisfied give 1."""
+        cq = tuple(q for q, _ in controls)
+        cv = tuple(v for _, v in controls)
+        allq = cq + tuple(tq)
+        k = len(allq)
+        nt = len(tq)
+
+        def expand(d):  # (B, 2^nt) -> (B, 2^k) big-endian over allq
+            B = d.shape[0]
+            out = np.ones((B, 2**k), dtype=np.complex128)
+            ctl = 0
+            for v in cv:
+                ctl = ctl * 2 + v
+            out[:, ctl * 2**nt : (ctl + 1) * 2**nt] = d
+            return out
+
+        def to_diag(b):
+            return expand(np.diagonal(b, axis1=-2, axis2=-1))
+
+        d = None if m is None else to_diag(m[None])[0]
+        # pure CZPow-like phase on two qubits: diag(1, 1, 1, phi)
+        if self.dm:
+            rbits = tuple(self.row_bit(q) for q in reversed(allq))
+            cbits = tuple(self.col_bit(q) for q in reversed(allq))
+
+            def dm_diag(dd):  # (B, 2^k) -> (B, 4^k): index = r * 2^k + c over bits (cbits, rbits)
+                return (dd[:, :, None] * np.conj(dd[:, None, :])).reshape(dd.shape[0], -1)
+
+            payload = None if d is None else dm_diag(d[None])[0]
+            self.ops.append(
+                LOp("diag", cbits + rbits, payload, (), _sub_ref(ref, lambda b: dm_diag(to_diag(b))))
+            )
+            return
+        bits = tuple(self.row_bit(q) for q in reversed(allq))
+        if d is not None and k == 2 and np.allclose(d[:3], 1, atol=0, rtol=0):
+            self.ops.append(LOp("pair", bits, np.complex128(d[3])))
+            return
+        self.ops.append(LOp("diag", bits, d, (), _sub_ref(ref, to_diag)))
+
+    def _lower_channel(self, g, qubits):
+        ks = kraus_of(g)
+        k = len(qubits)
+        if k > 2:
+            raise UnsupportedOperation(f"{g!r}: channels on more than 2 qubits are not supported")
+        sop = sum(np.kron(K, np.conj(K)) for K in ks)
+        cols = tuple(self.col_bit(q) for q in reversed(qubits))
+        rows = tuple(self.row_bit(q) for q in reversed(qubits))
+        self.ops.append(LOp("dense", cols + rows, sop))
+
+
+def _sub_ref(ref: ParamRef | None, transform):
+    if ref is None:
+        return None
+    base = ref.transform
+    return ParamRef(ref.gate, lambda b, base=base: transform(base(b)), ref.exprs, ref.slot)
+
+
+# --------------------------------------------------------------------------------------------
+def _as_matrix(op: LOp) -> np.ndarray:
+    if op.kind == "dense":
+        return op.mat
+    if op.kind == "diag":
+        return np.diag(op.mat)
+    raise ValueError(op.kind)
+
+
+def fuse_single_bit(ops: list[LOp]) -> list[LOp]:
+    """Multiplies runs of fixed one-bit ops (dense or diagonal) on the same bit into one op.
+
+    Exact matrix products in complex128, as merge_k_qubit_unitaries does
+    (cirq-core/cirq/transformers/merge_k_qubit_gates.py:55-63).
+    """
+    out: list[LOp] = []
+    pending: dict[int, int] = {}
+    for op in ops:
+        if op.fixed and op.kind in ("dense", "diag") and len(op.targets) == 1 and not op.controls:
+            b = op.targets[0]
+            if b in pending:
+                prev = out[pending[b]]
+                if prev.kind == "diag" and op.kind == "diag":
+                    prev.mat = op.mat * prev.mat
+                else:
+                    prev.mat = _as_matrix(op) @ _as_matrix(prev)
+                    prev.kind = "dense"
+                continue
+            out.append(LOp(op.kind, op.targets, np.array(op.mat), (), None))
+            pending[b] = len(out) - 1
+            continue
+        for b in op.bits:
+            pending.pop(b, None)
+        out.append(op)
+    return out
+
+
+# --------------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class Pass:
+    L: int
+    lpos: tuple  # ascending flat bit positions
+    ops: list
+
+
+def schedule(ops: list[LOp], n_bits: int, L: int, coalesce: int) -> list[Pass]:
+    """Greedy pass construction.
+
+    Each pass starts with the `coalesce` lowest bits local (so every tile row is a contiguous run
+    of 2^coalesce amplitudes) and walks the remaining ops in order: a dense op joins the pass if
+    its targets fit into the L local bits, diagonal ops always join; an op that cannot join blocks
+    its bits for the rest of the pass so that per-bit order is preserved (ops on disjoint bits
+    commute).
+    """
+    L = min(L, n_bits)
+    c = min(coalesce, L)
+    passes: list[Pass] = []
+    rest = list(ops)
+
+    def one_pass(forced: set):
+        S = set(forced)
+        blocked: set = set()
+        take, defer = [], []
+        for op in rest:
+            bits = op.bits
+            if bits & blocked:
+                defer.append(op)
+                blocked |= bits
+                continue
+            if op.kind == "dense":
+                need = set(op.targets) - S
+                if len(S) + len(need) <= L:
+                    S |= need
+                    take.append(op)
+                else:
+                    defer.append(op)
+                    blocked |= bits
+            else:
+                take.append(op)
+        return S, take, defer
+
+    while rest:
+        S, take, defer = one_pass(set(range(c)))
+        if not take:  # the first op is too wide to share the tile with the coalescing bits
+            S, take, defer = one_pass(set())
+        if not take:
+            raise UnsupportedOperation("scheduler made no progress (op wider than the tile?)")
+        # fill the tile with the lowest unused bits
+        b = 0
+        while len(S) < L:
+            if b not in S:
+                S.add(b)
+            b += 1
+        passes.append(Pass(L, tuple(sorted(S)), take))
+        rest = defer
+    return passes
+
+
+# --------------------------------------------------------------------------------------------
+class PackedProgram:
+    """Host/device arrays of a scheduled program in the C ABI layout."""
+
+    def __init__(self, passes: list[Pass], n_bits: int):
+        self.n_bits = n_bits
+        recs = []
+        mats: list[np.ndarray] = []
+        aux: list[int] = []
+        self.param_entries: list[tuple[int, ParamRef, int]] = []  # (mats offset, ref, size)
+        self.pass_arr = np.zeros(len(passes), dtype=_lib.PASS_DTYPE)
+        mat_off = 0
+
+        def add_mat(arr) -> int:
+            nonlocal mat_off
+            a = np.asarray(arr, dtype=np.complex64).reshape(-1)
+            mats.append(a)
+            o = mat_off
+            mat_off += a.size
+            return o
+
+        for pi, ps in enumerate(self.passes_merged(passes)):
+            local = {b: j for j, b in enumerate(ps.lpos)}
+            begin = len(recs)
+            max_k = 0
+            for op in ps.ops:
+                r = np.zeros((), dtype=_lib.OP_DTYPE)
+                r["slot"] = -1
+                if op.kind == "dense":
+                    k = len(op.targets)
+                    r["kind"] = _lib.OP_DENSE
+                    r["k"] = k
+                    max_k = max(max_k, k)
+                    for m, b in enumerate(op.targets):
+                        r["t"][m] = local[b]
+                    clm = clv = 0
+                    com = cov = 0
+                    for b, v in op.controls:
+                        if b in local:
+                            clm |= 1 << local[b]
+                            clv |= int(v) << local[b]
+                        else:
+                            com |= 1 << b
+                            cov |= int(v) << b
+                    r["cl_mask"], r["cl_val"] = clm, clv
+                    r["co_mask_lo"], r["co_mask_hi"] = com & 0xFFFFFFFF, com >> 32
+                    r["co_val_lo"], r["co_val_hi"] = cov & 0xFFFFFFFF, cov >> 32
+                    d = 2**k
+                    payload = op.mat if op.mat is not None else np.zeros((d, d))
+                    r["mat"] = add_mat(payload)
+                    if op.pref is not None:
+                        self.param_entries.append((int(r["mat"]), op.pref, d * d))
+                        r["slot"] = op.pref.slot
+                elif op.kind == "diag":
+                    k = len(op.targets)
+                    r["kind"] = _lib.OP_DIAG
+                    r["k"] = k
+                    for m, b in enumerate(op.targets):
+                        r["t"][m] = b
+                    payload = op.mat if op.mat is not None else np.zeros(2**k)
+                    r["mat"] = add_mat(payload)
+                    if op.pref is not None:
+                        self.param_entries.append((int(r["mat"]), op.pref, 2**k))
+                        r["slot"] = op.pref.slot
+                elif op.kind == "pairs":
+                    r["kind"] = _lib.OP_PAIRPHASE
+                    r["k"] = len(op.targets) // 2
+                    r["t"][0] = len(aux)
+                    aux.extend(int(b) for b in op.targets)
+                    r["mat"] = add_mat(op.mat)
+                elif op.kind == "scale":
+                    r["kind"] = _lib.OP_SCALE
+                    r["mat"] = add_mat(np.array([op.mat]))
+                else:
+                    raise ValueError(op.kind)
+                recs.append(r)
+            p = self.pass_arr[pi]
+            p["L"] = ps.L
+            p["op_begin"] = begin
+            p["op_count"] = len(recs) - begin
+            p["max_k"] = max_k
+            p["lpos"][: ps.L] = ps.lpos
+        self.ops_arr = np.array(recs, dtype=_lib.OP_DTYPE) if recs else np.zeros(1, dtype=_lib.OP_DTYPE)
+        self.mats_arr = np.concatenate(mats) if mats else np.zeros(1, dtype=np.complex64)
+        self.aux_arr = np.array(aux if aux else [0], dtype=np.int32)
+        self.n_passes = len(self.pass_arr)
+        self.n_ops = len(recs)
+
+    @staticmethod
+    def passes_merged(passes: list[Pass]) -> list[Pass]:
+        """Merges every run of elementwise ops' 'pair' phases into one PAIRPHASE op."""
+        out = []
+        for ps in passes:
+            ops: list[LOp] = []
+            run_pairs: list[LOp] = []
+
+            def flush():
+                if run_pairs:
+                    bits = tuple(b for p in run_pairs for b in p.targets)
+                    phases = np.array([p.mat for p in run_pairs], dtype=np.complex128)
+                    ops.append(LOp("pairs", bits, phases))
+                    run_pairs.clear()
+
+            for op in ps.ops:
+                if op.kind == "pair":
+                    run_pairs.append(op)
+                elif op.kind == "dense":
+                    flush()
+                    ops.append(op)
+                else:
+                    ops.append(op)  # elementwise ops commute with the pending pair phases
+            flush()
+            out.append(Pass(ps.L, ps.lpos, ops))
+        return out
+
+
+def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True) -> PackedProgram:
+    if fuse:
+        ops = fuse_single_bit(ops)
+    return PackedProgram(schedule(ops, n_bits, L, coalesce), n_bits)
+
+
+# --------------------------------------------------------------------------------------------
+class ParamEvaluator:
+    """Evaluates parameter expressions for batches of resolvers (vectorised with sympy.lambdify)."""
+
+    def __init__(self, refs: Sequence[ParamRef]):
+        self.symbols: list[sympy.Symbol] = []
+        seen = set()
+        for r in refs:
+            for e in r.exprs:
+                if isinstance(e, sympy.Basic):
+                    for s in sorted(e.free_symbols, key=lambda s: s.name):
+                        if s.name not in seen:
+                            seen.add(s.name)
+                            self.symbols.append(s)
+        self._fns: dict = {}
+        self._dfns: dict = {}
+
+    def _fn(self, e):
+        key = e
+        if key not in self._fns:
+            if isinstance(e, sympy.Basic) and e.free_symbols:
+                self._fns[key] = sympy.lambdify(self.symbols, e, "numpy")
+            else:
+                v = float(e)
+                self._fns[key] = lambda *a, v=v: v
+        return self._fns[key]
+
+    def _dfn(self, e, sym):
+        key = (e, sym)
+        if key not in self._dfns:
+            if isinstance(e, sympy.Basic) and sym in e.free_symbols:
+                self._dfns[key] = sympy.lambdify(self.symbols, sympy.diff(e, sym), "numpy")
+            else:
+                self._dfns[key] = None
+        return self._dfns[key]
+
+    def symbol_values(self, resolvers) -> np.ndarray:
+        """(n_symbols, B) float64 array of resolved symbol values."""
+        B = len(resolvers)
+        out = np.zeros((len(self.symbols), B))
+        for j, r in enumerate(resolvers):
+            for i, s in enumerate(self.symbols):
+                v = r.value_of(s)
+                if isinstance(v, sympy.Basic):
+                    raise ValueError(f"Circuit contains unresolved symbol {s.name!r}")
+                out[i, j] = float(np.real(complex(v)))
+        return out
+
+    def values(self, e, symvals: np.ndarray) -> np.ndarray:
+        B = symvals.shape[1]
+        v = self._fn(e)(*symvals)
+        return np.broadcast_to(np.asarray(v, dtype=np.float64), (B,)).copy()
+
+    def dvalues(self, e, sym, symvals: np.ndarray):
+        f = self._dfn(e, sym)
+        if f is None:
+            return None
+        B = symvals.shape[1]
+        return np.broadcast_to(np.asarray(f(*symvals), dtype=np.float64), (B,)).copy()
+
+
+def fill_param_mats(prog: PackedProgram, ev: ParamEvaluator, symvals: np.ndarray) -> np.ndarray:
+    """(B, mats) complex64: the program's matrix buffer for each of B resolvers."""
+    B = symvals.shape[1]
+    out = np.broadcast_to(prog.mats_arr, (B, prog.mats_arr.size)).copy()
+    cache: dict = {}
+    for off, ref, size in prog.param_entries:
+        key = id(ref.gate)
+        if key not in cache:
+            vals = tuple(ev.values(e, symvals) for e in ref.exprs)
+            cache[key] = (vals, ref.gate._matrix_batch(vals))
+        vals, mb = cache[key]
+        payload = ref.transform(mb)
+        out[:, off : off + size] = payload.reshape(B, -1).astype(np.complex64)
+    return out

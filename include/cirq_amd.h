@@ -1,0 +1,211 @@
+/*
+ * cirq_amd.h - C ABI of the MI355X (gfx950) state-vector / density-matrix engine.
+ *
+ * The engine replaces the per-operation NumPy work that Cirq's simulators do on the host.
+ * Every entry point takes raw device pointers (for example a torch tensor's data_ptr()),
+ * explicit sizes and a hipStream_t passed as void*.  Calls are stream-ordered; nothing
+ * synchronises unless its comment says so.  Every function returns CQS_OK (0) or an error
+ * code; cqs_last_error() returns a thread-local message for the last failure.
+ * The library never frees caller memory.
+ *
+ * Layout conventions (identical to Cirq's big-endian flattening):
+ *   - A state vector over n qubits is complex64 (float re, float im interleaved), length
+ *     2^n.  Qubit axis i of Cirq's qubit_order is flat-index bit (n-1-i)
+ *     (cirq-core/cirq/sim/sparse_simulator.py:246-267, reshape(-1) of a (2,)*n tensor).
+ *   - A density matrix over n qubits is the same layout over 2n bits: row r, column c at
+ *     flat index r*2^n + c, so row qubit i is bit (2n-1-i) and column qubit i is bit (n-1-i)
+ *     (cirq-core/cirq/sim/density_matrix_simulation_state.py:177-178: left axes i, right
+ *     axes i+n).
+ *   - "bit positions" below always mean flat-index bit positions.
+ */
+#ifndef CIRQ_AMD_H
+#define CIRQ_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CQS_ABI_VERSION 1
+
+/* status codes */
+#define CQS_OK 0
+#define CQS_ERR_ARG 1         /* bad argument (maps to ValueError on the Python side)   */
+#define CQS_ERR_HIP 2         /* HIP runtime error                                      */
+#define CQS_ERR_OOM 3         /* allocation failure (maps to MemoryError)               */
+#define CQS_ERR_UNSUPPORTED 4 /* op kind or size the engine does not handle (TypeError) */
+
+/* op kinds inside a pass program */
+#define CQS_OP_DENSE 1     /* 2^k x 2^k complex matrix on k tile-local bits (k <= 5)        */
+#define CQS_OP_DIAG 2      /* 2^k diagonal on k arbitrary bit positions (k <= 6)            */
+#define CQS_OP_PAIRPHASE 3 /* product of phases on bit pairs (a CZ / CZPow layer)           */
+#define CQS_OP_SCALE 4     /* multiply every amplitude by one complex number (global phase) */
+
+#define CQS_MAX_LOCAL 16 /* largest tile: 2^16 amplitudes are never used; L <= 14 in practice */
+#define CQS_MAX_BITS 48  /* largest flat index width handled by the pass kernels            */
+
+/*
+ * One operation of a pass program (64 bytes).
+ *   DENSE:     t[0..k-1] are tile-local bit indices (0..L-1); bit m of the matrix row/column
+ *              index corresponds to tile bit t[m].  The matrix is row-major at mats[mat].
+ *              cl_mask/cl_val: controls on tile-local bits (the op acts only on groups whose
+ *              tile index satisfies (j & cl_mask) == cl_val); co_*: controls on bits outside
+ *              the tile (global positions, 64-bit masks split in two words).
+ *   DIAG:      t[0..k-1] are global bit positions; bit m of the diagonal index is
+ *              bit t[m] of the flat index.  Diagonal at mats[mat .. mat+2^k).
+ *   PAIRPHASE: k pairs; global positions at aux[t[0] + 2p], aux[t[0] + 2p + 1]; the phase
+ *              of pair p is mats[mat + p] and applies when both bits are 1.
+ *   SCALE:     every amplitude times mats[mat].
+ *   slot:      parameter slot for gradient bookkeeping (-1 when unused).
+ */
+typedef struct cqs_op {
+  int32_t kind;
+  int32_t k;
+  int32_t mat;
+  int32_t slot;
+  int32_t t[6];
+  int32_t cl_mask;
+  int32_t cl_val;
+  uint32_t co_mask_lo, co_mask_hi;
+  uint32_t co_val_lo, co_val_hi;
+} cqs_op;
+
+/*
+ * One pass: every workgroup gathers the 2^L amplitudes whose flat indices differ only in
+ * the L bit positions lpos[0..L-1] (ascending), runs ops[op_begin .. op_begin+op_count)
+ * on them in LDS and writes them back.  One pass reads and writes the state once.
+ */
+typedef struct cqs_pass {
+  int32_t L;
+  int32_t op_begin;
+  int32_t op_count;
+  int32_t max_k; /* largest DENSE k in the pass (sizes the LDS matrix stage) */
+  int32_t lpos[CQS_MAX_LOCAL];
+} cqs_pass;
+
+/* ---------------------------------------------------------------- library */
+
+/* ABI version (CQS_ABI_VERSION).  No reference counterpart. */
+int cqs_abi_version(void);
+
+/* Thread-local message of the last failing call ("" if none). */
+const char* cqs_last_error(void);
+
+/* Number of compute units and total device memory of the current device. */
+int cqs_device_info(int32_t* n_cu, uint64_t* total_mem_bytes);
+
+/* ---------------------------------------------------------- state set-up */
+
+/*
+ * amps[0 .. 2^n_bits) = 0 except amps[index] = 1.
+ * Replaces the initial-state construction of
+ * cirq-core/cirq/sim/state_vector_simulation_state.py:49-76 (qis.to_valid_state_vector)
+ * and, with n_bits = 2n and index = b*2^n + b, the density-matrix one of
+ * cirq-core/cirq/sim/density_matrix_simulation_state.py:41-76.
+ */
+int cqs_init_basis(void* amps, int32_t n_bits, uint64_t index, void* stream);
+
+/* ------------------------------------------------------------ gate passes */
+
+/*
+ * Runs n_passes passes in order on one state (2^n_bits complex64 amplitudes, in place).
+ * passes is a HOST array; ops/mats/aux are DEVICE arrays (mats: complex64).
+ * Replaces the per-operation loop of cirq-core/cirq/sim/simulator_base.py:198-212 and the
+ * NumPy kernels it reaches: linalg.apply_matrix_to_slices
+ * (cirq-core/cirq/linalg/transformations.py:310-380), linalg.targeted_left_multiply
+ * (cirq-core/cirq/linalg/transformations.py:105-172), the gates' own _apply_unitary_
+ * (cirq-core/cirq/ops/common_gates.py:123-133,454-464,658-669,1072-1082) and, on a density
+ * matrix, protocols.apply_channel (cirq-core/cirq/protocols/apply_channel_protocol.py:168-356).
+ */
+int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
+                   const cqs_op* ops, const void* mats, const int32_t* aux, void* stream);
+
+/* ---------------------------------------------------------- reductions */
+
+/*
+ * Pauli-string expectations, FP64 accumulation.  Term t has flip mask xmask[t] (qubits with
+ * X or Y), sign mask zmask[t] (qubits with Z or Y) and complex coefficient coef[2t],coef[2t+1]
+ * that already includes i^(number of Y).  xmask/zmask/coef are HOST arrays; out is DEVICE
+ * double[2*n_terms] (re, im).  For a state vector (is_dm = 0):
+ *   out[t] = coef_t * sum_i (-1)^popcount(i & zmask) conj(psi[i ^ xmask]) psi[i]
+ * which is cirq-core/cirq/ops/pauli_string.py:625-656 summed as in
+ * cirq-core/cirq/ops/linear_combinations.py:717-719.  For a density matrix over n qubits
+ * (is_dm = 1, amps has 2^(2n) entries):
+ *   out[t] = coef_t * sum_r (-1)^popcount(r & zmask) rho[r, r ^ xmask]
+ * which is cirq-core/cirq/ops/pauli_string.py:734-764 (before its final real part).
+ * workspace: DEVICE scratch of cqs_expect_workspace_bytes(n_terms) bytes.
+ */
+size_t cqs_expect_workspace_bytes(int32_t n_terms);
+int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms,
+                      const uint64_t* xmask, const uint64_t* zmask, const double* coef,
+                      double* out, void* workspace, void* stream);
+
+/*
+ * phi = sum_t coef_t P_t psi (the PauliSum applied to a state vector), complex64 out; the
+ * term arrays are HOST arrays as above.
+ * Used by the adjoint gradient.  No direct reference counterpart: Cirq evaluates each term
+ * separately (cirq-core/cirq/ops/pauli_string.py:648-656).
+ */
+int cqs_apply_paulisum(const void* psi, void* phi, int32_t n_qubits, int32_t n_terms,
+                       const uint64_t* xmask, const uint64_t* zmask, const double* coef,
+                       void* stream);
+
+/* out[0] = sum_i |amps[i]|^2 in FP64 (DEVICE double).  Norm check of
+ * cirq-core/cirq/qis/states.py:865-897 and cirq-core/cirq/sim/state_vector_simulator.py:131-146. */
+int cqs_norm2(const void* amps, int32_t n_bits, double* out, void* workspace, void* stream);
+
+/* amps[i] = ((i & mask) == value) ? amps[i] * (re + i im) : 0: projection onto a measurement
+ * outcome and renormalisation (cirq-core/cirq/sim/state_vector.py:235-322,
+ * cirq-core/cirq/sim/density_matrix_utils.py:97-182 with the masks of both row and column bits). */
+int cqs_collapse(void* amps, int32_t n_bits, uint64_t mask, uint64_t value, float re, float im, void* stream);
+
+/* amps *= scale (complex64) over 2^n_bits entries: the renormalisation of
+ * cirq-core/cirq/sim/state_vector_simulator.py:139-146. */
+int cqs_scale(void* amps, int32_t n_bits, float re, float im, void* stream);
+
+/* ---------------------------------------------------------- sampling */
+
+/*
+ * Sampling all qubits of a state (is_dm = 0: |psi_i|^2; is_dm = 1: Re rho[i,i]), bit-exact
+ * with the NumPy path of cirq-core/cirq/sim/state_vector.py:219-226 /
+ * cirq-core/cirq/sim/density_matrix_utils.py:84-89:
+ *   p = (psi * conj(psi)).real (float32), p /= p.sum() (NumPy's float32 summation order),
+ *   cdf = cumsum(float64(p)) (sequential), cdf /= cdf[-1],
+ *   idx = searchsorted(cdf, u, 'right') for the caller's uniforms u (RandomState.random_sample).
+ * cqs_sample_prepare builds the per-chunk CDF state in workspace; cqs_sample_draw maps
+ * uniforms (DEVICE double[reps]) to indices (DEVICE int64[reps]).
+ * probs_in (optional, DEVICE float32[2^n_qubits]): use these probabilities instead of the
+ * amplitudes (marginal measurements); pass NULL otherwise.
+ * qubit_axes (optional, HOST int32[n_qubits]): measure all qubits in this order (Cirq axes; the
+ * first is the most significant bit of the drawn index), the transpose of
+ * cirq-core/cirq/sim/simulation_utils.py:44-48 with nothing summed; NULL = natural order.
+ */
+size_t cqs_sample_workspace_bytes(int32_t n_qubits);
+int cqs_sample_prepare(const void* amps, const float* probs_in, int32_t n_qubits, int32_t is_dm,
+                       const int32_t* qubit_axes, void* workspace, void* stream);
+int cqs_sample_draw(const void* amps, const float* probs_in, int32_t n_qubits, int32_t is_dm,
+                    const int32_t* qubit_axes, const double* uniforms, int64_t reps, int64_t* out_index,
+                    void* workspace, void* stream);
+
+/*
+ * Marginal probabilities over measured qubits, in NumPy's order: p (float32, as above),
+ * transposed so that measured qubits come first in the order given (qubit_axes[0..m) are Cirq
+ * axes), then summed over the rest (cirq-core/cirq/sim/simulation_utils.py:41-60).
+ * qubit_axes: HOST int32[m].  out: DEVICE float32[2^m], not normalised (cqs_sample_prepare
+ * normalises).
+ */
+int cqs_marginal_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const int32_t* qubit_axes,
+                       int32_t m, float* out, void* stream);
+
+/*
+ * Index -> big-endian digits: out[r*m + k] = bit (m-1-k) of index[r]
+ * (cirq-core/cirq/value/digits.py:125-170 as used by cirq-core/cirq/sim/state_vector.py:228-232).
+ */
+int cqs_indices_to_bits(const int64_t* index, int64_t reps, int32_t m, uint8_t* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CIRQ_AMD_H */

@@ -1,0 +1,182 @@
+"""Test helpers: circuit -> oracle op conversion and a NumPy model of the pass kernel.
+
+``to_oracle_ops`` describes each operation by its gate family and public parameters so that the
+oracle computes every matrix from its own tables (not from cirq_amd's gate code).
+
+``run_pass_model`` executes a PackedProgram with the semantics the HIP pass kernel implements
+(tile gather over local bits, dense ops on tile bits with local/outer controls, diagonals and pair
+phases on global bits, scales).  It lets the host compiler (lowering, fusion, scheduling,
+packing) be tested on a machine without a GPU.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from cirq_amd import _lib
+
+
+def _gate_spec(gate):
+    name = type(gate).__name__
+    if name in ("_PauliX", "_PauliY", "_PauliZ"):
+        return "Pauli", {"pauli": name[-1]}
+    if name in ("Rx", "Ry", "Rz"):
+        return name, {"rads": float(gate.rads)}
+    if name in (
+        "XPowGate", "YPowGate", "ZPowGate", "HPowGate", "CZPowGate", "CXPowGate", "SwapPowGate",
+        "ISwapPowGate", "XXPowGate", "YYPowGate", "ZZPowGate", "CCZPowGate", "CCXPowGate",
+    ):
+        return name, {"exponent": float(gate.exponent), "global_shift": float(gate.global_shift)}
+    if name == "PhasedXPowGate":
+        return name, {
+            "phase_exponent": float(gate.phase_exponent),
+            "exponent": float(gate.exponent),
+            "global_shift": float(gate.global_shift),
+        }
+    if name == "PhasedXZGate":
+        return name, {
+            "x_exponent": float(gate.x_exponent),
+            "z_exponent": float(gate.z_exponent),
+            "axis_phase_exponent": float(gate.axis_phase_exponent),
+        }
+    if name == "FSimGate":
+        return name, {"theta": float(gate.theta), "phi": float(gate.phi)}
+    if name == "DiagonalGate":
+        return name, {"angles": [float(a) for a in gate.diag_angles_radians]}
+    if name == "ControlledGate":
+        sf, sp = _gate_spec(gate.sub_gate)
+        return "Controlled", {"sub_family": sf, "sub_params": sp, "control_values": list(gate.control_values)}
+    if name == "IdentityGate":
+        return name, {"num_qubits": gate.num_qubits()}
+    if name == "CSwapGate":
+        return name, {}
+    return "MatrixGate", {"matrix": np.asarray(gate._unitary_())}
+
+
+def _channel_spec(gate):
+    name = type(gate).__name__
+    if name == "DepolarizingChannel":
+        return "depolarize", {"p": gate.p, "n_qubits": gate.n_qubits}
+    if name == "AmplitudeDampingChannel":
+        return "amplitude_damp", {"gamma": gate.gamma}
+    if name == "GeneralizedAmplitudeDampingChannel":
+        return "generalized_amplitude_damp", {"p": gate.p, "gamma": gate.gamma}
+    if name == "PhaseDampingChannel":
+        return "phase_damp", {"gamma": gate.gamma}
+    if name == "BitFlipChannel":
+        return "bit_flip", {"p": gate.p}
+    if name == "PhaseFlipChannel":
+        return "phase_flip", {"p": gate.p}
+    if name == "ResetChannel":
+        return "reset", {}
+    return "kraus", {"ops": list(gate._kraus_())}
+
+
+def to_oracle_ops(circuit, qubits, resolver=None):
+    idx = {q: i for i, q in enumerate(qubits)}
+    out = []
+    for moment in circuit:
+        for op in moment:
+            if resolver is not None and hasattr(op, "_resolve_parameters_"):
+                op = op._resolve_parameters_(resolver, True)
+            gate = getattr(op, "gate", None)
+            axes = tuple(idx[q] for q in op.qubits)
+            if gate is None:  # a multi-qubit PauliString
+                for q, p in op.items():
+                    out.append(("gate", "Pauli", {"pauli": p._name}, (idx[q],)))
+                if op.coefficient != 1:
+                    out.append(("gate", "GlobalPhase", {"coefficient": complex(op.coefficient)}, ()))
+                continue
+            name = type(gate).__name__
+            if name == "MeasurementGate":
+                continue
+            if name == "GlobalPhaseGate":
+                out.append(("gate", "GlobalPhase", {"coefficient": complex(gate.coefficient)}, ()))
+                continue
+            if hasattr(gate, "_kraus_") or hasattr(gate, "_mixture_"):
+                fam, params = _channel_spec(gate)
+                out.append(("channel", fam, params, axes))
+                continue
+            fam, params = _gate_spec(gate)
+            out.append(("gate", fam, params, axes))
+    return out
+
+
+def terms_of(psum, qubits):
+    """PauliSum -> oracle terms [(coefficient, {axis: 'X'|'Y'|'Z'})]."""
+    idx = {q: i for i, q in enumerate(qubits)}
+    return [(complex(t.coefficient), {idx[q]: p._name for q, p in t.items()}) for t in psum]
+
+
+# ------------------------------------------------------------------------------ pass model
+def _deposit(x, pos):
+    r = np.zeros_like(x, dtype=np.int64)
+    for b, p in enumerate(pos):
+        r |= ((x >> b) & 1) << p
+    return r
+
+
+def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) -> np.ndarray:
+    """Executes the packed program on a copy of `state` (flat complex array)."""
+    st = np.array(state, dtype=np.complex128)
+    n = packed.n_bits
+    ops = packed.ops_arr
+    mats = np.asarray(packed.mats_arr if mats is None else mats, dtype=np.complex128)
+    aux = packed.aux_arr
+    for ps in packed.pass_arr:
+        L = int(ps["L"])
+        lpos = [int(x) for x in ps["lpos"][:L]]
+        opos = [p for p in range(n) if p not in lpos]
+        tiles = np.arange(1 << (n - L), dtype=np.int64)
+        base = _deposit(tiles, opos)  # (ntiles,)
+        off = _deposit(np.arange(1 << L, dtype=np.int64), lpos)  # (T,)
+        I = base[:, None] + off[None, :]
+        tile = st[I]
+        gi = I
+        for r in ops[int(ps["op_begin"]) : int(ps["op_begin"]) + int(ps["op_count"])]:
+            kind = int(r["kind"])
+            com = int(r["co_mask_lo"]) | (int(r["co_mask_hi"]) << 32)
+            cov = int(r["co_val_lo"]) | (int(r["co_val_hi"]) << 32)
+            tile_ok = (base & com) == cov  # (ntiles,)
+            if kind == _lib.OP_DENSE:
+                k = int(r["k"])
+                t = [int(x) for x in r["t"][:k]]
+                D = 1 << k
+                M = mats[int(r["mat"]) : int(r["mat"]) + D * D].reshape(D, D)
+                j = np.arange(1 << L)
+                j0 = j[np.all([(j >> b) & 1 == 0 for b in t], axis=0)]
+                pat = np.array([sum(((m >> b) & 1) << t[b] for b in range(k)) for m in range(D)])
+                idx = j0[:, None] + pat[None, :]  # (G, D)
+                grp_ok = (j0 & int(r["cl_mask"])) == int(r["cl_val"])
+                V = tile[:, idx]  # (ntiles, G, D)
+                out = V @ M.T
+                ok = tile_ok[:, None, None] & grp_ok[None, :, None]
+                tile[:, idx] = np.where(ok, out, V)
+            elif kind == _lib.OP_DIAG:
+                k = int(r["k"])
+                t = [int(x) for x in r["t"][:k]]
+                Dv = mats[int(r["mat"]) : int(r["mat"]) + (1 << k)]
+                di = np.zeros_like(gi)
+                for m, b in enumerate(t):
+                    di |= ((gi >> b) & 1) << m
+                tile = np.where(tile_ok[:, None], tile * Dv[di], tile)
+            elif kind == _lib.OP_PAIRPHASE:
+                k = int(r["k"])
+                a0 = int(r["t"][0])
+                ph = mats[int(r["mat"]) : int(r["mat"]) + k]
+                for p in range(k):
+                    a, b = int(aux[a0 + 2 * p]), int(aux[a0 + 2 * p + 1])
+                    both = ((gi >> a) & (gi >> b) & 1).astype(bool)
+                    tile = np.where(both & tile_ok[:, None], tile * ph[p], tile)
+            elif kind == _lib.OP_SCALE:
+                tile = np.where(tile_ok[:, None], tile * mats[int(r["mat"])], tile)
+            else:
+                raise ValueError(kind)
+        st[I] = tile
+    return st
+
+
+def fidelity(a: np.ndarray, b: np.ndarray) -> float:
+    a = np.asarray(a, dtype=np.complex128).reshape(-1)
+    b = np.asarray(b, dtype=np.complex128).reshape(-1)
+    return float(abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real))

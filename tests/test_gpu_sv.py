@@ -1,0 +1,210 @@
+"""GPU parity tests: the HIP engine against the NumPy oracle and NumPy's own sampling.
+
+Tolerances (complex64 engine vs complex128 oracle): max-abs amplitude error <= 1e-5 and fidelity
+>= 1 - 1e-6; density matrices within 1e-5 Frobenius; samples bit-exact given the same uniforms.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import cirq_amd as cirq
+from cirq_amd import engine, generators
+from oracle import cirq_np as O
+from tests.helpers import fidelity, terms_of, to_oracle_ops
+from tests.test_host import _random_circuit
+
+pytestmark = pytest.mark.gpu
+
+ATOL = 1e-5
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("tile", [(3, 1), (5, 2), (8, 4), (10, 4)])
+def test_random_circuits_match_oracle(gpu, seed, tile):
+    _log(f"random circuit seed={seed} tile={tile}")
+    rng = np.random.RandomState(seed)
+    n = 10
+    c, qs = _random_circuit(n, 120, rng)
+    sim = cirq.Simulator(tile_bits=tile[0], coalesce_bits=tile[1])
+    got = sim.simulate(c, qubit_order=qs).final_state_vector
+    want = O.simulate_state_vector(to_oracle_ops(c, qs), n)
+    assert np.max(np.abs(got - want)) <= ATOL
+    assert fidelity(got, want) >= 1 - 1e-6
+
+
+@pytest.mark.parametrize("n,depth,seed", [(12, 12, 0), (16, 14, 1), (20, 20, 0)])
+def test_grid_circuit_matches_oracle(gpu, n, depth, seed):
+    _log(f"grid circuit n={n} depth={depth}")
+    rows = {12: (3, 4), 16: (4, 4), 20: (4, 5)}[n]
+    c = generators.grid_circuit(*rows, depth, seed=seed)
+    qs = sorted(c.all_qubits())
+    got = cirq.Simulator().simulate(c).final_state_vector
+    want = O.simulate_state_vector(to_oracle_ops(c, qs), n)
+    assert np.max(np.abs(got - want)) <= ATOL
+    assert fidelity(got, want) >= 1 - 1e-6
+
+
+def test_sampling_bit_exact_with_numpy_choice(gpu):
+    """Same p-vector (from the GPU state) + same RandomState -> identical indices."""
+    _log("sampling bit-exact")
+    for n, seed in [(1, 1234), (3, 5), (8, 7), (14, 11), (18, 3), (21, 2)]:
+        c = generators.grid_circuit(3, 7, 8, seed=seed, n_qubits=n) if n > 3 else cirq.Circuit(
+            cirq.H.on_each(*cirq.LineQubit.range(n)), cirq.X(cirq.LineQubit(0)) ** 0.3
+        )
+        sim = cirq.Simulator()
+        res = sim.simulate(c)
+        st = res._state
+        psi = st.cpu().numpy()
+        reps = 20000
+        p = O.state_vector_probs(psi, list(range(n)))
+        want = np.random.RandomState(seed).choice(len(p), size=reps, p=p)
+        u = np.random.RandomState(seed).random_sample(reps)
+        idx = sim._sampler.draw(st, n, False, u).cpu().numpy()
+        assert np.array_equal(idx, want), f"n={n}: {np.sum(idx != want)} mismatches"
+        bits = sim._sampler.bits(torch.from_numpy(idx).to(st.device), n)
+        want_bits = np.array([O.big_endian_int_to_digits(int(i), (2,) * n) for i in want[:200]])
+        assert np.array_equal(bits[:200], want_bits)
+
+
+def test_marginal_sampling_bit_exact(gpu):
+    _log("marginal sampling")
+    c = generators.grid_circuit(3, 4, 6, seed=4)
+    sim = cirq.Simulator()
+    st = sim.simulate(c)._state
+    psi = st.cpu().numpy()
+    for axes in ([3, 0, 7], [11, 2], [5], list(range(12))[::-1]):
+        p = O.state_vector_probs(psi, axes)
+        probs = sim._sampler.marginal(st, 12, False, axes)
+        raw = (psi * psi.conj()).real
+        t = np.transpose(raw.reshape((2,) * 12), axes + [a for a in range(12) if a not in axes]).reshape(2 ** len(axes), -1)
+        assert np.array_equal(probs.cpu().numpy(), np.sum(t, axis=-1)), axes
+        u = np.random.RandomState(9).random_sample(5000)
+        want = np.random.RandomState(9).choice(len(p), size=5000, p=p)
+        got = sim._sampler.draw(None, len(axes), False, u, probs=probs).cpu().numpy()
+        assert np.array_equal(got, want), axes
+
+
+def test_run_seeded_known_answer(gpu):
+    # cirq-core/cirq/sim/sparse_simulator_test.py:1046-1110
+    a = cirq.NamedQubit("a")
+    circuit = cirq.Circuit(cirq.X(a) ** 0.5, cirq.measure(a, key="a"))
+    sim = cirq.Simulator(seed=1234)
+    r1 = sim.run(circuit, repetitions=30).measurements["a"][:, 0]
+    r2 = sim.run(circuit, repetitions=30).measurements["a"][:, 0]
+    assert "".join(map(str, r1)) == "010110011101110110110100110101"
+    assert "".join(map(str, r2)) == "101011010100011101010110111111"
+
+
+def test_run_matches_oracle_sampling_order(gpu):
+    """Entangled circuit: one factor sampled in Cirq's merge order, same prng stream."""
+    _log("run vs oracle")
+    qs = cirq.LineQubit.range(5)
+    c = cirq.Circuit(
+        cirq.H(qs[4]), cirq.CNOT(qs[4], qs[1]), cirq.ry(0.7).on(qs[2]), cirq.CZ(qs[2], qs[1]),
+        cirq.X(qs[0]) ** 0.4, cirq.CNOT(qs[0], qs[3]), cirq.ISWAP(qs[3], qs[2]) ** 0.5,
+        cirq.measure(*qs, key="m"),
+    )
+    sim = cirq.Simulator(seed=77)
+    got = sim.run(c, repetitions=2000).measurements["m"]
+    # the oracle: factor order from Cirq's merge rule, probabilities from the GPU state
+    from cirq_amd.simulator import product_state_factors, _split_terminal_measurements
+
+    body, _ = _split_terminal_measurements(c)
+    factors = product_state_factors(body, tuple(qs))
+    assert len(factors) == 1
+    order = factors[0]
+    psi = cirq.Simulator().simulate(body, qubit_order=qs)._state.cpu().numpy()
+    axes = [qs.index(q) for q in order]
+    s = O.sample_state_vector(psi, axes, 2000, np.random.RandomState(77))
+    want = s[:, [order.index(q) for q in qs]]
+    assert np.array_equal(got, want)
+
+
+def test_expectations_match_oracle(gpu):
+    _log("expectations")
+    c = generators.grid_circuit(3, 4, 8, seed=1)
+    qs = sorted(c.all_qubits())
+    psum = generators.random_pauli_sum(qs, 20, 3, seed=2)
+    zz = generators.nearest_neighbour_zz(qs)
+    sim = cirq.Simulator()
+    got = sim.simulate_expectation_values(c, [psum] + zz)
+    psi = O.simulate_state_vector(to_oracle_ops(c, qs), 12)
+    want = [sum(O.expectation_from_state_vector(psi, t, 12) for t in terms_of(psum, qs))]
+    want += [sum(O.expectation_from_state_vector(psi, t, 12) for t in terms_of(cirq.PauliSum.wrap(z), qs)) for z in zz]
+    assert np.max(np.abs(np.array(got) - np.array(want))) < 1e-5
+
+
+def test_density_matrix_matches_oracle(gpu):
+    _log("density matrix")
+    nc = generators.noisy_line_circuit(5, 4, seed=3, p=0.02, gamma=0.03)
+    qs = sorted(nc.all_qubits())
+    sim = cirq.DensityMatrixSimulator()
+    got = sim.simulate(nc).final_density_matrix
+    want = O.simulate_density_matrix(to_oracle_ops(nc, qs), 5)
+    assert np.linalg.norm(got - want) < 1e-5
+    ev = sim.simulate_expectation_values(nc, [generators.sum_z(qs)])[0]
+    want_ev = sum(O.expectation_from_density_matrix(want, t, 5) for t in terms_of(generators.sum_z(qs), qs))
+    assert isinstance(ev, float) and abs(ev - want_ev) < 1e-5
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_density_matrix_random_channels(gpu, seed):
+    rng = np.random.RandomState(200 + seed)
+    c, qs = _random_circuit(5, 60, rng, with_channels=True)
+    got = cirq.DensityMatrixSimulator(tile_bits=6, coalesce_bits=2).simulate(c, qubit_order=qs).final_density_matrix
+    want = O.simulate_density_matrix(to_oracle_ops(c, qs), 5)
+    assert np.linalg.norm(got - want) < 1e-5
+
+
+def test_density_matrix_sampling_known_answer(gpu):
+    # cirq-core/cirq/sim/density_matrix_utils_test.py:77-93 through the simulator's run():
+    # H then measure gives p = (1/2, 1/2) from diag(rho)
+    q = cirq.LineQubit(0)
+    c = cirq.Circuit(cirq.H(q), cirq.measure(q, key="m"))
+    r = cirq.DensityMatrixSimulator(seed=1234).run(c, repetitions=10).measurements["m"][:, 0]
+    assert r.tolist() == [0, 1, 0, 1, 1, 0, 0, 1, 1, 1]
+
+
+def test_parameter_sweep_matches_oracle(gpu):
+    circ, syms, vals = generators.hardware_efficient_ansatz(8, 3, seed=1)
+    qs = sorted(circ.all_qubits())
+    rng = np.random.RandomState(3)
+    resolvers = [cirq.ParamResolver(dict(zip(syms, rng.uniform(-3, 3, len(syms))))) for _ in range(4)]
+    res = cirq.Simulator().simulate_sweep(circ, resolvers)
+    for r, rr in zip(resolvers, res):
+        want = O.simulate_state_vector(to_oracle_ops(circ, qs, resolver=r), 8)
+        assert np.max(np.abs(rr.final_state_vector - want)) <= ATOL
+
+
+def test_norm_and_inverse_round_trip_at_26_qubits(gpu):
+    """Size-independent checks at a large size: norm preserved, U^-1 U |0> = |0>."""
+    _log("26q round trip")
+    c = generators.grid_circuit(5, 6, 12, seed=5, n_qubits=26)
+    qs = sorted(c.all_qubits())
+    inv = cirq.Circuit([cirq.MatrixGate(np.linalg.inv(op.gate._unitary_())).on(*op.qubits) for op in reversed(list(c.all_operations()))])
+    sim = cirq.Simulator()
+    comp = sim._compile(c, qs)
+    comp_inv = sim._compile(inv, qs)
+    st = engine.new_state(26, sim._dev())
+    comp.dev.run(st)
+    n2 = engine.norm2(st, 26, sim._ws)
+    assert abs(n2 - 1) < 1e-5
+    comp_inv.dev.run(st)
+    a0 = st[0].item()
+    assert abs(abs(a0) ** 2 - 1) < 1e-5
+
+
+def test_unsupported_ops_raise_type_error(gpu):
+    q = cirq.LineQubit.range(2)
+    with pytest.raises(TypeError):
+        cirq.Simulator().simulate(cirq.Circuit(cirq.depolarize(0.1).on(q[0])))
+    with pytest.raises(ValueError):
+        cirq.Simulator().run(cirq.Circuit(cirq.H(q[0])))
