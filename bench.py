@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--reps", type=int, default=1_000_000)
     ap.add_argument("--tile-bits", type=int, default=None)
     ap.add_argument("--coalesce-bits", type=int, default=None)
+    ap.add_argument("--no-blocks", action="store_true", help="disable MFMA block fusion")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-qubits", type=int, default=26)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
@@ -106,7 +107,9 @@ def main():
     qmap = {q: i for i, q in enumerate(qubits)}
     terms = [pauli_masks(t, qmap, n) for z in zz for t in cirq.PauliSum.wrap(z)]
 
-    sim = cirq.Simulator(seed=1234, tile_bits=args.tile_bits, coalesce_bits=args.coalesce_bits)
+    sim = cirq.Simulator(
+        seed=1234, tile_bits=args.tile_bits, coalesce_bits=args.coalesce_bits, mfma_blocks=not args.no_blocks
+    )
     comp = sim._compile(circuit, qubits)
     factors = product_state_factors(circuit, qubits)
     assert len(factors) == 1 and len(factors[0]) == n
@@ -177,6 +180,7 @@ def main():
             "zz_pairs": len(zz),
             "passes": n_pass,
             "tile_bits": int(comp.packed.pass_arr["L"].max()) if n_pass else 0,
+            "mfma_blocks": int((comp.packed.ops_arr["kind"] == 5).sum()),
         },
         "roofline": {
             "bound": "hbm",

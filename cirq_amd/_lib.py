@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libcirq_amd.so"
 
 CQS_OK, CQS_ERR_ARG, CQS_ERR_HIP, CQS_ERR_OOM, CQS_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
-OP_DENSE, OP_DIAG, OP_PAIRPHASE, OP_SCALE = 1, 2, 3, 4
+OP_DENSE, OP_DIAG, OP_PAIRPHASE, OP_SCALE, OP_BLOCK = 1, 2, 3, 4, 5
 MAX_LOCAL = 16
 
 # numpy mirror of cqs_op (16 x 32-bit words)

@@ -97,6 +97,7 @@ bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const 
     return false;
   }
   P.max_k = ps.max_k;
+  P.mscale = 32768.f;
   P.ops = ops + ps.op_begin;
   P.mats = mats;
   P.aux = aux;

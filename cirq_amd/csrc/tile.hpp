@@ -18,6 +18,7 @@ struct TileProg {
   int32_t nouter;    // n - L
   int32_t op_count;  // ops in this pass
   int32_t max_k;     // largest dense k the LDS matrix stage was sized for
+  float mscale;      // amplitude scale of the split-FP16 MFMA path (2^15 for normalised states)
   int32_t lpos[CQS_MAX_LOCAL];
   int32_t opos[CQS_MAX_BITS];
   const cqs_op* ops;
@@ -78,6 +79,85 @@ __device__ __forceinline__ void tile_dense(float2* tile, int L, const cqs_op& op
   }
 }
 
+// ---------------------------------------------------------------- fused blocks on MFMA
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+
+// Y = U X on tile bits op.t[0..K) with v_mfma_f32_32x32x16_f16.  The tile is viewed as a
+// (2^K rows) x (2^(L-K) columns) complex matrix; in real-interleaved form (row 2r = Re, 2r+1 = Im)
+// it is R (2^(K+1) square) times X.  Each wave owns whole 32-column blocks, so reads and writes of
+// different waves never overlap and the block needs no barrier inside.
+template <int K>
+__device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_op& op, const half8_t* A,
+                                                float scale, float inv_scale) {
+  constexpr int D = 1 << K;
+  constexpr int R = 2 * D;
+  constexpr int KS = R / 16;
+  constexpr int RB = R / 32;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, c32 = lane & 31;
+  int tmask = 0;
+#pragma unroll
+  for (int m = 0; m < K; ++m) tmask |= 1 << op.t[m];
+  // tile offset of complex row r
+  auto rowoff = [&](int r) {
+    int j = 0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) j |= ((r >> m) & 1) << op.t[m];
+    return j;
+  };
+  const int ncol = 1 << (L - K);
+  const int ncb = (ncol + 31) >> 5;
+  for (int cb = wave; cb < ncb; cb += WG / 64) {
+    const int col = cb * 32 + c32;
+    const bool valid = col < ncol;
+    int jc = 0;
+    for (int b = 0, i = 0; b < L; ++b) {
+      if (!((tmask >> b) & 1)) {
+        jc |= ((col >> i) & 1) << b;
+        ++i;
+      }
+    }
+    floatx16_t acc[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int a = 0; a < 16; ++a) acc[rb][a] = 0.f;
+#pragma unroll 1
+    for (int ks = 0; ks < KS; ++ks) {
+      half8_t bhi, blo;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = ks * 8 + 4 * h + e;
+        const float2 v = valid ? tile[jc | rowoff(r)] : make_float2(0.f, 0.f);
+        const float xr = v.x * scale, xi = v.y * scale;
+        const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+        bhi[2 * e] = hr;
+        bhi[2 * e + 1] = hi;
+        blo[2 * e] = (_Float16)(xr - (float)hr);
+        blo[2 * e + 1] = (_Float16)(xi - (float)hi);
+      }
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        // A operands stream from the L1/L2 caches (every workgroup reads the same matrix)
+        const half8_t ahi = A[(ks * RB + rb) * 64 + lane];
+        const half8_t alo = A[(KS * RB + ks * RB + rb) * 64 + lane];
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bhi, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bhi, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, blo, acc[rb], 0, 0, 0);
+      }
+    }
+    if (valid) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int a = 0; a < 16; a += 2) {
+          const int i = rb * 32 + (a & 3) + 8 * (a >> 2) + 4 * h;  // even: real part of row i/2
+          tile[jc | rowoff(i >> 1)] = make_float2(acc[rb][a] * inv_scale, acc[rb][a + 1] * inv_scale);
+        }
+    }
+  }
+}
+
 // Global flat index of this thread's e-th element.
 __device__ __forceinline__ uint64_t elem_index(const TileProg& P, uint64_t base, uint64_t off_lo,
                                                int nlo, int e) {
@@ -130,6 +210,7 @@ __device__ __forceinline__ void run_dense(float2* tile, float2* mstage, int L, i
                                           const float2* mats) {
   const float2* M = mats + op.mat;
   if (op.k > max_k && op.k >= 3) return;  // the stage was not sized for it (host bug)
+  // k = 3 stages its 8x8 matrix in LDS; k <= 2 reads it through the scalar cache
   switch (op.k) {
     case 1: tile_dense<1>(tile, L, op, M); break;
     case 2: tile_dense<2>(tile, L, op, M); break;
@@ -138,17 +219,7 @@ __device__ __forceinline__ void run_dense(float2* tile, float2* mstage, int L, i
       __syncthreads();
       tile_dense<3>(tile, L, op, mstage);
       break;
-    case 4:
-      stage_matrix(mstage, M, 256);
-      __syncthreads();
-      tile_dense<4>(tile, L, op, mstage);
-      break;
-    case 5:
-      stage_matrix(mstage, M, 1024);
-      __syncthreads();
-      tile_dense<5>(tile, L, op, mstage);
-      break;
-    default: break;
+    default: break;  // wider matrices run as CQS_OP_BLOCK on MFMA
   }
 }
 
@@ -166,6 +237,14 @@ __device__ __forceinline__ bool run_program(float2* tile, float2* mstage, const 
     if (op.kind == CQS_OP_DENSE) {
       __syncthreads();
       run_dense(tile, mstage, P.L, P.max_k, op, P.mats);
+      elem_mode = false;
+    } else if (op.kind == CQS_OP_BLOCK) {
+      __syncthreads();
+      const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat);
+      if (op.k == 4)
+        tile_block_mfma<4>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
+      else if (op.k == 5)
+        tile_block_mfma<5>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
       elem_mode = false;
     } else {
       if (!elem_mode) {

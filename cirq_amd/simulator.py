@@ -131,7 +131,7 @@ def product_state_factors(circuit, qubits) -> list[list]:
 class _Compiled:
     """A circuit lowered, scheduled and uploaded for one qubit order."""
 
-    def __init__(self, circuit, qubits, dm: bool, device, L: int, coalesce: int, fuse: bool):
+    def __init__(self, circuit, qubits, dm: bool, device, L: int, coalesce: int, fuse: bool, blocks: bool = True):
         self.qubits = tuple(qubits)
         self.n = len(self.qubits)
         self.dm = dm
@@ -140,7 +140,7 @@ class _Compiled:
         low.lower_circuit(circuit)
         self.lowering = low
         self.measurements: list[Measurement] = low.measurements
-        self.packed = compile_ops(low.ops, self.n_bits, L=L, coalesce=coalesce, fuse=fuse)
+        self.packed = compile_ops(low.ops, self.n_bits, L=L, coalesce=coalesce, fuse=fuse, blocks=blocks)
         self.evaluator = ParamEvaluator(low.param_refs)
         self.has_params = bool(self.packed.param_entries)
         self.dev = engine.DeviceProgram(self.packed, device)
@@ -163,6 +163,7 @@ class _SimulatorBase:
         tile_bits: int | None = None,
         coalesce_bits: int | None = None,
         fuse: bool = True,
+        mfma_blocks: bool = True,
     ):
         if np.dtype(dtype).kind != "c":
             raise ValueError(f"dtype must be a complex type but was {dtype}")
@@ -175,6 +176,7 @@ class _SimulatorBase:
         self._tile_bits = tile_bits
         self._coalesce = coalesce_bits
         self._fuse = fuse
+        self._blocks = mfma_blocks
         self._cache: "collections.OrderedDict" = collections.OrderedDict()
         self._device = None
         self._sampler = None
@@ -210,7 +212,7 @@ class _SimulatorBase:
             return hit[1]
         nb = 2 * len(qubits) if self._dm else len(qubits)
         L, c = self._tile_params(nb)
-        comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse)
+        comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse, self._blocks)
         try:
             ref = weakref.ref(circuit)
         except TypeError:

@@ -329,6 +329,189 @@ def fuse_single_bit(ops: list[LOp]) -> list[LOp]:
 
 
 # --------------------------------------------------------------------------------------------
+# Block fusion inside a pass.  The reference's analogue is merge_k_qubit_unitaries
+# (cirq-core/cirq/transformers/merge_k_qubit_gates.py:29-125): connected runs of gates on at most
+# k qubits become one matrix, equal to the exact product of the originals.
+
+def embed(M: np.ndarray, tbits, ctrls, B) -> np.ndarray:
+    """An op's matrix (little-endian over tbits, controls ((bit, val), ...)) over block bits B."""
+    nb = len(B)
+    D = 1 << nb
+    pos = [B.index(b) for b in tbits]
+    k = len(pos)
+    cols = np.arange(D, dtype=np.int64)
+    ci = np.zeros(D, dtype=np.int64)
+    for m, p in enumerate(pos):
+        ci |= ((cols >> p) & 1) << m
+    tmask = 0
+    for p in pos:
+        tmask |= 1 << p
+    base = cols & ~tmask
+    active = np.ones(D, dtype=bool)
+    for b, v in ctrls:
+        active &= ((cols >> B.index(b)) & 1) == int(v)
+    dep = np.zeros(1 << k, dtype=np.int64)
+    for ri in range(1 << k):
+        for m, p in enumerate(pos):
+            dep[ri] |= ((ri >> m) & 1) << p
+    out = np.zeros((D, D), dtype=np.complex128)
+    ac = np.nonzero(active)[0]
+    rows = base[None, ac] | dep[:, None]
+    out[rows, ac[None, :]] = M[:, ci[ac]]
+    ia = np.nonzero(~active)[0]
+    out[ia, ia] = 1
+    return out
+
+
+def _op_matrix(op: LOp) -> np.ndarray:
+    if op.kind == "dense":
+        return op.mat
+    if op.kind == "diag":
+        return np.diag(op.mat)
+    if op.kind == "pair":
+        return np.diag([1, 1, 1, op.mat]).astype(np.complex128)
+    raise ValueError(op.kind)
+
+
+# Relative costs (per amplitude, arbitrary units) used to choose how a fused group runs.
+COSTS = {"dense1": 1.0, "dense2": 1.8, "dense3": 3.5, "diag": 0.3, "pair": 0.2, "scale": 0.2, "mfma4": 0.7,
+         "mfma5": 1.1}
+
+
+def _valu_cost(op: LOp) -> float:
+    if op.kind == "dense":
+        return COSTS[f"dense{len(op.targets)}"] if len(op.targets) <= 3 else 1e9
+    return COSTS.get(op.kind, 0.3)
+
+
+class _Block:
+    __slots__ = ("bits", "mat", "members")
+
+    def __init__(self, bits, mat, members):
+        self.bits = list(bits)
+        self.mat = mat
+        self.members = list(members)
+
+
+def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = True) -> list[LOp]:
+    """Fuses a pass's fixed ops on local bits into blocks of at most kmax bits.
+
+    Several blocks stay open at once (one per disjoint bit group), so a block keeps absorbing
+    gates on its bits across many cycles until an op links it to bits it cannot take.  A closed
+    block runs as: its original ops (VALU), one fused dense op (k <= 3, VALU) or an MFMA block
+    (k = 4, 5; smaller groups padded), whichever the cost table rates cheapest."""
+    out: list[LOp] = []
+    open_blocks: list[_Block] = []
+    owner: dict = {}
+    use_mfma = mfma and L >= 9
+
+    def close(blk: _Block):
+        open_blocks.remove(blk)
+        for b in blk.bits:
+            owner.pop(b, None)
+        members = blk.members
+        if len(members) == 1:
+            out.append(members[0])
+            return
+        nb = len(blk.bits)
+        separate = sum(_valu_cost(m) for m in members)
+        fused = COSTS[f"dense{nb}"] if nb <= 3 else 1e9
+        mf = (COSTS["mfma4"] if nb <= 4 else COSTS["mfma5"]) if use_mfma else 1e9
+        best = min(separate, fused, mf)
+        if best == separate:
+            out.extend(members)
+        elif best == fused:
+            out.append(LOp("dense", tuple(blk.bits), blk.mat))
+        else:
+            bits = list(blk.bits)
+            mat = blk.mat
+            if nb < 4:
+                extra = [b for b in sorted(local) if b not in bits][: 4 - nb]
+                mat = embed(mat, bits, (), bits + extra)
+                bits = bits + extra
+            out.append(LOp("block", tuple(bits), mat))
+
+    for op in ops:
+        fixed = op.fixed and op.kind in ("dense", "diag", "pair")
+        bits = list(op.targets) + [b for b, _ in op.controls]
+        absorbable = fixed and all(b in local for b in bits) and bits
+        if not absorbable:
+            for blk in {id(owner[b]): owner[b] for b in op.bits if b in owner}.values():
+                close(blk)
+            out.append(op)
+            continue
+        touched = []
+        for b in bits:
+            blk = owner.get(b)
+            if blk is not None and blk not in touched:
+                touched.append(blk)
+        union = []
+        for blk in touched:
+            union += blk.bits
+        union += [b for b in bits if b not in union]
+        if len(union) > kmax:
+            for blk in touched:
+                close(blk)
+            touched = []
+            union = list(dict.fromkeys(bits))
+        mat = np.eye(1 << len(union), dtype=np.complex128)
+        members = []
+        for blk in touched:
+            mat = embed(blk.mat, blk.bits, (), union) @ mat
+            members += blk.members
+            open_blocks.remove(blk)
+        mat = embed(_op_matrix(op), list(op.targets), op.controls, union) @ mat
+        nb = _Block(union, mat, members + [op])
+        open_blocks.append(nb)
+        for b in union:
+            owner[b] = nb
+    for blk in list(open_blocks):
+        close(blk)
+    return out
+
+
+def block_payload(U: np.ndarray) -> np.ndarray:
+    """MFMA A operands of a 2^k block (k = 4, 5): real form split into FP16 hi + lo, in
+    [part][ks][rb][lane][8] order (see CQS_OP_BLOCK), returned as complex64 slots."""
+    D = U.shape[0]
+    R = 2 * D
+    Rm = np.zeros((R, R))
+    Rm[0::2, 0::2] = U.real
+    Rm[0::2, 1::2] = -U.imag
+    Rm[1::2, 0::2] = U.imag
+    Rm[1::2, 1::2] = U.real
+    hi = Rm.astype(np.float16)
+    lo = (Rm - hi.astype(np.float64)).astype(np.float16)
+    KS, RB = R // 16, R // 32
+    lane = np.arange(64)
+    out = np.zeros((2, KS, RB, 64, 8), dtype=np.float16)
+    for part, M in enumerate((hi, lo)):
+        for ks in range(KS):
+            for rb in range(RB):
+                rows = rb * 32 + lane % 32
+                cols = ks * 16 + 8 * (lane // 32)[:, None] + np.arange(8)[None, :]
+                out[part, ks, rb] = M[rows[:, None], cols]
+    return np.frombuffer(out.tobytes(), dtype=np.complex64)
+
+
+def block_matrix_from_payload(payload: np.ndarray, k: int) -> np.ndarray:
+    """Inverse of block_payload (hi + lo in float64), for tests."""
+    D = 1 << k
+    R = 2 * D
+    KS, RB = R // 16, R // 32
+    arr = np.frombuffer(np.ascontiguousarray(payload).tobytes(), dtype=np.float16).reshape(2, KS, RB, 64, 8)
+    Rm = np.zeros((R, R))
+    lane = np.arange(64)
+    for part in range(2):
+        for ks in range(KS):
+            for rb in range(RB):
+                rows = rb * 32 + lane % 32
+                cols = ks * 16 + 8 * (lane // 32)[:, None] + np.arange(8)[None, :]
+                Rm[rows[:, None], cols] += arr[part, ks, rb].astype(np.float64)
+    return Rm[0::2, 0::2] + 1j * Rm[1::2, 0::2]
+
+
+# --------------------------------------------------------------------------------------------
 @dataclasses.dataclass
 class Pass:
     L: int
@@ -418,30 +601,69 @@ class PackedProgram:
                 r = np.zeros((), dtype=_lib.OP_DTYPE)
                 r["slot"] = -1
                 if op.kind == "dense":
-                    k = len(op.targets)
-                    r["kind"] = _lib.OP_DENSE
-                    r["k"] = k
-                    max_k = max(max_k, k)
-                    for m, b in enumerate(op.targets):
-                        r["t"][m] = local[b]
-                    clm = clv = 0
+                    local_ctl = [(b, v) for b, v in op.controls if b in local]
+                    outer_ctl = [(b, v) for b, v in op.controls if b not in local]
                     com = cov = 0
-                    for b, v in op.controls:
-                        if b in local:
-                            clm |= 1 << local[b]
-                            clv |= int(v) << local[b]
-                        else:
-                            com |= 1 << b
-                            cov |= int(v) << b
-                    r["cl_mask"], r["cl_val"] = clm, clv
+                    for b, v in outer_ctl:
+                        com |= 1 << b
+                        cov |= int(v) << b
                     r["co_mask_lo"], r["co_mask_hi"] = com & 0xFFFFFFFF, com >> 32
                     r["co_val_lo"], r["co_val_hi"] = cov & 0xFFFFFFFF, cov >> 32
-                    d = 2**k
-                    payload = op.mat if op.mat is not None else np.zeros((d, d))
-                    r["mat"] = add_mat(payload)
-                    if op.pref is not None:
-                        self.param_entries.append((int(r["mat"]), op.pref, d * d))
-                        r["slot"] = op.pref.slot
+                    k = len(op.targets)
+                    if k + (len(local_ctl) if k >= 4 else 0) > 5:
+                        raise UnsupportedOperation("dense op on more than 5 tile bits")
+                    if k <= 3:
+                        r["kind"] = _lib.OP_DENSE
+                        r["k"] = k
+                        max_k = max(max_k, k)
+                        for m, b in enumerate(op.targets):
+                            r["t"][m] = local[b]
+                        clm = clv = 0
+                        for b, v in local_ctl:
+                            clm |= 1 << local[b]
+                            clv |= int(v) << local[b]
+                        r["cl_mask"], r["cl_val"] = clm, clv
+                        d = 2**k
+                        payload = op.mat if op.mat is not None else np.zeros((d, d))
+                        r["mat"] = add_mat(payload)
+                        if op.pref is not None:
+                            self.param_entries.append((int(r["mat"]), op.pref, d * d))
+                            r["slot"] = op.pref.slot
+                    else:
+                        # 4-5 bit matrices run on MFMA; local controls become part of the matrix
+                        bits = list(op.targets) + [b for b, _ in local_ctl]
+                        r["kind"] = _lib.OP_BLOCK
+                        r["k"] = len(bits)
+                        for m, b in enumerate(bits):
+                            r["t"][m] = local[b]
+                        tb, lc = list(op.targets), tuple(local_ctl)
+                        to_block = lambda M, tb=tb, lc=lc, bits=bits: embed(M, tb, lc, bits)
+                        if mat_off % 2:
+                            add_mat(np.zeros(1))
+                        d = 2 ** len(bits)
+                        payload = op.mat if op.mat is not None else np.eye(d)
+                        r["mat"] = add_mat(block_payload(to_block(payload) if op.mat is not None else payload))
+                        if op.pref is not None:
+                            base_tf = op.pref.transform
+                            ref = ParamRef(
+                                op.pref.gate,
+                                lambda b, base_tf=base_tf, to_block=to_block: np.stack(
+                                    [block_payload(to_block(x)) for x in base_tf(b)]
+                                ),
+                                op.pref.exprs,
+                                op.pref.slot,
+                            )
+                            self.param_entries.append((int(r["mat"]), ref, 2 * d * d))
+                            r["slot"] = op.pref.slot
+                elif op.kind == "block":
+                    k = len(op.targets)
+                    r["kind"] = _lib.OP_BLOCK
+                    r["k"] = k
+                    for m, b in enumerate(op.targets):
+                        r["t"][m] = local[b]
+                    if mat_off % 2:  # 16-byte alignment of the half8 operands
+                        add_mat(np.zeros(1))
+                    r["mat"] = add_mat(block_payload(op.mat))
                 elif op.kind == "diag":
                     k = len(op.targets)
                     r["kind"] = _lib.OP_DIAG
@@ -495,7 +717,7 @@ class PackedProgram:
             for op in ps.ops:
                 if op.kind == "pair":
                     run_pairs.append(op)
-                elif op.kind == "dense":
+                elif op.kind in ("dense", "block"):
                     flush()
                     ops.append(op)
                 else:
@@ -505,10 +727,14 @@ class PackedProgram:
         return out
 
 
-def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True) -> PackedProgram:
+def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
+                blocks: bool = True, kmax: int = 5) -> PackedProgram:
     if fuse:
         ops = fuse_single_bit(ops)
-    return PackedProgram(schedule(ops, n_bits, L, coalesce), n_bits)
+    passes = schedule(ops, n_bits, L, coalesce)
+    if blocks:
+        passes = [Pass(p.L, p.lpos, fuse_blocks(p.ops, set(p.lpos), p.L, kmax)) for p in passes]
+    return PackedProgram(passes, n_bits)
 
 
 # --------------------------------------------------------------------------------------------

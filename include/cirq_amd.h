@@ -38,10 +38,11 @@ extern "C" {
 #define CQS_ERR_UNSUPPORTED 4 /* op kind or size the engine does not handle (TypeError) */
 
 /* op kinds inside a pass program */
-#define CQS_OP_DENSE 1     /* 2^k x 2^k complex matrix on k tile-local bits (k <= 5)        */
+#define CQS_OP_DENSE 1     /* 2^k x 2^k complex matrix on k tile-local bits (k <= 3)        */
 #define CQS_OP_DIAG 2      /* 2^k diagonal on k arbitrary bit positions (k <= 6)            */
 #define CQS_OP_PAIRPHASE 3 /* product of phases on bit pairs (a CZ / CZPow layer)           */
 #define CQS_OP_SCALE 4     /* multiply every amplitude by one complex number (global phase) */
+#define CQS_OP_BLOCK 5     /* fused 2^k x 2^k block (k = 4 or 5) on MFMA, split-FP16 operands  */
 
 #define CQS_MAX_LOCAL 16 /* largest tile: 2^16 amplitudes are never used; L <= 14 in practice */
 #define CQS_MAX_BITS 48  /* largest flat index width handled by the pass kernels            */
@@ -58,6 +59,12 @@ extern "C" {
  *   PAIRPHASE: k pairs; global positions at aux[t[0] + 2p], aux[t[0] + 2p + 1]; the phase
  *              of pair p is mats[mat + p] and applies when both bits are 1.
  *   SCALE:     every amplitude times mats[mat].
+ *   BLOCK:     like DENSE (targets t[0..k-1], k = 4 or 5) but applied with f16 MFMA: the matrix is
+ *              given as its real form R (R[2r+a][2c+b] = Re/Im parts of U[r][c], size 2^(k+1)),
+ *              split as R = Rhi + Rlo in FP16 and pre-arranged in MFMA A-operand order:
+ *              half8 entries [part][ks][rb][lane] = R_part[rb*32 + lane%32][ks*16 + 8*(lane/32) + e]
+ *              starting at mats + mat (16-byte aligned).  Amplitudes are scaled by 2^15 and split
+ *              the same way; three products (hi*hi, lo*hi, hi*lo) accumulate in FP32.
  *   slot:      parameter slot for gradient bookkeeping (-1 when unused).
  */
 typedef struct cqs_op {

@@ -121,7 +121,8 @@ def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) ->
     st = np.array(state, dtype=np.complex128)
     n = packed.n_bits
     ops = packed.ops_arr
-    mats = np.asarray(packed.mats_arr if mats is None else mats, dtype=np.complex128)
+    mats_c64 = np.asarray(packed.mats_arr if mats is None else mats, dtype=np.complex64)
+    mats = mats_c64.astype(np.complex128)
     aux = packed.aux_arr
     for ps in packed.pass_arr:
         L = int(ps["L"])
@@ -138,11 +139,17 @@ def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) ->
             com = int(r["co_mask_lo"]) | (int(r["co_mask_hi"]) << 32)
             cov = int(r["co_val_lo"]) | (int(r["co_val_hi"]) << 32)
             tile_ok = (base & com) == cov  # (ntiles,)
-            if kind == _lib.OP_DENSE:
+            if kind in (_lib.OP_DENSE, _lib.OP_BLOCK):
                 k = int(r["k"])
                 t = [int(x) for x in r["t"][:k]]
                 D = 1 << k
-                M = mats[int(r["mat"]) : int(r["mat"]) + D * D].reshape(D, D)
+                if kind == _lib.OP_DENSE:
+                    M = mats[int(r["mat"]) : int(r["mat"]) + D * D].reshape(D, D)
+                else:
+                    from cirq_amd.tape import block_matrix_from_payload
+
+                    raw = np.asarray(mats_c64[int(r["mat"]) : int(r["mat"]) + 2 * D * D], dtype=np.complex64)
+                    M = block_matrix_from_payload(raw, k)
                 j = np.arange(1 << L)
                 j0 = j[np.all([(j >> b) & 1 == 0 for b in t], axis=0)]
                 pat = np.array([sum(((m >> b) & 1) << t[b] for b in range(k)) for m in range(D)])

@@ -251,3 +251,41 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(lib, name), name
     assert lib.cqs_abi_version() == 1
+
+
+# ------------------------------------------------------------------ MFMA block fusion (packing checked by the model)
+def test_block_payload_round_trip():
+    rng = np.random.RandomState(4)
+    for k in (4, 5):
+        u = np.linalg.qr(rng.randn(2**k, 2**k) + 1j * rng.randn(2**k, 2**k))[0]
+        back = tape.block_matrix_from_payload(tape.block_payload(u), k)
+        assert np.max(np.abs(back - u)) < 2**-21
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("L,coalesce", [(9, 4), (10, 3), (11, 5)])
+def test_pass_model_with_mfma_blocks(seed, L, coalesce):
+    rng = np.random.RandomState(50 + seed)
+    n = 11
+    c, qs = _random_circuit(n, 150, rng)
+    packed = _compile_sv(c, qs, L, coalesce)
+    kinds = set(int(k) for k in packed.ops_arr["kind"])
+    psi0 = np.zeros(2**n, dtype=np.complex128)
+    psi0[0] = 1
+    got = run_pass_model(packed, psi0)
+    want = O.simulate_state_vector(to_oracle_ops(c, qs), n)
+    assert np.max(np.abs(got - want)) < 2e-6
+    assert fidelity(got, want) > 1 - 1e-9
+
+
+def test_grid_circuit_fuses_into_blocks():
+    c = generators.grid_circuit(4, 5, 12, seed=1)
+    qs = sorted(c.all_qubits())
+    packed = _compile_sv(c, qs, 10, 4)
+    kinds = [int(k) for k in packed.ops_arr["kind"]]
+    assert kinds.count(5) > 0  # MFMA blocks present
+    psi0 = np.zeros(2**20, dtype=np.complex128)
+    psi0[0] = 1
+    got = run_pass_model(packed, psi0)
+    want = O.simulate_state_vector(to_oracle_ops(c, qs), 20)
+    assert np.max(np.abs(got - want)) < 2e-6
