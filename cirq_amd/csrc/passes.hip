@@ -38,22 +38,29 @@ __global__ void k_scale(float2* __restrict__ a, uint64_t n, float2 s) {
     a[i] = cmul(a[i], s);
 }
 
-// One pass over a single state: workgroup b owns tile b.
-__global__ __launch_bounds__(WG) void k_pass(float2* __restrict__ psi, TileProg P) {
+// One pass over a single state: workgroup b owns tile b (2^L amplitudes).
+template <int L>
+__global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_pass(float2* __restrict__ psi, TileProg P) {
+  constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
   extern __shared__ float2 lds[];
-  const int L = P.L;
-  const int T = 1 << L;
   float2* tile = lds;
-  float2* mstage = lds + T;
-  const int tid = threadIdx.x;
+  float2* mstage = lds + (1 << L);
+  const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
   const uint64_t base = deposit64((uint64_t)blockIdx.x, P.opos, P.nouter);
-  const int nlo = L < 8 ? L : 8;
-  const uint64_t off_lo = deposit((uint32_t)tid, P.lpos, nlo);
-  const int per = T >= WG ? T / WG : (tid < T ? 1 : 0);
-  for (int e = 0; e < per; ++e) tile[tid + e * WG] = psi[elem_index(P, base, off_lo, nlo, e)];
-  const bool elem = run_program(tile, mstage, P, P.ops, P.op_count, base, off_lo, nlo, per);
-  if (!elem) __syncthreads();
-  for (int e = 0; e < per; ++e) psi[elem_index(P, base, off_lo, nlo, e)] = tile[tid + e * WG];
+  uint64_t off_lo, off_hi[PER];
+  tile_offsets<L>(P, off_lo, off_hi);
+  if (act) {
+    float2 v[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) tile[threadIdx.x + e * NT] = v[e];
+  }
+  run_program<PER, NT>(tile, mstage, P, base, act);
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = tile[threadIdx.x + e * NT];
+  }
 }
 
 // amps[i] = ((i & mask) == value) ? amps[i] * s : 0
@@ -62,6 +69,12 @@ __global__ void k_collapse(float2* __restrict__ a, uint64_t n, uint64_t mask, ui
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     a[i] = ((i & mask) == value) ? cmul(a[i], s) : make_float2(0.f, 0.f);
 }
+
+typedef void (*PassKernel)(float2*, TileProg);
+static const PassKernel kPassKernels[15] = {k_pass<0>,  k_pass<1>,  k_pass<2>,  k_pass<3>, k_pass<4>,
+                                            k_pass<5>,  k_pass<6>,  k_pass<7>,  k_pass<8>, k_pass<9>,
+                                            k_pass<10>, k_pass<11>, k_pass<12>, k_pass<13>, k_pass<14>};
+static int threads_for_L(int L) { return L <= 12 ? 256 : (1 << L) / 16; }
 
 static int grid_for(uint64_t n) {
   uint64_t g = (n + WG - 1) / WG;
@@ -82,8 +95,8 @@ bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const 
   uint64_t used = 0;
   for (int b = 0; b < ps.L; ++b) {
     const int p = ps.lpos[b];
-    if (p < 0 || p >= n || ((used >> p) & 1ull) || (b > 0 && p <= ps.lpos[b - 1])) {
-      set_error("pass local positions must be ascending, distinct and < n (bit %d = %d)", b, p);
+    if (p < 0 || p >= n || ((used >> p) & 1ull)) {
+      set_error("pass local positions must be distinct and < n (bit %d = %d)", b, p);
       return false;
     }
     used |= 1ull << p;
@@ -157,12 +170,13 @@ int cqs_collapse(void* amps, int32_t n_bits, uint64_t mask, uint64_t value, floa
 }
 
 // Launches needing more than the default 64 KB of dynamic LDS raise the kernel's limit first.
-static int ensure_pass_lds(size_t shmem) {
-  static size_t granted = 64 * 1024;
-  if (shmem > granted) {
+static int ensure_pass_lds(int L, size_t shmem) {
+  static size_t granted[15] = {0};
+  if (shmem > 64 * 1024 && shmem > granted[L]) {
     CQS_REQUIRE(shmem <= 160 * 1024, "pass needs %zu bytes of LDS (> 160 KB)", shmem);
-    CQS_HIP(hipFuncSetAttribute((const void*)k_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    granted = shmem;
+    CQS_HIP(hipFuncSetAttribute((const void*)kPassKernels[L], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shmem));
+    granted[L] = shmem;
   }
   return CQS_OK;
 }
@@ -184,8 +198,9 @@ int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n
     const int mk = ps.max_k;
     const size_t stage = mk >= 3 ? (size_t(1) << (2 * mk)) : 0;
     const size_t shmem = ((size_t(1) << ps.L) + stage) * sizeof(float2);
-    if (int rc = ensure_pass_lds(shmem)) return rc;
-    hipLaunchKernelGGL(k_pass, dim3((unsigned)tiles), dim3(WG), shmem, s, (float2*)amps, P);
+    if (int rc = ensure_pass_lds(ps.L, shmem)) return rc;
+    hipLaunchKernelGGL(kPassKernels[ps.L], dim3((unsigned)tiles), dim3(threads_for_L(ps.L)), shmem, s,
+                       (float2*)amps, P);
     CQS_HIP(hipGetLastError());
   }
   return CQS_OK;

@@ -2,10 +2,11 @@
 //
 // A workgroup owns one tile: the 2^L amplitudes whose flat indices differ only in the L
 // "local" bit positions lpos[] (ascending).  Tile index j (0..2^L) maps to flat offset
-// deposit(j, lpos); the tile number maps to the remaining "outer" positions opos[].
-// Thread tid owns elements j = tid + e*WG for the load/store and all elementwise ops, so
-// elementwise ops need no barrier between them; dense ops are group-parallel and are
-// fenced by barriers.
+// deposit(j, lpos); the workgroup number maps to the remaining "outer" positions opos[].
+//
+// Thread tid of NT owns elements j = tid + e*NT (e < PER): it loads them from HBM into
+// registers, applies runs of elementwise ops (diagonals, pair phases, scales) there, and only
+// spills the tile to LDS around dense / block ops, which mix elements across threads.
 #pragma once
 
 #include "common.hpp"
@@ -17,7 +18,7 @@ struct TileProg {
   int32_t L;         // local bits
   int32_t nouter;    // n - L
   int32_t op_count;  // ops in this pass
-  int32_t max_k;     // largest dense k the LDS matrix stage was sized for
+  int32_t max_k;     // largest VALU dense k the LDS matrix stage was sized for
   float mscale;      // amplitude scale of the split-FP16 MFMA path (2^15 for normalised states)
   int32_t lpos[CQS_MAX_LOCAL];
   int32_t opos[CQS_MAX_BITS];
@@ -26,7 +27,9 @@ struct TileProg {
   const int32_t* aux;
 };
 
-// LDS layout of a tile kernel: [2^L tile amplitudes][4^max_k staged matrix entries if max_k >= 3]
+// Bit encoding of DIAG / PAIRPHASE positions: values < kOuter are tile-local indices, values
+// >= kOuter are kOuter + a global position outside the tile.
+constexpr int kOuter = 64;
 
 __device__ __forceinline__ bool outer_controls_ok(const cqs_op& op, uint64_t base) {
   const uint64_t m = ((uint64_t)op.co_mask_hi << 32) | op.co_mask_lo;
@@ -34,9 +37,10 @@ __device__ __forceinline__ bool outer_controls_ok(const cqs_op& op, uint64_t bas
   return (base & m) == v;
 }
 
-// Dense 2^K x 2^K matrix on tile bits op.t[0..K); M row-major, bit m of the matrix index is
-// tile bit op.t[m].  Each thread owns whole groups, so no barrier is needed inside.
-template <int K>
+// ---------------------------------------------------------------- dense ops on the LDS tile
+// 2^K x 2^K matrix on tile bits op.t[0..K); M row-major, bit m of the matrix index is tile bit
+// op.t[m].  Each thread owns whole groups, so no barrier is needed inside.
+template <int K, int NT>
 __device__ __forceinline__ void tile_dense(float2* tile, int L, const cqs_op& op, const float2* M) {
   constexpr int D = 1 << K;
   int ts[K];
@@ -61,7 +65,7 @@ __device__ __forceinline__ void tile_dense(float2* tile, int L, const cqs_op& op
   }
   const int G = (1 << L) >> K;
   const int cm = op.cl_mask, cv = op.cl_val;
-  for (int g = threadIdx.x; g < G; g += WG) {
+  for (int g = threadIdx.x; g < G; g += NT) {
     int j0 = g;
 #pragma unroll
     for (int s = 0; s < K; ++s) j0 = insert0(j0, ts[s]);
@@ -86,8 +90,9 @@ typedef float floatx16_t __attribute__((ext_vector_type(16)));
 // Y = U X on tile bits op.t[0..K) with v_mfma_f32_32x32x16_f16.  The tile is viewed as a
 // (2^K rows) x (2^(L-K) columns) complex matrix; in real-interleaved form (row 2r = Re, 2r+1 = Im)
 // it is R (2^(K+1) square) times X.  Each wave owns whole 32-column blocks, so reads and writes of
-// different waves never overlap and the block needs no barrier inside.
-template <int K>
+// different waves never overlap and the block needs no barrier inside.  X is split as
+// hi + lo FP16 after scaling by `scale`; U's split comes from the host in A-operand order.
+template <int K, int NT>
 __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_op& op, const half8_t* A,
                                                 float scale, float inv_scale) {
   constexpr int D = 1 << K;
@@ -98,7 +103,6 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_o
   int tmask = 0;
 #pragma unroll
   for (int m = 0; m < K; ++m) tmask |= 1 << op.t[m];
-  // tile offset of complex row r
   auto rowoff = [&](int r) {
     int j = 0;
 #pragma unroll
@@ -107,7 +111,7 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_o
   };
   const int ncol = 1 << (L - K);
   const int ncb = (ncol + 31) >> 5;
-  for (int cb = wave; cb < ncb; cb += WG / 64) {
+  for (int cb = wave; cb < ncb; cb += NT / 64) {
     const int col = cb * 32 + c32;
     const bool valid = col < ncol;
     int jc = 0;
@@ -158,109 +162,236 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_o
   }
 }
 
-// Global flat index of this thread's e-th element.
-__device__ __forceinline__ uint64_t elem_index(const TileProg& P, uint64_t base, uint64_t off_lo,
-                                               int nlo, int e) {
-  return base + off_lo + deposit((uint32_t)e, P.lpos + nlo, P.L - nlo);
-}
-
-__device__ __forceinline__ void tile_diag(float2* tile, const TileProg& P, const cqs_op& op,
-                                          uint64_t base, uint64_t off_lo, int nlo, int per) {
-  const float2* Dg = P.mats + op.mat;
-  const int k = op.k;
-  for (int e = 0; e < per; ++e) {
-    const uint64_t gi = elem_index(P, base, off_lo, nlo, e);
-    int idx = 0;
-    for (int m = 0; m < k; ++m) idx |= (int)((gi >> op.t[m]) & 1ull) << m;
-    const int j = threadIdx.x + e * WG;
-    tile[j] = cmul(tile[j], Dg[idx]);
-  }
-}
-
-__device__ __forceinline__ void tile_pairphase(float2* tile, const TileProg& P, const cqs_op& op,
-                                               uint64_t base, uint64_t off_lo, int nlo, int per) {
-  const float2* ph = P.mats + op.mat;
-  const int32_t* pr = P.aux + op.t[0];
-  const int np = op.k;
-  for (int e = 0; e < per; ++e) {
-    const uint64_t gi = elem_index(P, base, off_lo, nlo, e);
-    const int j = threadIdx.x + e * WG;
-    float2 v = tile[j];
-    for (int p = 0; p < np; ++p) {
-      if ((gi >> pr[2 * p]) & (gi >> pr[2 * p + 1]) & 1ull) v = cmul(v, ph[p]);
-    }
-    tile[j] = v;
-  }
-}
-
-__device__ __forceinline__ void tile_scale(float2* tile, const TileProg& P, const cqs_op& op, int per) {
-  const float2 s = P.mats[op.mat];
-  for (int e = 0; e < per; ++e) {
-    const int j = threadIdx.x + e * WG;
-    tile[j] = cmul(tile[j], s);
-  }
-}
-
-// Stage a matrix into LDS (for K >= 3 dense ops).  Caller fences before and after.
+template <int NT>
 __device__ __forceinline__ void stage_matrix(float2* dst, const float2* src, int count) {
-  for (int i = threadIdx.x; i < count; i += WG) dst[i] = src[i];
+  for (int i = threadIdx.x; i < count; i += NT) dst[i] = src[i];
 }
 
-__device__ __forceinline__ void run_dense(float2* tile, float2* mstage, int L, int max_k, const cqs_op& op,
-                                          const float2* mats) {
-  const float2* M = mats + op.mat;
-  if (op.k > max_k && op.k >= 3) return;  // the stage was not sized for it (host bug)
-  // k = 3 stages its 8x8 matrix in LDS; k <= 2 reads it through the scalar cache
+// Dense (VALU, k <= 3) or block (MFMA, k = 4, 5) op on the LDS tile.  Caller fences before.
+template <int NT>
+__device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, const TileProg& P, const cqs_op& op) {
+  if (op.kind == CQS_OP_BLOCK) {
+    const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat);
+    if (op.k == 4)
+      tile_block_mfma<4, NT>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
+    else if (op.k == 5)
+      tile_block_mfma<5, NT>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
+    return;
+  }
+  const float2* M = P.mats + op.mat;
   switch (op.k) {
-    case 1: tile_dense<1>(tile, L, op, M); break;
-    case 2: tile_dense<2>(tile, L, op, M); break;
+    case 1: tile_dense<1, NT>(tile, P.L, op, M); break;
+    case 2: tile_dense<2, NT>(tile, P.L, op, M); break;
     case 3:
-      stage_matrix(mstage, M, 64);
+      if (P.max_k < 3) return;  // the stage was not sized for it (host bug)
+      stage_matrix<NT>(mstage, M, 64);
       __syncthreads();
-      tile_dense<3>(tile, L, op, mstage);
+      tile_dense<3, NT>(tile, P.L, op, mstage);
       break;
-    default: break;  // wider matrices run as CQS_OP_BLOCK on MFMA
+    default: break;
   }
 }
 
-// Runs the program of P on one tile that is already in LDS.  The caller has loaded the
-// tile with the element mapping (no barrier yet).  Returns with the tile complete in LDS in
-// a state where the caller must fence before re-reading it with the element mapping iff
-// the function returns false.
-__device__ __forceinline__ bool run_program(float2* tile, float2* mstage, const TileProg& P,
-                                            const cqs_op* ops, int op_count, uint64_t base,
-                                            uint64_t off_lo, int nlo, int per) {
-  bool elem_mode = true;
-  for (int i = 0; i < op_count; ++i) {
-    const cqs_op& op = ops[i];
-    if (!outer_controls_ok(op, base)) continue;
-    if (op.kind == CQS_OP_DENSE) {
-      __syncthreads();
-      run_dense(tile, mstage, P.L, P.max_k, op, P.mats);
-      elem_mode = false;
-    } else if (op.kind == CQS_OP_BLOCK) {
-      __syncthreads();
-      const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat);
-      if (op.k == 4)
-        tile_block_mfma<4>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
-      else if (op.k == 5)
-        tile_block_mfma<5>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
-      elem_mode = false;
-    } else {
-      if (!elem_mode) {
-        __syncthreads();
-        elem_mode = true;
-      }
-      if (op.kind == CQS_OP_DIAG) {
-        tile_diag(tile, P, op, base, off_lo, nlo, per);
-      } else if (op.kind == CQS_OP_PAIRPHASE) {
-        tile_pairphase(tile, P, op, base, off_lo, nlo, per);
-      } else if (op.kind == CQS_OP_SCALE) {
-        tile_scale(tile, P, op, per);
-      }
+// ---------------------------------------------------------------- elementwise ops in registers
+// v[e] holds tile element j = threadIdx.x + e*NT.
+template <int PER, int NT>
+__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const TileProg& P, const cqs_op& op, uint64_t base) {
+  constexpr int TB = __builtin_ctz(NT);
+  const int tid = threadIdx.x;
+  int fixed = 0;   // index bits known for the whole thread
+  int emask = 0;   // index bits that come from element bits
+  for (int m = 0; m < op.k; ++m) {
+    const int t = op.t[m];
+    if (t >= kOuter)
+      fixed |= (int)((base >> (t - kOuter)) & 1ull) << m;
+    else if (t < TB)
+      fixed |= ((tid >> t) & 1) << m;
+    else
+      emask |= 1 << m;
+  }
+  const float2* Dg = P.mats + op.mat;
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    int idx = fixed;
+    for (int m = 0; m < op.k; ++m)
+      if ((emask >> m) & 1) idx |= ((e >> (op.t[m] - TB)) & 1) << m;
+    v[e] = cmul(v[e], Dg[idx]);
+  }
+}
+
+template <int PER, int NT>
+__device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const TileProg& P, const cqs_op& op, uint64_t base) {
+  const int32_t* pr = P.aux + op.t[0];
+  const float2* ph = P.mats + op.mat;
+  const int tid = threadIdx.x;
+  for (int p = 0; p < op.k; ++p) {
+    const int a = pr[2 * p], b = pr[2 * p + 1];
+    int need = 0;
+    bool en = true;
+    if (a >= kOuter)
+      en = en && ((base >> (a - kOuter)) & 1ull);
+    else
+      need |= 1 << a;
+    if (b >= kOuter)
+      en = en && ((base >> (b - kOuter)) & 1ull);
+    else
+      need |= 1 << b;
+    if (!en) continue;
+    const float2 f = ph[p];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int j = tid + e * NT;
+      if ((j & need) == need) v[e] = cmul(v[e], f);
     }
   }
-  return elem_mode;
+}
+
+template <int PER>
+__device__ __forceinline__ void ew_scale(float2 (&v)[PER], const TileProg& P, const cqs_op& op) {
+  const float2 s = P.mats[op.mat];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], s);
+}
+
+// 1- and 2-qubit matrices on element bits (tile bits >= log2(NT)) stay inside one thread:
+// element e of the thread pairs with e ^ 2^R.
+template <int PER, int R>
+__device__ __forceinline__ void reg_dense1(float2 (&v)[PER], const float2* M) {
+  const float2 m00 = M[0], m01 = M[1], m10 = M[2], m11 = M[3];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    if ((e >> R) & 1) continue;
+    const int e1 = e | (1 << R);
+    const float2 a = v[e], b = v[e1];
+    v[e] = cfma(m01, b, cmul(m00, a));
+    v[e1] = cfma(m11, b, cmul(m10, a));
+  }
+}
+
+template <int PER, int R0, int R1>
+__device__ __forceinline__ void reg_dense2(float2 (&v)[PER], const float2* M) {
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    if (((e >> R0) & 1) || ((e >> R1) & 1)) continue;
+    const int idx[4] = {e, e | (1 << R0), e | (1 << R1), e | (1 << R0) | (1 << R1)};
+    float2 x[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = v[idx[c]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float2 acc = cmul(M[r * 4], x[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) acc = cfma(M[r * 4 + c], x[c], acc);
+      v[idx[r]] = acc;
+    }
+  }
+}
+
+template <int PER, int NT>
+__device__ __forceinline__ bool is_register_dense(const cqs_op& op) {
+  constexpr int TB = __builtin_ctz(NT);
+  if (op.kind != CQS_OP_DENSE || op.k > 2 || op.cl_mask != 0 || PER < 2) return false;
+  for (int m = 0; m < op.k; ++m)
+    if (op.t[m] < TB) return false;
+  return true;
+}
+
+template <int PER, int NT>
+__device__ __forceinline__ void reg_dense(float2 (&v)[PER], const TileProg& P, const cqs_op& op) {
+  constexpr int TB = __builtin_ctz(NT);
+  constexpr int NR = __builtin_ctz(PER);  // element bits
+  const float2* M = P.mats + op.mat;
+  const int r0 = op.t[0] - TB;
+  if (op.k == 1) {
+    switch (r0) {
+      case 0: if constexpr (NR > 0) reg_dense1<PER, 0>(v, M); break;
+      case 1: if constexpr (NR > 1) reg_dense1<PER, 1>(v, M); break;
+      case 2: if constexpr (NR > 2) reg_dense1<PER, 2>(v, M); break;
+      case 3: if constexpr (NR > 3) reg_dense1<PER, 3>(v, M); break;
+      default: break;
+    }
+    return;
+  }
+  const int r1 = op.t[1] - TB;
+#define CQS_R2(A, B) \
+  if (r0 == A && r1 == B) { if constexpr (NR > (A > B ? A : B)) reg_dense2<PER, A, B>(v, M); return; }
+  CQS_R2(0, 1) CQS_R2(0, 2) CQS_R2(0, 3) CQS_R2(1, 0) CQS_R2(1, 2) CQS_R2(1, 3)
+  CQS_R2(2, 0) CQS_R2(2, 1) CQS_R2(2, 3) CQS_R2(3, 0) CQS_R2(3, 1) CQS_R2(3, 2)
+#undef CQS_R2
+}
+
+template <int PER, int NT>
+__device__ __forceinline__ bool is_mixing(const cqs_op& op) {
+  return (op.kind == CQS_OP_DENSE && !is_register_dense<PER, NT>(op)) || op.kind == CQS_OP_BLOCK;
+}
+
+// Runs the pass program on the tile, which the caller has written to LDS (no barrier yet).
+// Mixing ops work LDS to LDS; each run of elementwise ops loads this thread's elements into
+// registers once, applies the whole run and writes them back.  Returns with the tile in LDS and
+// all threads synchronised.
+template <int PER, int NT>
+__device__ __forceinline__ void run_program(float2* tile, float2* mstage, const TileProg& P, uint64_t base,
+                                            bool act) {
+  int i = 0;
+  while (i < P.op_count) {
+    if (is_mixing<PER, NT>(P.ops[i])) {
+      __syncthreads();
+      if (outer_controls_ok(P.ops[i], base)) run_mixing_op<NT>(tile, mstage, P, P.ops[i]);
+      ++i;
+      continue;
+    }
+    __syncthreads();
+    float2 v[PER];
+    if (act) {
+#pragma unroll
+      for (int e = 0; e < PER; ++e) v[e] = tile[threadIdx.x + e * NT];
+    }
+    for (; i < P.op_count && !is_mixing<PER, NT>(P.ops[i]); ++i) {
+      const cqs_op& op = P.ops[i];
+      if (!outer_controls_ok(op, base)) continue;
+      if (op.kind == CQS_OP_DENSE)
+        reg_dense<PER, NT>(v, P, op);
+      else if (op.kind == CQS_OP_DIAG)
+        ew_diag<PER, NT>(v, P, op, base);
+      else if (op.kind == CQS_OP_PAIRPHASE)
+        ew_pairs<PER, NT>(v, P, op, base);
+      else if (op.kind == CQS_OP_SCALE)
+        ew_scale<PER>(v, P, op);
+    }
+    if (act) {
+#pragma unroll
+      for (int e = 0; e < PER; ++e) tile[threadIdx.x + e * NT] = v[e];
+    }
+  }
+  __syncthreads();
+}
+
+// Threads per workgroup for a tile of 2^L amplitudes (L <= 14): 256 threads up to L = 12 (threads
+// beyond 2^L idle when L < 8), then 16 elements per thread.  NT is always a multiple of 64, so the
+// wave loops of the MFMA path always advance.
+template <int L>
+struct TileShape {
+  static constexpr int NT = L <= 12 ? 256 : (1 << L) / 16;
+  static constexpr int PER = L <= 8 ? 1 : (1 << L) / NT;
+  static constexpr int TB = __builtin_ctz(NT);
+  static constexpr int TBL = L < TB ? L : TB;  // thread bits that are tile bits
+};
+
+// Flat offsets of this thread's elements: off_lo (thread bits) + off_hi[e] (element bits).
+template <int L>
+__device__ __forceinline__ void tile_offsets(const TileProg& P, uint64_t& off_lo,
+                                             uint64_t (&off_hi)[TileShape<L>::PER]) {
+  constexpr int TBL = TileShape<L>::TBL, PER = TileShape<L>::PER;
+  off_lo = 0;
+#pragma unroll
+  for (int b = 0; b < TBL; ++b) off_lo |= (uint64_t)((threadIdx.x >> b) & 1) << P.lpos[b];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int b = 0; b < L - TBL; ++b) o |= (uint64_t)((e >> b) & 1) << P.lpos[TBL + b];
+    off_hi[e] = o;
+  }
 }
 
 }  // namespace cqs

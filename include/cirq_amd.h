@@ -81,7 +81,9 @@ typedef struct cqs_op {
 
 /*
  * One pass: every workgroup gathers the 2^L amplitudes whose flat indices differ only in
- * the L bit positions lpos[0..L-1] (ascending), runs ops[op_begin .. op_begin+op_count)
+ * the L distinct bit positions lpos[0..L-1] (tile index bit b <-> flat bit lpos[b]; the lowest
+ * tile bits map to threads, the highest log2(2^L / threads) bits to registers of one thread,
+ * where 1- and 2-bit DENSE ops run without LDS), runs ops[op_begin .. op_begin+op_count)
  * on them in LDS and writes them back.  One pass reads and writes the state once.
  */
 typedef struct cqs_pass {

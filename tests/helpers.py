@@ -116,6 +116,11 @@ def _deposit(x, pos):
     return r
 
 
+def _glob(code: int, lpos) -> int:
+    """Global bit of a DIAG / PAIRPHASE position code (tile-local index or 64 + global bit)."""
+    return code - 64 if code >= 64 else lpos[code]
+
+
 def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) -> np.ndarray:
     """Executes the packed program on a copy of `state` (flat complex array)."""
     st = np.array(state, dtype=np.complex128)
@@ -161,7 +166,7 @@ def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) ->
                 tile[:, idx] = np.where(ok, out, V)
             elif kind == _lib.OP_DIAG:
                 k = int(r["k"])
-                t = [int(x) for x in r["t"][:k]]
+                t = [_glob(int(x), lpos) for x in r["t"][:k]]
                 Dv = mats[int(r["mat"]) : int(r["mat"]) + (1 << k)]
                 di = np.zeros_like(gi)
                 for m, b in enumerate(t):
@@ -172,7 +177,7 @@ def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) ->
                 a0 = int(r["t"][0])
                 ph = mats[int(r["mat"]) : int(r["mat"]) + k]
                 for p in range(k):
-                    a, b = int(aux[a0 + 2 * p]), int(aux[a0 + 2 * p + 1])
+                    a, b = _glob(int(aux[a0 + 2 * p]), lpos), _glob(int(aux[a0 + 2 * p + 1]), lpos)
                     both = ((gi >> a) & (gi >> b) & 1).astype(bool)
                     tile = np.where(both & tile_ok[:, None], tile * ph[p], tile)
             elif kind == _lib.OP_SCALE:

@@ -181,7 +181,7 @@ def test_schedule_respects_tile_and_coalescing():
     packed = _compile_sv(c, qs, 10, 4)
     for ps in packed.pass_arr:
         lp = list(ps["lpos"][: ps["L"]])
-        assert lp == sorted(lp) and lp[:4] == [0, 1, 2, 3]
+        assert len(set(lp)) == len(lp) and lp[:4] == [0, 1, 2, 3]
     # fewer passes than gates: the point of the tile scheduler
     n_gates = sum(1 for _ in c.all_operations())
     assert packed.n_passes < n_gates / 10
