@@ -41,7 +41,53 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-qubits", type=int, default=26)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--adjoint-qubits", type=int, default=20)
+    ap.add_argument("--adjoint-layers", type=int, default=10)
+    ap.add_argument("--adjoint-reps", type=int, default=5)
     return ap.parse_args()
+
+
+def adjoint_leg(args):
+    """Hardware-efficient ansatz (Ry, Rz per qubit + CZ ladder per layer; 20 qubits x 10 layers =
+    400 symbols), H = sum_i Z_i + 32 random 4-local strings: gradients/s of one adjoint sweep vs
+    the GPU parameter-shift rule (2 forward simulations + <H> per symbol), and their max
+    difference."""
+    import torch
+
+    from cirq_amd import generators
+    from cirq_amd.adjoint import AdjointGradient
+
+    circuit, syms, vals = generators.hardware_efficient_ansatz(args.adjoint_qubits, args.adjoint_layers, seed=0)
+    psum = generators.random_pauli_sum(sorted(circuit.all_qubits()), 32, 4, seed=0)
+    ag = AdjointGradient(circuit, psum)
+    ag.value_and_grad(vals)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.adjoint_reps):
+        v, g = ag.value_and_grad(vals)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / args.adjoint_reps
+    ag.parameter_shift(vals)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v2, g2 = ag.parameter_shift(vals)
+    torch.cuda.synchronize()
+    t_ps = time.perf_counter() - t0
+    G = len(syms)
+    return {
+        "workload": f"HEA {args.adjoint_qubits} qubits x {args.adjoint_layers} layers, {G} symbols, "
+        f"{len(list(psum))} Pauli terms",
+        "adjoint_ms": t_adj * 1e3,
+        "adjoint_grads_per_s": G / t_adj,
+        "param_shift_ms": t_ps * 1e3,
+        "param_shift_grads_per_s": G / t_ps,
+        "speedup": t_ps / t_adj,
+        "max_abs_grad_diff": float(np.abs(g - g2).max()),
+        "value_diff": abs(v - v2),
+        "forward_passes": int(ag.prog.fwd.n_passes),
+        "reverse_passes": int(ag.prog.rev.n_passes),
+    }
 
 
 def cpu_baseline(args, circuit_ops_fn, n_gpu: int, layers: int, ops_per_layer: float):
@@ -198,6 +244,8 @@ def main():
             "sample_check": int(bits.shape[0]),
         },
     }
+    if not args.no_adjoint:
+        out["extra"]["adjoint"] = adjoint_leg(args)
     if not args.no_cpu_baseline:
         from tests.helpers import to_oracle_ops
 

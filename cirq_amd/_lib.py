@@ -17,6 +17,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libcirq_amd.so"
 
 CQS_OK, CQS_ERR_ARG, CQS_ERR_HIP, CQS_ERR_OOM, CQS_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 OP_DENSE, OP_DIAG, OP_PAIRPHASE, OP_SCALE, OP_BLOCK = 1, 2, 3, 4, 5
+OP_GRAD_DENSE, OP_GRAD_DIAG = 6, 7
+SEL_BOTH, SEL_PSI, SEL_PHI = 0, 1, 2
 MAX_LOCAL = 16
 
 # numpy mirror of cqs_op (16 x 32-bit words)
@@ -74,6 +76,8 @@ _SIGNATURES = {
     "cqs_sample_draw": ([_vp, _vp, _i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp], ctypes.c_int),
     "cqs_marginal_probs": ([_vp, _i32, _i32, _vp, _i32, _vp, _vp], ctypes.c_int),
     "cqs_indices_to_bits": ([_vp, _i64, _i32, _vp, _vp], ctypes.c_int),
+    "cqs_adjoint_passes": ([_vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp], ctypes.c_int),
+    "cqs_grad_reduce": ([_vp, _i32, _i64, _vp, _vp], ctypes.c_int),
 }
 
 

@@ -1,0 +1,302 @@
+"""Gradients of expectation values: one adjoint sweep on the GPU, plus a parameter-shift baseline.
+
+Cirq has no gradient machinery: a Cirq user differentiates <psi(theta)|H|psi(theta)> by calling
+simulate_expectation_values_sweep (cirq-core/cirq/sim/simulator.py:349-371) at shifted parameter
+values.  AdjointGradient instead returns the value and d<H>/d(symbol) for every symbol from one
+forward simulation and one reverse sweep (Jones & Gacon 2020, arXiv:2009.02823): the forward state
+psi and phi = H psi are walked back gate by gate, un-computing each gate with its inverse rather than
+storing intermediate states, and every parameterised gate contributes 2 Re <phi|dU/dt|psi>.
+
+ExpectationLayer wraps it in a torch.autograd.Function.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import sympy
+import torch
+
+from . import _lib, engine
+from ._lib import check, lib, ptr
+from .gates import is_parameterized
+from .paulis import PauliSum, pauli_masks
+from .qubits import QubitOrder
+from .study import ParamResolver
+from .tape import (
+    LOp,
+    Lowering,
+    Pass,
+    PackedProgram,
+    ParamEvaluator,
+    ParamRef,
+    compile_ops,
+    embed,
+    fill_param_mats,
+    fuse_blocks,
+    fuse_single_bit,
+    order_tile_bits,
+    schedule,
+)
+
+
+def _dagger(kind: str, x):
+    """Inverse payload of an op: conjugate transpose (dense) or conjugate (diagonal / phases)."""
+    if x is None:
+        return None
+    if kind in ("dense", "block"):
+        return np.conj(np.swapaxes(x, -1, -2))
+    return np.conj(x)
+
+
+def _inverse(op: LOp, sel: int) -> LOp:
+    if op.kind == "scale" and abs(abs(complex(op.mat)) - 1) > 1e-9:
+        raise ValueError("adjoint gradients need a unitary circuit (non-unit scale factor)")
+    ref = op.pref
+    if ref is not None:
+        base, kind = ref.transform, op.kind
+        ref = ParamRef(ref.gate, lambda b, base=base, kind=kind: _dagger(kind, base(b)), ref.exprs, ref.slot)
+    return LOp(op.kind, op.targets, _dagger(op.kind, op.mat), op.controls, ref, sel)
+
+
+def _grad_op(op: LOp, k: int, slot: int) -> LOp:
+    """d(op)/d(parameter k) as a GRAD op on the op's bits.
+
+    The lowering's payload transforms are affine in the gate matrix (a controlled diagonal puts
+    constant ones outside the controlled block), so the derivative payload is
+    transform(dM) - transform(0).  Controls of a dense op are folded into its matrix the same way:
+    the blocks where the controls fail do not depend on the parameter."""
+    ref = op.pref
+    base = ref.transform
+
+    def linear(b, base=base):
+        return base(b) - base(np.zeros_like(b))
+
+    if op.kind == "diag":
+        return LOp("grad_diag", op.targets, None, (), ParamRef(ref.gate, linear, ref.exprs, ref.slot, k), 0, slot)
+    if op.kind != "dense":
+        raise TypeError(f"cannot differentiate a {op.kind} op")
+    if not op.controls:
+        return LOp("grad_dense", op.targets, None, (), ParamRef(ref.gate, linear, ref.exprs, ref.slot, k), 0, slot)
+    tb = list(op.targets)
+    ctrls = tuple(op.controls)
+    bits = tb + [b for b, _ in ctrls]
+
+    def controlled(b, linear=linear, tb=tb, ctrls=ctrls, bits=bits):
+        d = linear(b)
+        zero = embed(np.zeros_like(d[0]), tb, ctrls, bits)  # identity where the controls fail
+        return np.stack([embed(x, tb, ctrls, bits) - zero for x in d])
+
+    return LOp("grad_dense", tuple(bits), None, (), ParamRef(ref.gate, controlled, ref.exprs, ref.slot, k), 0, slot)
+
+
+def shift_rule_ok(gate, k: int) -> bool:
+    """True when parameter k of `gate` enters as exp(i pi t h) with two eigenvalues h one apart,
+    the case where the two-term shift rule with shifts +-1/2 is exact."""
+    from .gates import EigenGate, PhasedXPowGate
+
+    if isinstance(gate, EigenGate):
+        hs = sorted({round(float(h), 12) for h, _ in gate._eigen_components()})
+        return len(hs) == 2 and abs(hs[1] - hs[0] - 1) < 1e-12
+    return isinstance(gate, PhasedXPowGate) and k == 1
+
+
+class AdjointProgram:
+    """Host side of an adjoint gradient: the forward program, the reversed program and the map
+    from gradient slots to parameter expressions.  Needs no GPU.
+
+    The reversed program walks the lowered ops backwards.  A fixed op U becomes U^dagger on both
+    states; a parameterised op becomes [U^dagger on psi][dU/dt_k for each symbolic parameter
+    t_k][U^dagger on phi].  Ops on disjoint bits commute, so the scheduler may move whole ops
+    across each other; each triple keeps its order because its members share their bits.
+    """
+
+    def __init__(self, circuit, observable, qubit_order=QubitOrder.DEFAULT, *, tile_bits: int | None = None,
+                 adjoint_tile_bits: int | None = None, coalesce_bits: int = 4, mfma_blocks: bool = True):
+        qubits = QubitOrder.as_qubit_order(qubit_order).order_for(circuit.all_qubits())
+        self.qubits = tuple(qubits)
+        n = len(qubits)
+        self.n = n
+        low = Lowering(qubits, dm=False)
+        low.lower_circuit(circuit)
+        if low.measurements:
+            raise ValueError("adjoint gradients need a circuit without measurements")
+        qmap = {q: i for i, q in enumerate(qubits)}
+        psum = PauliSum.wrap(observable)
+        for t in psum:
+            for q in t.qubits:
+                if q not in qmap:
+                    raise KeyError(q)
+        self.terms = [pauli_masks(t, qmap, n) for t in psum]
+        self.evaluator = ParamEvaluator(low.param_refs)
+        self.symbols = list(self.evaluator.symbols)
+        L = min(tile_bits if tile_bits is not None else 12, n)
+        La = min(adjoint_tile_bits if adjoint_tile_bits is not None else 12, n)
+        self.fwd = compile_ops(low.ops, n, L=L, coalesce=coalesce_bits, blocks=mfma_blocks)
+        rev: list[LOp] = []
+        self.grad_slots: list[tuple[ParamRef, int]] = []  # slot -> (forward op's ref, parameter index)
+        for op in reversed(low.ops):
+            if op.fixed:
+                rev.append(_inverse(op, _lib.SEL_BOTH))
+                continue
+            rev.append(_inverse(op, _lib.SEL_PSI))
+            for k, e in enumerate(op.pref.exprs):
+                if is_parameterized(e):
+                    slot = len(self.grad_slots)
+                    self.grad_slots.append((op.pref, k))
+                    rev.append(_grad_op(op, k, slot))
+            rev.append(_inverse(op, _lib.SEL_PHI))
+        rev = fuse_single_bit(rev)
+        passes = schedule(rev, n, La, coalesce_bits)
+        if mfma_blocks:
+            passes = [Pass(p.L, p.lpos, fuse_blocks(p.ops, set(p.lpos), p.L)) for p in passes]
+        passes = [order_tile_bits(p, coalesce_bits) for p in passes]
+        self.rev = PackedProgram(passes, n)
+        self.tile_bits_rev = La
+        self.partial_stride = 1 << (n - La)
+        self._slot_exprs = [ref.exprs[k] for ref, k in self.grad_slots]
+
+    @property
+    def n_slots(self) -> int:
+        return len(self.grad_slots)
+
+    def symbol_values(self, values) -> np.ndarray:
+        """(n_symbols, 1) array from a resolver/dict or a vector in `symbols` order."""
+        if isinstance(values, (ParamResolver, dict)):
+            return self.evaluator.symbol_values([ParamResolver(values)])
+        v = np.asarray(values, dtype=np.float64).reshape(-1)
+        if v.size != len(self.symbols):
+            raise ValueError(f"expected {len(self.symbols)} symbol values, got {v.size}")
+        return v.reshape(-1, 1).copy()
+
+    def mats(self, symvals: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """Matrix buffers of the forward and reversed programs at the given symbol values."""
+        return (fill_param_mats(self.fwd, self.evaluator, symvals)[0],
+                fill_param_mats(self.rev, self.evaluator, symvals)[0])
+
+    def chain(self, gs: np.ndarray, symvals: np.ndarray) -> np.ndarray:
+        """d<H>/d symbol from per-slot derivatives d<H>/d t_slot."""
+        grad = np.zeros(len(self.symbols))
+        for i, s in enumerate(self.symbols):
+            for g, e in zip(gs, self._slot_exprs):
+                d = self.evaluator.dvalues(e, s, symvals)
+                if d is not None:
+                    grad[i] += g * d[0]
+        return grad
+
+    def shifted_mats(self, symvals: np.ndarray) -> np.ndarray:
+        """(2 * n_slots, mats) forward matrix buffers with parameter `slot` of one gate occurrence
+        shifted by +1/2 (even rows) and -1/2 (odd rows) in exponent units."""
+        base = fill_param_mats(self.fwd, self.evaluator, symvals)[0]
+        out = np.broadcast_to(base, (2 * self.n_slots, base.size)).copy()
+        for slot, (ref, k) in enumerate(self.grad_slots):
+            if not shift_rule_ok(ref.gate, k):
+                raise ValueError(f"the two-term shift rule does not hold for parameter {k} of {ref.gate!r}")
+            vals = [self.evaluator.values(e, symvals) for e in ref.exprs]
+            vv = [np.repeat(v, 2) for v in vals]
+            vv[k] = vv[k] + np.array([0.5, -0.5])
+            mb = ref.gate._matrix_batch(tuple(vv))
+            for off, r, size in self.fwd.param_entries:
+                if r.slot == ref.slot:  # every lowered op of this gate occurrence
+                    out[2 * slot : 2 * slot + 2, off : off + size] = r.transform(mb).reshape(2, -1)
+        return out
+
+
+class AdjointGradient:
+    """<H> and d<H>/d(symbol) of a parameterised unitary circuit on one GPU.
+
+    circuit: sympy-parameterised gates, no measurements; observable: PauliSumLike.
+    """
+
+    def __init__(self, circuit, observable, qubit_order=QubitOrder.DEFAULT, **kw):
+        self.device = engine.require_gpu()
+        self.prog = AdjointProgram(circuit, observable, qubit_order, **kw)
+        self.symbols = self.prog.symbols
+        self.n = self.prog.n
+        self.dev_fwd = engine.DeviceProgram(self.prog.fwd, self.device)
+        self.dev_rev = engine.DeviceProgram(self.prog.rev, self.device)
+        self.ws = engine.Workspace(self.device)
+        self.psi = engine.new_state(self.n, self.device)
+        self.phi = torch.empty_like(self.psi)
+        G = self.prog.n_slots
+        self.partial = torch.zeros(max(G, 1) * self.prog.partial_stride, dtype=torch.float64, device=self.device)
+        self.gslot = torch.zeros(max(G, 1), dtype=torch.float64, device=self.device)
+
+    def _forward(self):
+        check(lib().cqs_init_basis(ptr(self.psi), self.n, 0, engine.stream_handle()), "cqs_init_basis")
+        self.dev_fwd.run(self.psi)
+
+    def value_and_grad(self, values) -> tuple[float, np.ndarray]:
+        """(<H>, d<H>/d symbol in `symbols` order) at the given symbol values."""
+        prog, n = self.prog, self.n
+        symvals = prog.symbol_values(values)
+        mf, mr = prog.mats(symvals)
+        self.dev_fwd.set_mats(mf)
+        self.dev_rev.set_mats(mr)
+        self._forward()
+        ev = engine.expect_terms(self.psi, n, False, prog.terms, self.ws).sum()
+        engine.apply_paulisum(self.psi, self.phi, n, prog.terms)
+        G = prog.n_slots
+        rv = prog.rev
+        check(
+            lib().cqs_adjoint_passes(
+                ptr(self.psi), ptr(self.phi), n, rv.pass_arr.ctypes.data, int(rv.n_passes), ptr(self.dev_rev.ops),
+                ptr(self.dev_rev.mats), ptr(self.dev_rev.aux), ptr(self.partial), prog.partial_stride,
+                engine.stream_handle(),
+            ),
+            "cqs_adjoint_passes",
+        )
+        check(lib().cqs_grad_reduce(ptr(self.partial), G, prog.partial_stride, ptr(self.gslot), engine.stream_handle()),
+              "cqs_grad_reduce")
+        gs = self.gslot.cpu().numpy()[:G]
+        return float(ev.real.item()), prog.chain(gs, symvals)
+
+    def parameter_shift(self, values) -> tuple[float, np.ndarray]:
+        """The same gradient by the two-term shift rule: two forward simulations and <H> per
+        symbolic parameter of every gate occurrence, all matrix buffers resident on the device
+        and the expectation values read back once."""
+        prog, n = self.prog, self.n
+        symvals = prog.symbol_values(values)
+        shifted = prog.shifted_mats(symvals)
+        base = prog.mats(symvals)[0]
+        rows = np.vstack([shifted, base[None]])
+        width = -(-rows.shape[1] // 32) * 32  # keeps every row's MFMA operands 16-byte aligned
+        padded = np.zeros((rows.shape[0], width), dtype=np.complex64)
+        padded[:, : rows.shape[1]] = rows
+        allm = torch.from_numpy(padded.view(np.float32)).to(self.device)
+        own = self.dev_fwd.mats
+        evs = []
+        try:
+            for r in range(allm.shape[0]):
+                self.dev_fwd.set_mats_device(allm[r])
+                self._forward()
+                evs.append(engine.expect_terms(self.psi, n, False, prog.terms, self.ws).sum().real)
+            ev = torch.stack(evs).cpu().numpy()
+        finally:
+            self.dev_fwd.set_mats_device(own)
+        gs = (np.pi / 2) * (ev[0:-1:2] - ev[1:-1:2])
+        return float(ev[-1]), prog.chain(gs, symvals)
+
+
+class _ExpectationFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, values: torch.Tensor, grad_engine: AdjointGradient):
+        value, grad = grad_engine.value_and_grad(values.detach().cpu().numpy())
+        ctx.save_for_backward(torch.as_tensor(grad, dtype=values.dtype, device=values.device))
+        return torch.tensor(value, dtype=values.dtype, device=values.device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (grad,) = ctx.saved_tensors
+        return gout * grad, None
+
+
+class ExpectationLayer(torch.nn.Module):
+    """torch module: symbol values (1-D tensor, in `.symbols` order) -> <H>, differentiable."""
+
+    def __init__(self, circuit, observable, qubit_order=QubitOrder.DEFAULT, **kw):
+        super().__init__()
+        self.engine = AdjointGradient(circuit, observable, qubit_order, **kw)
+        self.symbols = self.engine.symbols
+
+    def forward(self, values: torch.Tensor) -> torch.Tensor:
+        return _ExpectationFn.apply(values, self.engine)

@@ -39,6 +39,7 @@ class ParamRef:
     transform: Callable
     exprs: tuple
     slot: int = -1
+    deriv: int | None = None  # evaluate d(matrix)/d(parameter `deriv`) instead of the matrix
 
 
 @dataclasses.dataclass
@@ -48,6 +49,8 @@ class LOp:
     mat: np.ndarray | None  # dense: (d, d); diag: (d,); pair: scalar phase; scale: scalar
     controls: tuple = ()  # ((bit, value), ...) for dense ops
     pref: ParamRef | None = None
+    sel: int = 0  # adjoint programs: 0 both states, 1 psi only, 2 phi only
+    slot: int = -1  # adjoint GRAD ops: gradient slot
 
     @property
     def bits(self) -> frozenset:
@@ -55,7 +58,7 @@ class LOp:
 
     @property
     def fixed(self) -> bool:
-        return self.pref is None
+        return self.pref is None and self.sel == 0 and not self.kind.startswith("grad")
 
 
 @dataclasses.dataclass
@@ -285,11 +288,12 @@ class Lowering:
         self.ops.append(LOp("dense", cols + rows, sop))
 
 
-def _sub_ref(ref: ParamRef | None, transform):
+def _sub_ref(ref: ParamRef | None, transform, deriv: int | None = None):
     if ref is None:
         return None
     base = ref.transform
-    return ParamRef(ref.gate, lambda b, base=base: transform(base(b)), ref.exprs, ref.slot)
+    return ParamRef(ref.gate, lambda b, base=base: transform(base(b)), ref.exprs, ref.slot,
+                    ref.deriv if deriv is None else deriv)
 
 
 # --------------------------------------------------------------------------------------------
@@ -546,7 +550,7 @@ def schedule(ops: list[LOp], n_bits: int, L: int, coalesce: int) -> list[Pass]:
                 defer.append(op)
                 blocked |= bits
                 continue
-            if op.kind == "dense":
+            if op.kind in ("dense", "grad_dense"):
                 need = set(op.targets) - S
                 if len(S) + len(need) <= L:
                     S |= need
@@ -687,8 +691,31 @@ class PackedProgram:
                 elif op.kind == "scale":
                     r["kind"] = _lib.OP_SCALE
                     r["mat"] = add_mat(np.array([op.mat]))
+                elif op.kind == "grad_dense":
+                    k = len(op.targets)
+                    if k > 3:
+                        raise UnsupportedOperation("gradients of gates on more than 3 qubits are not supported")
+                    r["kind"] = _lib.OP_GRAD_DENSE
+                    r["k"] = k
+                    for m, b in enumerate(op.targets):
+                        r["t"][m] = local[b]
+                    d = 2**k
+                    r["mat"] = add_mat(np.zeros((d, d)))
+                    self.param_entries.append((int(r["mat"]), op.pref, d * d))
+                    r["slot"] = op.slot
+                elif op.kind == "grad_diag":
+                    k = len(op.targets)
+                    r["kind"] = _lib.OP_GRAD_DIAG
+                    r["k"] = k
+                    for m, b in enumerate(op.targets):
+                        r["t"][m] = local[b] if b in local else OUTER + b
+                    r["mat"] = add_mat(np.zeros(2**k))
+                    self.param_entries.append((int(r["mat"]), op.pref, 2**k))
+                    r["slot"] = op.slot
                 else:
                     raise ValueError(op.kind)
+                if op.sel:
+                    r["kind"] = int(r["kind"]) | (op.sel << 8)
                 recs.append(r)
             p = self.pass_arr[pi]
             p["L"] = ps.L
@@ -720,7 +747,7 @@ class PackedProgram:
             for op in ps.ops:
                 if op.kind == "pair":
                     run_pairs.append(op)
-                elif op.kind in ("dense", "block"):
+                elif op.kind in ("dense", "block", "grad_dense", "grad_diag") or op.sel != 0:
                     flush()
                     ops.append(op)
                 else:
@@ -831,10 +858,13 @@ def fill_param_mats(prog: PackedProgram, ev: ParamEvaluator, symvals: np.ndarray
     out = np.broadcast_to(prog.mats_arr, (B, prog.mats_arr.size)).copy()
     cache: dict = {}
     for off, ref, size in prog.param_entries:
-        key = id(ref.gate)
+        key = (id(ref.gate), ref.deriv)
         if key not in cache:
             vals = tuple(ev.values(e, symvals) for e in ref.exprs)
-            cache[key] = (vals, ref.gate._matrix_batch(vals))
+            if ref.deriv is None:
+                cache[key] = (vals, ref.gate._matrix_batch(vals))
+            else:
+                cache[key] = (vals, ref.gate._dmatrix_batch(vals, ref.deriv))
         vals, mb = cache[key]
         payload = ref.transform(mb)
         out[:, off : off + size] = payload.reshape(B, -1).astype(np.complex64)

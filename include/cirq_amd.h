@@ -43,6 +43,9 @@ extern "C" {
 #define CQS_OP_PAIRPHASE 3 /* product of phases on bit pairs (a CZ / CZPow layer)           */
 #define CQS_OP_SCALE 4     /* multiply every amplitude by one complex number (global phase) */
 #define CQS_OP_BLOCK 5     /* fused 2^k x 2^k block (k = 4 or 5) on MFMA, split-FP16 operands  */
+#define CQS_OP_GRAD_DENSE 6 /* adjoint only: partial of 2 Re<phi|dU|psi> for a dense dU (k <= 3) */
+#define CQS_OP_GRAD_DIAG 7  /* adjoint only: the same for a diagonal dU (DIAG bit encoding)     */
+/* In adjoint programs bits 8-9 of `kind` select the tile an op acts on: 0 both, 1 psi, 2 phi. */
 
 #define CQS_MAX_LOCAL 16 /* largest tile: 2^16 amplitudes are never used; L <= 14 in practice */
 #define CQS_MAX_BITS 48  /* largest flat index width handled by the pass kernels            */
@@ -65,7 +68,8 @@ extern "C" {
  *              half8 entries [part][ks][rb][lane] = R_part[rb*32 + lane%32][ks*16 + 8*(lane/32) + e]
  *              starting at mats + mat (16-byte aligned).  Amplitudes are scaled by 2^15 and split
  *              the same way; three products (hi*hi, lo*hi, hi*lo) accumulate in FP32.
- *   slot:      parameter slot for gradient bookkeeping (-1 when unused).
+ *   slot:      parameter slot for gradient bookkeeping (-1 when unused); GRAD ops write their
+ *              per-tile partial to grad_partial[slot * partial_stride + tile].
  */
 typedef struct cqs_op {
   int32_t kind;
@@ -130,6 +134,23 @@ int cqs_init_basis(void* amps, int32_t n_bits, uint64_t index, void* stream);
  */
 int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
                    const cqs_op* ops, const void* mats, const int32_t* aux, void* stream);
+
+/* ---------------------------------------------------------- adjoint gradient */
+
+/*
+ * One reverse sweep of an adjoint program over psi (the final state) and phi (H psi), both
+ * updated in place, writing FP64 per-tile partials of every GRAD op.  The program is built by the
+ * host from the forward circuit reversed and inverted (cirq_amd/adjoint.py).  Cirq itself has no
+ * gradient machinery; callers differentiate expectation values from
+ * cirq-core/cirq/sim/simulator.py:349-371 (simulate_expectation_values_sweep) by parameter shifts,
+ * which this replaces with one sweep (Jones & Gacon 2020, arXiv:2009.02823).
+ */
+int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
+                       const cqs_op* ops, const void* mats, const int32_t* aux, double* grad_partial,
+                       int64_t partial_stride, void* stream);
+
+/* out[g] = sum over t < partial_stride of grad_partial[g * partial_stride + t], g < n_slots. */
+int cqs_grad_reduce(const double* grad_partial, int32_t n_slots, int64_t partial_stride, double* out, void* stream);
 
 /* ---------------------------------------------------------- reductions */
 

@@ -121,71 +121,106 @@ def _glob(code: int, lpos) -> int:
     return code - 64 if code >= 64 else lpos[code]
 
 
+def _apply_op(r, tile, gi, base, L, lpos, mats, mats_c64, aux):
+    """One non-gradient op of a pass on the gathered tiles (ntiles, 2^L); returns the new tiles."""
+    kind = int(r["kind"]) & 0xFF
+    com = int(r["co_mask_lo"]) | (int(r["co_mask_hi"]) << 32)
+    cov = int(r["co_val_lo"]) | (int(r["co_val_hi"]) << 32)
+    tile_ok = (base & com) == cov  # (ntiles,)
+    if kind in (_lib.OP_DENSE, _lib.OP_BLOCK):
+        k = int(r["k"])
+        t = [int(x) for x in r["t"][:k]]
+        D = 1 << k
+        if kind == _lib.OP_DENSE:
+            M = mats[int(r["mat"]) : int(r["mat"]) + D * D].reshape(D, D)
+        else:
+            from cirq_amd.tape import block_matrix_from_payload
+
+            raw = np.asarray(mats_c64[int(r["mat"]) : int(r["mat"]) + 2 * D * D], dtype=np.complex64)
+            M = block_matrix_from_payload(raw, k)
+        j = np.arange(1 << L)
+        j0 = j[np.all([(j >> b) & 1 == 0 for b in t], axis=0)]
+        pat = np.array([sum(((m >> b) & 1) << t[b] for b in range(k)) for m in range(D)])
+        idx = j0[:, None] + pat[None, :]  # (G, D)
+        grp_ok = (j0 & int(r["cl_mask"])) == int(r["cl_val"])
+        V = tile[:, idx]  # (ntiles, G, D)
+        out = V @ M.T
+        ok = tile_ok[:, None, None] & grp_ok[None, :, None]
+        tile = tile.copy()
+        tile[:, idx] = np.where(ok, out, V)
+        return tile
+    if kind == _lib.OP_DIAG:
+        k = int(r["k"])
+        t = [_glob(int(x), lpos) for x in r["t"][:k]]
+        Dv = mats[int(r["mat"]) : int(r["mat"]) + (1 << k)]
+        di = np.zeros_like(gi)
+        for m, b in enumerate(t):
+            di |= ((gi >> b) & 1) << m
+        return np.where(tile_ok[:, None], tile * Dv[di], tile)
+    if kind == _lib.OP_PAIRPHASE:
+        k = int(r["k"])
+        a0 = int(r["t"][0])
+        ph = mats[int(r["mat"]) : int(r["mat"]) + k]
+        for p in range(k):
+            a, b = _glob(int(aux[a0 + 2 * p]), lpos), _glob(int(aux[a0 + 2 * p + 1]), lpos)
+            both = ((gi >> a) & (gi >> b) & 1).astype(bool)
+            tile = np.where(both & tile_ok[:, None], tile * ph[p], tile)
+        return tile
+    if kind == _lib.OP_SCALE:
+        return np.where(tile_ok[:, None], tile * mats[int(r["mat"])], tile)
+    raise ValueError(kind)
+
+
+def _gather(packed, ps):
+    n = packed.n_bits
+    L = int(ps["L"])
+    lpos = [int(x) for x in ps["lpos"][:L]]
+    opos = [p for p in range(n) if p not in lpos]
+    tiles = np.arange(1 << (n - L), dtype=np.int64)
+    base = _deposit(tiles, opos)  # (ntiles,)
+    off = _deposit(np.arange(1 << L, dtype=np.int64), lpos)  # (T,)
+    return L, lpos, base, base[:, None] + off[None, :]
+
+
 def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) -> np.ndarray:
     """Executes the packed program on a copy of `state` (flat complex array)."""
     st = np.array(state, dtype=np.complex128)
-    n = packed.n_bits
-    ops = packed.ops_arr
     mats_c64 = np.asarray(packed.mats_arr if mats is None else mats, dtype=np.complex64)
     mats = mats_c64.astype(np.complex128)
-    aux = packed.aux_arr
     for ps in packed.pass_arr:
-        L = int(ps["L"])
-        lpos = [int(x) for x in ps["lpos"][:L]]
-        opos = [p for p in range(n) if p not in lpos]
-        tiles = np.arange(1 << (n - L), dtype=np.int64)
-        base = _deposit(tiles, opos)  # (ntiles,)
-        off = _deposit(np.arange(1 << L, dtype=np.int64), lpos)  # (T,)
-        I = base[:, None] + off[None, :]
+        L, lpos, base, I = _gather(packed, ps)
         tile = st[I]
-        gi = I
-        for r in ops[int(ps["op_begin"]) : int(ps["op_begin"]) + int(ps["op_count"])]:
-            kind = int(r["kind"])
-            com = int(r["co_mask_lo"]) | (int(r["co_mask_hi"]) << 32)
-            cov = int(r["co_val_lo"]) | (int(r["co_val_hi"]) << 32)
-            tile_ok = (base & com) == cov  # (ntiles,)
-            if kind in (_lib.OP_DENSE, _lib.OP_BLOCK):
-                k = int(r["k"])
-                t = [int(x) for x in r["t"][:k]]
-                D = 1 << k
-                if kind == _lib.OP_DENSE:
-                    M = mats[int(r["mat"]) : int(r["mat"]) + D * D].reshape(D, D)
-                else:
-                    from cirq_amd.tape import block_matrix_from_payload
-
-                    raw = np.asarray(mats_c64[int(r["mat"]) : int(r["mat"]) + 2 * D * D], dtype=np.complex64)
-                    M = block_matrix_from_payload(raw, k)
-                j = np.arange(1 << L)
-                j0 = j[np.all([(j >> b) & 1 == 0 for b in t], axis=0)]
-                pat = np.array([sum(((m >> b) & 1) << t[b] for b in range(k)) for m in range(D)])
-                idx = j0[:, None] + pat[None, :]  # (G, D)
-                grp_ok = (j0 & int(r["cl_mask"])) == int(r["cl_val"])
-                V = tile[:, idx]  # (ntiles, G, D)
-                out = V @ M.T
-                ok = tile_ok[:, None, None] & grp_ok[None, :, None]
-                tile[:, idx] = np.where(ok, out, V)
-            elif kind == _lib.OP_DIAG:
-                k = int(r["k"])
-                t = [_glob(int(x), lpos) for x in r["t"][:k]]
-                Dv = mats[int(r["mat"]) : int(r["mat"]) + (1 << k)]
-                di = np.zeros_like(gi)
-                for m, b in enumerate(t):
-                    di |= ((gi >> b) & 1) << m
-                tile = np.where(tile_ok[:, None], tile * Dv[di], tile)
-            elif kind == _lib.OP_PAIRPHASE:
-                k = int(r["k"])
-                a0 = int(r["t"][0])
-                ph = mats[int(r["mat"]) : int(r["mat"]) + k]
-                for p in range(k):
-                    a, b = _glob(int(aux[a0 + 2 * p]), lpos), _glob(int(aux[a0 + 2 * p + 1]), lpos)
-                    both = ((gi >> a) & (gi >> b) & 1).astype(bool)
-                    tile = np.where(both & tile_ok[:, None], tile * ph[p], tile)
-            elif kind == _lib.OP_SCALE:
-                tile = np.where(tile_ok[:, None], tile * mats[int(r["mat"])], tile)
-            else:
-                raise ValueError(kind)
+        for r in packed.ops_arr[int(ps["op_begin"]) : int(ps["op_begin"]) + int(ps["op_count"])]:
+            tile = _apply_op(r, tile, I, base, L, lpos, mats, mats_c64, packed.aux_arr)
         st[I] = tile
     return st
+
+
+def run_adjoint_model(packed, psi: np.ndarray, phi: np.ndarray, mats: np.ndarray, n_slots: int):
+    """The reverse sweep of the adjoint kernel: tile selectors (kind bits 8-9: 0 both, 1 psi,
+    2 phi) and GRAD ops accumulating 2 Re <phi| dU |psi> per slot.  Returns (psi, phi, grads)."""
+    psi = np.array(psi, dtype=np.complex128)
+    phi = np.array(phi, dtype=np.complex128)
+    mats_c64 = np.asarray(mats, dtype=np.complex64)
+    mats = mats_c64.astype(np.complex128)
+    grads = np.zeros(n_slots)
+    for ps in packed.pass_arr:
+        L, lpos, base, I = _gather(packed, ps)
+        ta, tb = psi[I], phi[I]
+        for r in packed.ops_arr[int(ps["op_begin"]) : int(ps["op_begin"]) + int(ps["op_count"])]:
+            kind, sel = int(r["kind"]) & 0xFF, (int(r["kind"]) >> 8) & 3
+            if kind in (_lib.OP_GRAD_DENSE, _lib.OP_GRAD_DIAG):
+                rr = r.copy()
+                rr["kind"] = _lib.OP_DENSE if kind == _lib.OP_GRAD_DENSE else _lib.OP_DIAG
+                y = _apply_op(rr, ta, I, base, L, lpos, mats, mats_c64, packed.aux_arr)  # dU psi
+                grads[int(r["slot"])] += 2 * np.real(np.vdot(tb, y))
+                continue
+            if sel != 2:
+                ta = _apply_op(r, ta, I, base, L, lpos, mats, mats_c64, packed.aux_arr)
+            if sel != 1:
+                tb = _apply_op(r, tb, I, base, L, lpos, mats, mats_c64, packed.aux_arr)
+        psi[I], phi[I] = ta, tb
+    return psi, phi, grads
 
 
 def fidelity(a: np.ndarray, b: np.ndarray) -> float:

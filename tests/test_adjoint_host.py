@@ -1,0 +1,165 @@
+"""CPU tests of the adjoint-gradient compiler (forward + reversed programs, gradient slots, chain
+rule and the parameter-shift matrix buffers).
+
+The packed programs run on the NumPy kernel model (tests/helpers.py); gradients are compared with
+central finite differences of the oracle's complex128 expectation value, which pins the sign,
+the factor 2 Re, the slot-to-symbol chain rule and the un-computation order independently.
+"""
+
+import numpy as np
+import pytest
+import sympy
+
+import cirq_amd as cirq
+from cirq_amd import generators
+from cirq_amd.adjoint import AdjointProgram, shift_rule_ok
+from oracle import cirq_np as O
+from tests.helpers import run_adjoint_model, run_pass_model, terms_of, to_oracle_ops
+
+GRAD_ATOL = 2e-5  # complex64 matrices, complex128 model arithmetic
+
+
+def _oracle_ev(circuit, qubits, psum, resolver):
+    ops = to_oracle_ops(circuit, qubits, resolver)
+    psi = O.simulate_state_vector(ops, len(qubits))
+    return sum(O.expectation_from_state_vector(psi, t, len(qubits)) for t in terms_of(psum, qubits)).real
+
+
+def _oracle_grad(circuit, qubits, psum, symbols, values, h=1e-5):
+    g = np.zeros(len(symbols))
+    for i in range(len(symbols)):
+        up, dn = dict(zip(symbols, values)), dict(zip(symbols, values))
+        up[symbols[i]] = values[i] + h
+        dn[symbols[i]] = values[i] - h
+        g[i] = (_oracle_ev(circuit, qubits, psum, cirq.ParamResolver(up))
+                - _oracle_ev(circuit, qubits, psum, cirq.ParamResolver(dn))) / (2 * h)
+    return g
+
+
+def _apply_psum(psi, psum, qubits):
+    """phi = H psi with the oracle's Pauli matrices."""
+    n = len(qubits)
+    out = np.zeros(2**n, dtype=np.complex128)
+    for coef, paulis in terms_of(psum, qubits):
+        t = psi.reshape((2,) * n).astype(np.complex128)
+        for ax, p in paulis.items():
+            t = O.apply_unitary(t, O.gate_unitary("Pauli", {"pauli": p}), [ax])
+        out += coef * t.reshape(-1)
+    return out
+
+
+def _model_value_and_grad(prog, psum, values):
+    symvals = prog.symbol_values(values)
+    mf, mr = prog.mats(symvals)
+    psi0 = np.zeros(2**prog.n, dtype=np.complex128)
+    psi0[0] = 1
+    psi = run_pass_model(prog.fwd, psi0, mf)
+    phi = _apply_psum(psi, psum, list(prog.qubits))
+    value = np.vdot(psi, phi).real
+    psi_back, _, gs = run_adjoint_model(prog.rev, psi, phi, mr, prog.n_slots)
+    return value, prog.chain(gs, symvals), psi_back
+
+
+def _mixed_circuit():
+    """Every differentiable kind the lowering produces: dense 1-2 qubit, diagonal, controlled
+    dense, controlled diagonal, multi-parameter gates and symbol expressions."""
+    a, b, c, d = sympy.symbols("a b c d")
+    q = cirq.LineQubit.range(5)
+    circuit = cirq.Circuit(
+        [cirq.H(x) for x in q],
+        cirq.rx(a).on(q[0]),
+        cirq.ry(2 * b + 0.3).on(q[1]),
+        cirq.rz(a * b).on(q[2]),
+        cirq.XPowGate(exponent=c, global_shift=0.25).on(q[3]),
+        cirq.CZPowGate(exponent=d).on(q[1], q[2]),
+        cirq.ZZPowGate(exponent=a).on(q[0], q[4]),
+        cirq.XXPowGate(exponent=b).on(q[2], q[3]),
+        cirq.PhasedXPowGate(phase_exponent=c, exponent=d).on(q[4]),
+        cirq.FSimGate(theta=a, phi=c).on(q[3], q[4]),
+        cirq.ControlledGate(cirq.ry(c)).on(q[0], q[2]),
+        cirq.ControlledGate(cirq.rz(d), control_values=[0]).on(q[4], q[1]),
+        cirq.ControlledGate(cirq.XPowGate(exponent=b), num_controls=2).on(q[1], q[3], q[0]),
+        cirq.CNOT(q[2], q[0]),
+        cirq.PhasedXZGate(x_exponent=a, z_exponent=b, axis_phase_exponent=c).on(q[2]),
+        cirq.ry(d).on(q[3]),
+    )
+    return circuit, [a, b, c, d]
+
+
+@pytest.mark.parametrize("tiles", [(5, 5, 4), (3, 3, 1), (2, 3, 0)])
+def test_adjoint_model_matches_finite_differences_mixed(tiles):
+    circuit, syms = _mixed_circuit()
+    qubits = sorted(circuit.all_qubits())
+    psum = generators.random_pauli_sum(qubits, 6, 3, seed=3)
+    L, La, c = tiles
+    prog = AdjointProgram(circuit, psum, tile_bits=L, adjoint_tile_bits=La, coalesce_bits=c)
+    assert [s.name for s in prog.symbols] == [s.name for s in syms]
+    vals = np.array([0.37, -1.1, 0.52, 0.81])
+    value, grad, psi_back = _model_value_and_grad(prog, psum, vals)
+    ref_value = _oracle_ev(circuit, qubits, psum, cirq.ParamResolver(dict(zip(syms, vals))))
+    assert abs(value - ref_value) < 1e-5
+    ref = _oracle_grad(circuit, qubits, psum, syms, vals)
+    np.testing.assert_allclose(grad, ref, atol=GRAD_ATOL)
+    # un-computation returns psi to |0...0>
+    assert abs(psi_back[0] - 1) < 1e-5 and np.abs(psi_back[1:]).max() < 1e-5
+
+
+def test_adjoint_model_hea_multi_pass():
+    circuit, syms, vals = generators.hardware_efficient_ansatz(6, 3, seed=1)
+    qubits = sorted(circuit.all_qubits())
+    psum = generators.random_pauli_sum(qubits, 5, 2, seed=2)
+    prog = AdjointProgram(circuit, psum, tile_bits=4, adjoint_tile_bits=3, coalesce_bits=2)
+    assert prog.rev.n_passes > 3  # outer bits and several passes are exercised
+    value, grad, _ = _model_value_and_grad(prog, psum, vals)
+    ref = _oracle_grad(circuit, qubits, psum, syms, vals)
+    np.testing.assert_allclose(grad, ref, atol=GRAD_ATOL)
+
+
+def test_adjoint_shared_symbol_sums_occurrences():
+    t = sympy.Symbol("t")
+    q = cirq.LineQubit.range(3)
+    circuit = cirq.Circuit([cirq.rx(t).on(x) for x in q], cirq.CZ(q[0], q[1]), [cirq.ry(2 * t).on(x) for x in q])
+    psum = cirq.Z(q[0]) * cirq.Z(q[1]) + 0.5 * cirq.X(q[2])
+    prog = AdjointProgram(circuit, psum, tile_bits=3, adjoint_tile_bits=2, coalesce_bits=1)
+    assert prog.n_slots == 6
+    _, grad, _ = _model_value_and_grad(prog, psum, [0.7])
+    ref = _oracle_grad(circuit, sorted(circuit.all_qubits()), psum, [t], [0.7])
+    np.testing.assert_allclose(grad, ref, atol=GRAD_ATOL)
+
+
+def test_parameter_shift_buffers_match_finite_differences():
+    circuit, syms, vals = generators.hardware_efficient_ansatz(4, 2, seed=5)
+    qubits = sorted(circuit.all_qubits())
+    psum = generators.random_pauli_sum(qubits, 3, 2, seed=1)
+    prog = AdjointProgram(circuit, psum, tile_bits=3, coalesce_bits=1)
+    symvals = prog.symbol_values(vals)
+    rows = prog.shifted_mats(symvals)
+    assert rows.shape[0] == 2 * prog.n_slots
+    psi0 = np.zeros(2**prog.n, dtype=np.complex128)
+    psi0[0] = 1
+    evs = []
+    for r in rows:
+        psi = run_pass_model(prog.fwd, psi0, r)
+        evs.append(np.vdot(psi, _apply_psum(psi, psum, qubits)).real)
+    evs = np.array(evs)
+    gs = (np.pi / 2) * (evs[0::2] - evs[1::2])
+    np.testing.assert_allclose(prog.chain(gs, symvals), _oracle_grad(circuit, qubits, psum, syms, vals),
+                               atol=GRAD_ATOL)
+
+
+def test_shift_rule_applicability():
+    assert shift_rule_ok(cirq.rx(sympy.Symbol("x")), 0)
+    assert shift_rule_ok(cirq.CZPowGate(exponent=sympy.Symbol("x")), 0)
+    assert shift_rule_ok(cirq.PhasedXPowGate(phase_exponent=0.1, exponent=sympy.Symbol("x")), 1)
+    assert not shift_rule_ok(cirq.PhasedXPowGate(phase_exponent=sympy.Symbol("x")), 0)
+    assert not shift_rule_ok(cirq.ISwapPowGate(exponent=sympy.Symbol("x")), 0)
+    assert not shift_rule_ok(cirq.FSimGate(theta=sympy.Symbol("x"), phi=0.1), 0)
+
+
+def test_adjoint_rejects_measurements_and_foreign_observables():
+    q = cirq.LineQubit.range(2)
+    t = sympy.Symbol("t")
+    with pytest.raises(ValueError):
+        AdjointProgram(cirq.Circuit(cirq.rx(t).on(q[0]), cirq.measure(q[0])), cirq.Z(q[0]))
+    with pytest.raises(KeyError):
+        AdjointProgram(cirq.Circuit(cirq.rx(t).on(q[0])), cirq.Z(q[1]))
