@@ -17,7 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libcirq_amd.so"
 
 CQS_OK, CQS_ERR_ARG, CQS_ERR_HIP, CQS_ERR_OOM, CQS_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 OP_DENSE, OP_DIAG, OP_PAIRPHASE, OP_SCALE, OP_BLOCK = 1, 2, 3, 4, 5
-OP_GRAD_DENSE, OP_GRAD_DIAG = 6, 7
+OP_GRAD_DENSE, OP_GRAD_DIAG, OP_PERMUTE = 6, 7, 8
 SEL_BOTH, SEL_PSI, SEL_PHI = 0, 1, 2
 MAX_LOCAL = 16
 
@@ -46,9 +46,10 @@ PASS_DTYPE = np.dtype(
         ("op_count", np.int32),
         ("max_k", np.int32),
         ("lpos", np.int32, (MAX_LOCAL,)),
+        ("lpos_out", np.int32, (MAX_LOCAL,)),
     ]
 )
-assert PASS_DTYPE.itemsize == 80
+assert PASS_DTYPE.itemsize == 144
 
 _lib = None
 

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(TileShape<L>::NT) void k_adjoint_pass(float2* __res
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
   const uint64_t base = deposit64((uint64_t)blockIdx.x, P.opos, P.nouter);
   uint64_t off_lo, off_hi[PER];
-  tile_offsets<L>(P, off_lo, off_hi);
+  tile_offsets<L>(P.lpos, off_lo, off_hi);
   if (act) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
@@ -233,6 +233,8 @@ int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* pas
     CQS_REQUIRE(ops != nullptr && mats != nullptr, "ops/mats are NULL");
     TileProg P;
     if (!make_prog(P, n_bits, ps, ops, (const float2*)mats, aux)) return CQS_ERR_ARG;
+    for (int b = 0; b < ps.L; ++b)
+      CQS_REQUIRE(ps.lpos_out[b] == ps.lpos[b], "adjoint passes cannot re-lay their tiles (PERMUTE)");
     const uint64_t tiles = 1ull << P.nouter;
     CQS_REQUIRE((int64_t)tiles <= partial_stride, "partial_stride %lld < tiles %llu", (long long)partial_stride,
                 (unsigned long long)tiles);

@@ -38,7 +38,8 @@ __global__ void k_scale(float2* __restrict__ a, uint64_t n, float2 s) {
     a[i] = cmul(a[i], s);
 }
 
-// One pass over a single state: workgroup b owns tile b (2^L amplitudes).
+// One pass over a single state: workgroup b owns tile b (2^L amplitudes).  The tile goes from HBM
+// straight into registers and back; LDS only carries the ops that mix elements across threads.
 template <int L>
 __global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_pass(float2* __restrict__ psi, TileProg P) {
   constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
@@ -47,19 +48,23 @@ __global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_pass(floa
   float2* mstage = lds + (1 << L);
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
   const uint64_t base = deposit64((uint64_t)blockIdx.x, P.opos, P.nouter);
-  uint64_t off_lo, off_hi[PER];
-  tile_offsets<L>(P, off_lo, off_hi);
-  if (act) {
-    float2 v[PER];
+  float2 v[PER];
+  {
+    uint64_t off_lo, off_hi[PER];
+    tile_offsets<L>(P.lpos, off_lo, off_hi);
+    if (act) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
-#pragma unroll
-    for (int e = 0; e < PER; ++e) tile[threadIdx.x + e * NT] = v[e];
+      for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
+    }
   }
-  run_program<PER, NT>(tile, mstage, P, base, act);
-  if (act) {
+  run_program<PER, NT>(v, tile, mstage, P, base, act);
+  {
+    uint64_t off_lo, off_hi[PER];
+    tile_offsets<L>(P.lpos_out, off_lo, off_hi);
+    if (act) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = tile[threadIdx.x + e * NT];
+      for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = v[e];
+    }
   }
 }
 
@@ -92,15 +97,25 @@ bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const 
   P.L = ps.L;
   P.nouter = n - ps.L;
   P.op_count = ps.op_count;
-  uint64_t used = 0;
+  uint64_t used = 0, used_out = 0;
   for (int b = 0; b < ps.L; ++b) {
-    const int p = ps.lpos[b];
+    const int p = ps.lpos[b], q = ps.lpos_out[b];
     if (p < 0 || p >= n || ((used >> p) & 1ull)) {
       set_error("pass local positions must be distinct and < n (bit %d = %d)", b, p);
       return false;
     }
+    if (q < 0 || q >= n || ((used_out >> q) & 1ull)) {
+      set_error("pass output positions must be distinct and < n (bit %d = %d)", b, q);
+      return false;
+    }
     used |= 1ull << p;
+    used_out |= 1ull << q;
     P.lpos[b] = p;
+    P.lpos_out[b] = q;
+  }
+  if (used != used_out) {
+    set_error("pass output positions are not a permutation of its local positions");
+    return false;
   }
   int k = 0;
   for (int p = 0; p < n; ++p)

@@ -5,8 +5,10 @@
 // deposit(j, lpos); the workgroup number maps to the remaining "outer" positions opos[].
 //
 // Thread tid of NT owns elements j = tid + e*NT (e < PER): it loads them from HBM into
-// registers, applies runs of elementwise ops (diagonals, pair phases, scales) there, and only
-// spills the tile to LDS around dense / block ops, which mix elements across threads.
+// registers and keeps them there through elementwise ops (diagonals, pair phases, scales) and
+// dense ops on register bits (tile bits >= log2(NT)).  The tile goes through LDS only for ops
+// that mix elements of different threads (other dense ops, MFMA blocks) and for PERMUTE, which
+// re-lays the tile so that the next run of gates finds its targets among the register bits.
 #pragma once
 
 #include "common.hpp"
@@ -20,7 +22,8 @@ struct TileProg {
   int32_t op_count;  // ops in this pass
   int32_t max_k;     // largest VALU dense k the LDS matrix stage was sized for
   float mscale;      // amplitude scale of the split-FP16 MFMA path (2^15 for normalised states)
-  int32_t lpos[CQS_MAX_LOCAL];
+  int32_t lpos[CQS_MAX_LOCAL];      // tile bit -> flat bit when the pass loads
+  int32_t lpos_out[CQS_MAX_LOCAL];  // tile bit -> flat bit when it stores (after PERMUTE ops)
   int32_t opos[CQS_MAX_BITS];
   const cqs_op* ops;
   const float2* mats;
@@ -253,35 +256,34 @@ __device__ __forceinline__ void ew_scale(float2 (&v)[PER], const TileProg& P, co
   for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], s);
 }
 
-// 1- and 2-qubit matrices on element bits (tile bits >= log2(NT)) stay inside one thread:
-// element e of the thread pairs with e ^ 2^R.
-template <int PER, int R>
-__device__ __forceinline__ void reg_dense1(float2 (&v)[PER], const float2* M) {
-  const float2 m00 = M[0], m01 = M[1], m10 = M[2], m11 = M[3];
+// Dense ops on register bits stay inside one thread.  Register bit r is tile bit log2(NT) + r, i.e.
+// bit r of the element index e; targets come sorted (R0 < R1 < R2) with matrix index bit m <->
+// target m.  Local controls are tested on the group's base element (controls are never targets).
+template <int PER, int NT, int K, int R0, int R1, int R2>
+__device__ __forceinline__ void reg_dense_k(float2 (&v)[PER], const float2* __restrict__ M, int cm, int cv) {
+  constexpr int D = 1 << K;
+  constexpr int RS[3] = {R0, R1, R2};
+  constexpr int TMASK = (1 << R0) | (K > 1 ? 1 << R1 : 0) | (K > 2 ? 1 << R2 : 0);
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
-    if ((e >> R) & 1) continue;
-    const int e1 = e | (1 << R);
-    const float2 a = v[e], b = v[e1];
-    v[e] = cfma(m01, b, cmul(m00, a));
-    v[e1] = cfma(m11, b, cmul(m10, a));
-  }
-}
-
-template <int PER, int R0, int R1>
-__device__ __forceinline__ void reg_dense2(float2 (&v)[PER], const float2* M) {
+    if (e & TMASK) continue;
+    if (((threadIdx.x + e * NT) & cm) != cv) continue;
+    int idx[D];
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    if (((e >> R0) & 1) || ((e >> R1) & 1)) continue;
-    const int idx[4] = {e, e | (1 << R0), e | (1 << R1), e | (1 << R0) | (1 << R1)};
-    float2 x[4];
+    for (int m = 0; m < D; ++m) {
+      int x = e;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) x[c] = v[idx[c]];
+      for (int b = 0; b < K; ++b) x |= ((m >> b) & 1) << RS[b];
+      idx[m] = x;
+    }
+    float2 x[D];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float2 acc = cmul(M[r * 4], x[0]);
+    for (int c = 0; c < D; ++c) x[c] = v[idx[c]];
 #pragma unroll
-      for (int c = 1; c < 4; ++c) acc = cfma(M[r * 4 + c], x[c], acc);
+    for (int r = 0; r < D; ++r) {
+      float2 acc = cmul(M[r * D], x[0]);
+#pragma unroll
+      for (int c = 1; c < D; ++c) acc = cfma(M[r * D + c], x[c], acc);
       v[idx[r]] = acc;
     }
   }
@@ -290,34 +292,37 @@ __device__ __forceinline__ void reg_dense2(float2 (&v)[PER], const float2* M) {
 template <int PER, int NT>
 __device__ __forceinline__ bool is_register_dense(const cqs_op& op) {
   constexpr int TB = __builtin_ctz(NT);
-  if (op.kind != CQS_OP_DENSE || op.k > 2 || op.cl_mask != 0 || PER < 2) return false;
-  for (int m = 0; m < op.k; ++m)
-    if (op.t[m] < TB) return false;
+  constexpr int NR = __builtin_ctz(PER);
+  if (op.kind != CQS_OP_DENSE || op.k < 1 || op.k > 3 || op.k > NR) return false;
+  if (op.t[0] < TB) return false;
+  for (int m = 1; m < op.k; ++m)
+    if (op.t[m] <= op.t[m - 1]) return false;
   return true;
 }
 
 template <int PER, int NT>
 __device__ __forceinline__ void reg_dense(float2 (&v)[PER], const TileProg& P, const cqs_op& op) {
   constexpr int TB = __builtin_ctz(NT);
-  constexpr int NR = __builtin_ctz(PER);  // element bits
+  constexpr int NR = __builtin_ctz(PER);
   const float2* M = P.mats + op.mat;
-  const int r0 = op.t[0] - TB;
-  if (op.k == 1) {
-    switch (r0) {
-      case 0: if constexpr (NR > 0) reg_dense1<PER, 0>(v, M); break;
-      case 1: if constexpr (NR > 1) reg_dense1<PER, 1>(v, M); break;
-      case 2: if constexpr (NR > 2) reg_dense1<PER, 2>(v, M); break;
-      case 3: if constexpr (NR > 3) reg_dense1<PER, 3>(v, M); break;
-      default: break;
-    }
-    return;
-  }
-  const int r1 = op.t[1] - TB;
+  const int cm = op.cl_mask, cv = op.cl_val;
+  const int r0 = op.t[0] - TB, r1 = op.t[1] - TB, r2 = op.t[2] - TB;
+  const int code = op.k == 1 ? r0 : (op.k == 2 ? 16 + r0 * 4 + r1 : 32 + r0 * 16 + r1 * 4 + r2);
+#define CQS_R1(A) \
+  case A: if constexpr (NR > A) reg_dense_k<PER, NT, 1, A, 0, 0>(v, M, cm, cv); return;
 #define CQS_R2(A, B) \
-  if (r0 == A && r1 == B) { if constexpr (NR > (A > B ? A : B)) reg_dense2<PER, A, B>(v, M); return; }
-  CQS_R2(0, 1) CQS_R2(0, 2) CQS_R2(0, 3) CQS_R2(1, 0) CQS_R2(1, 2) CQS_R2(1, 3)
-  CQS_R2(2, 0) CQS_R2(2, 1) CQS_R2(2, 3) CQS_R2(3, 0) CQS_R2(3, 1) CQS_R2(3, 2)
+  case 16 + A * 4 + B: if constexpr (NR > B) reg_dense_k<PER, NT, 2, A, B, 0>(v, M, cm, cv); return;
+#define CQS_R3(A, B, C) \
+  case 32 + A * 16 + B * 4 + C: if constexpr (NR > C) reg_dense_k<PER, NT, 3, A, B, C>(v, M, cm, cv); return;
+  switch (code) {
+    CQS_R1(0) CQS_R1(1) CQS_R1(2) CQS_R1(3)
+    CQS_R2(0, 1) CQS_R2(0, 2) CQS_R2(0, 3) CQS_R2(1, 2) CQS_R2(1, 3) CQS_R2(2, 3)
+    CQS_R3(0, 1, 2) CQS_R3(0, 1, 3) CQS_R3(0, 2, 3) CQS_R3(1, 2, 3)
+    default: return;
+  }
+#undef CQS_R1
 #undef CQS_R2
+#undef CQS_R3
 }
 
 template <int PER, int NT>
@@ -325,29 +330,64 @@ __device__ __forceinline__ bool is_mixing(const cqs_op& op) {
   return (op.kind == CQS_OP_DENSE && !is_register_dense<PER, NT>(op)) || op.kind == CQS_OP_BLOCK;
 }
 
-// Runs the pass program on the tile, which the caller has written to LDS (no barrier yet).
-// Mixing ops work LDS to LDS; each run of elementwise ops loads this thread's elements into
-// registers once, applies the whole run and writes them back.  Returns with the tile in LDS and
-// all threads synchronised.
+// This thread's elements between registers and the LDS tile, natural layout (j = tid + e*NT).
 template <int PER, int NT>
-__device__ __forceinline__ void run_program(float2* tile, float2* mstage, const TileProg& P, uint64_t base,
-                                            bool act) {
-  int i = 0;
-  while (i < P.op_count) {
-    if (is_mixing<PER, NT>(P.ops[i])) {
-      __syncthreads();
-      if (outer_controls_ok(P.ops[i], base)) run_mixing_op<NT>(tile, mstage, P, P.ops[i]);
-      ++i;
-      continue;
-    }
-    __syncthreads();
-    float2 v[PER];
-    if (act) {
+__device__ __forceinline__ void tile_to_regs(float2 (&v)[PER], const float2* tile, bool act) {
+  if (act) {
 #pragma unroll
-      for (int e = 0; e < PER; ++e) v[e] = tile[threadIdx.x + e * NT];
+    for (int e = 0; e < PER; ++e) v[e] = tile[threadIdx.x + e * NT];
+  }
+}
+template <int PER, int NT>
+__device__ __forceinline__ void regs_to_tile(const float2 (&v)[PER], float2* tile, bool act) {
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) tile[threadIdx.x + e * NT] = v[e];
+  }
+}
+
+// Element j = tid + e*NT goes to tile index sum_b bit_b(j) << dst[b].
+template <int PER, int NT>
+__device__ __forceinline__ void regs_to_tile_permuted(const float2 (&v)[PER], float2* tile, int L,
+                                                      const int32_t* __restrict__ dst, bool act) {
+  constexpr int TB = __builtin_ctz(NT);
+  constexpr int NR = __builtin_ctz(PER);
+  const int tb = L < TB ? L : TB;
+  int jt = 0;
+  for (int b = 0; b < tb; ++b) jt |= ((threadIdx.x >> b) & 1) << dst[b];
+  int eb[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) eb[r] = 1 << dst[TB + r];
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      int j = jt;
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        if ((e >> r) & 1) j |= eb[r];
+      tile[j] = v[e];
     }
+  }
+}
+
+// Runs the pass program.  On entry v holds this thread's elements (natural layout); on return it
+// holds them after the last op, in the layout of the last PERMUTE.  The program alternates
+// register phases (elementwise ops, register-bit dense ops, PERMUTE) and LDS phases (ops that mix
+// elements of different threads, LDS to LDS between barriers); v is dead during an LDS phase.
+template <int PER, int NT>
+__device__ __forceinline__ void run_program(float2 (&v)[PER], float2* tile, float2* mstage, const TileProg& P,
+                                            uint64_t base, bool act) {
+  int i = 0;
+  for (;;) {
     for (; i < P.op_count && !is_mixing<PER, NT>(P.ops[i]); ++i) {
       const cqs_op& op = P.ops[i];
+      if (op.kind == CQS_OP_PERMUTE) {
+        __syncthreads();  // every thread has read its elements before any is overwritten
+        regs_to_tile_permuted<PER, NT>(v, tile, P.L, P.aux + op.t[0], act);
+        __syncthreads();
+        tile_to_regs<PER, NT>(v, tile, act);
+        continue;
+      }
       if (!outer_controls_ok(op, base)) continue;
       if (op.kind == CQS_OP_DENSE)
         reg_dense<PER, NT>(v, P, op);
@@ -358,12 +398,15 @@ __device__ __forceinline__ void run_program(float2* tile, float2* mstage, const 
       else if (op.kind == CQS_OP_SCALE)
         ew_scale<PER>(v, P, op);
     }
-    if (act) {
-#pragma unroll
-      for (int e = 0; e < PER; ++e) tile[threadIdx.x + e * NT] = v[e];
+    if (i >= P.op_count) return;
+    regs_to_tile<PER, NT>(v, tile, act);
+    for (; i < P.op_count && is_mixing<PER, NT>(P.ops[i]); ++i) {
+      __syncthreads();
+      if (outer_controls_ok(P.ops[i], base)) run_mixing_op<NT>(tile, mstage, P, P.ops[i]);
     }
+    __syncthreads();
+    tile_to_regs<PER, NT>(v, tile, act);
   }
-  __syncthreads();
 }
 
 // Threads per workgroup for a tile of 2^L amplitudes (L <= 14): 256 threads up to L = 12 (threads
@@ -379,17 +422,17 @@ struct TileShape {
 
 // Flat offsets of this thread's elements: off_lo (thread bits) + off_hi[e] (element bits).
 template <int L>
-__device__ __forceinline__ void tile_offsets(const TileProg& P, uint64_t& off_lo,
+__device__ __forceinline__ void tile_offsets(const int32_t* pos, uint64_t& off_lo,
                                              uint64_t (&off_hi)[TileShape<L>::PER]) {
   constexpr int TBL = TileShape<L>::TBL, PER = TileShape<L>::PER;
   off_lo = 0;
 #pragma unroll
-  for (int b = 0; b < TBL; ++b) off_lo |= (uint64_t)((threadIdx.x >> b) & 1) << P.lpos[b];
+  for (int b = 0; b < TBL; ++b) off_lo |= (uint64_t)((threadIdx.x >> b) & 1) << pos[b];
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     uint64_t o = 0;
 #pragma unroll
-    for (int b = 0; b < L - TBL; ++b) o |= (uint64_t)((e >> b) & 1) << P.lpos[TBL + b];
+    for (int b = 0; b < L - TBL; ++b) o |= (uint64_t)((e >> b) & 1) << pos[TBL + b];
     off_hi[e] = o;
   }
 }

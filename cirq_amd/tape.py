@@ -382,7 +382,7 @@ def _op_matrix(op: LOp) -> np.ndarray:
 # Measured on MI355X at 28 qubits, tile of 12 bits (ms per op over the state): LDS dense1 0.115,
 # dense2 0.21, dense3 0.20, MFMA block k=4 0.25, k=5 0.29, diagonal / pair phase ~0.003.
 COSTS = {"dense1": 1.0, "dense2": 1.8, "dense3": 1.75, "diag": 0.05, "pair": 0.03, "scale": 0.03, "mfma4": 2.2,
-         "mfma5": 2.5}
+         "mfma5": 2.5, "reg1": 0.45, "reg2": 0.9, "reg3": 1.8, "permute": 0.8}
 
 
 def _valu_cost(op: LOp) -> float:
@@ -602,11 +602,28 @@ class PackedProgram:
 
         for pi, ps in enumerate(self.passes_merged(passes)):
             local = {b: j for j, b in enumerate(ps.lpos)}
+            layout = list(ps.lpos)
             begin = len(recs)
             max_k = 0
             for op in ps.ops:
                 r = np.zeros((), dtype=_lib.OP_DTYPE)
                 r["slot"] = -1
+                if op.kind == "permute":
+                    new_local = {b: j for j, b in enumerate(op.targets)}
+                    if set(new_local) != set(local):
+                        raise ValueError("permute must keep the pass's local bits")
+                    r["kind"] = _lib.OP_PERMUTE
+                    r["k"] = ps.L
+                    r["t"][0] = len(aux)
+                    dst = [new_local[layout[b]] for b in range(ps.L)]
+                    if sorted(dst) != list(range(ps.L)):  # the kernel writes LDS at these indices
+                        raise ValueError("PERMUTE destinations must be a permutation of the tile bits")
+                    aux.extend(dst)
+                    local, layout = new_local, list(op.targets)
+                    recs.append(r)
+                    continue
+                if op.kind == "dense" and len(op.targets) <= 3:
+                    op = sorted_targets(op)
                 if op.kind == "dense":
                     local_ctl = [(b, v) for b, v in op.controls if b in local]
                     outer_ctl = [(b, v) for b, v in op.controls if b not in local]
@@ -723,6 +740,7 @@ class PackedProgram:
             p["op_count"] = len(recs) - begin
             p["max_k"] = max_k
             p["lpos"][: ps.L] = ps.lpos
+            p["lpos_out"][: ps.L] = layout
         self.ops_arr = np.array(recs, dtype=_lib.OP_DTYPE) if recs else np.zeros(1, dtype=_lib.OP_DTYPE)
         self.mats_arr = np.concatenate(mats) if mats else np.zeros(1, dtype=np.complex64)
         self.aux_arr = np.array(aux if aux else [0], dtype=np.int32)
@@ -780,6 +798,193 @@ def order_tile_bits(p: Pass, coalesce: int) -> Pass:
     return Pass(L, tuple(sorted(low) + rest), p.ops)
 
 
+def sorted_targets(op: LOp) -> LOp:
+    """The same dense op with its targets in ascending order (matrix indices permuted to match),
+    the form the pass kernel runs in registers."""
+    t = list(op.targets)
+    k = len(t)
+    order = sorted(range(k), key=lambda m: t[m])  # new target m' is old target order[m']
+    if order == list(range(k)):
+        return op
+    P = np.array([sum(((i >> m) & 1) << order[m] for m in range(k)) for i in range(1 << k)])
+    mat = None if op.mat is None else op.mat[np.ix_(P, P)]
+    ref = op.pref
+    if ref is not None:
+        base = ref.transform
+        ref = ParamRef(ref.gate, lambda b, base=base, P=P: base(b)[..., P, :][..., :, P], ref.exprs, ref.slot,
+                       ref.deriv)
+    return LOp(op.kind, tuple(t[m] for m in order), mat, op.controls, ref, op.sel, op.slot)
+
+
+_ELEMENTWISE = ("diag", "pair", "pairs", "scale")
+
+
+def _op_dependencies(ops: list[LOp]) -> list[list[int]]:
+    """succ[i]: later ops that must follow op i (they share a bit and not both are elementwise)."""
+    succ: list[list[int]] = [[] for _ in ops]
+    last_mix: dict = {}
+    ew_since: dict = {}
+    for j, op in enumerate(ops):
+        ew = op.kind in _ELEMENTWISE
+        deps = set()
+        for b in op.bits:
+            if b in last_mix:
+                deps.add(last_mix[b])
+            if not ew:
+                deps.update(ew_since.get(b, ()))
+        for i in deps:
+            succ[i].append(j)
+        for b in op.bits:
+            if ew:
+                ew_since.setdefault(b, []).append(j)
+            else:
+                last_mix[b] = j
+                ew_since[b] = []
+    return succ
+
+
+def plan_registers(p: Pass, coalesce: int) -> Pass:
+    """Orders a pass's ops and inserts PERMUTE ops so that dense ops on 1-3 bits run in registers.
+
+    The pass kernel keeps each thread's 2^NR tile elements in registers (NR = the top tile bits);
+    dense ops whose targets are all register bits run there, other dense ops and MFMA blocks
+    exchange the tile through LDS.  Ops are list-scheduled over their dependency graph (ops on
+    disjoint bits commute, elementwise ops commute with each other): ready elementwise and
+    register ops run first; when only LDS ops are ready, the planner either re-lays the tile
+    (PERMUTE) so that a new set of register bits lets more ops run in registers, or runs the
+    ready op through LDS, whichever the cost table says is cheaper.  The coalescing bits stay the
+    lowest tile bits throughout, so loads and stores remain contiguous."""
+    L = p.L
+    NT = threads_for_tile(L)
+    TB = NT.bit_length() - 1
+    NR = L - TB
+    if NR <= 0 or not p.ops:
+        return p
+    c = min(coalesce, L)
+    low = [b for b in p.lpos[:c]]
+    movable = [b for b in p.lpos if b not in low]
+    ops = p.ops
+    n_ops = len(ops)
+    succ = _op_dependencies(ops)
+    npred = [0] * n_ops
+    for i in range(n_ops):
+        for j in succ[i]:
+            npred[j] += 1
+    kmax_reg = min(3, NR)
+
+    def reg_ok(op, regs) -> bool:
+        return op.kind == "dense" and len(op.targets) <= kmax_reg and set(op.targets) <= regs
+
+    def lds_cost(op) -> float:
+        if op.kind == "dense":
+            return COSTS[f"dense{len(op.targets)}"] if len(op.targets) <= 3 else COSTS["mfma5"]
+        return COSTS["mfma5"]
+
+    def drain(regs, npred, ready, emit=None) -> float:
+        """Runs every elementwise / register op reachable; returns the LDS cost saved."""
+        gain = 0.0
+        progress = True
+        while progress:
+            progress = False
+            for j in sorted(ready):
+                op = ops[j]
+                if op.kind in _ELEMENTWISE or reg_ok(op, regs):
+                    ready.discard(j)
+                    if op.kind == "dense":
+                        gain += lds_cost(op) - COSTS[f"reg{len(op.targets)}"]
+                    if emit is not None:
+                        emit.append(op)
+                    for s2 in succ[j]:
+                        npred[s2] -= 1
+                        if npred[s2] == 0:
+                            ready.add(s2)
+                    progress = True
+        return gain
+
+    def future_use(npred_now) -> dict:
+        use = {b: 0 for b in movable}
+        for j in range(n_ops):
+            op = ops[j]
+            if npred_now[j] >= 0 and op.kind == "dense" and len(op.targets) <= kmax_reg:
+                for b in op.targets:
+                    if b in use:
+                        use[b] += 1
+        return use
+
+    def best_regs(npred, ready, done) -> tuple[set, float]:
+        alive = [-1 if done[j] else npred[j] for j in range(n_ops)]
+        use = future_use(alive)
+        S: set = set()
+        score = 0.0
+        for _ in range(NR):
+            best = None
+            for b in movable:
+                if b in S:
+                    continue
+                g = drain(S | {b}, list(npred), set(ready))
+                key = (g, use[b], -b)
+                if best is None or key > best[0]:
+                    best = (key, b)
+            S.add(best[1])
+            score = best[0][0]
+        return S, score
+
+    def layout_for(cur: list, S: set) -> list:
+        regs_now = cur[TB:]
+        keep = [b if b in S else None for b in regs_now]
+        newcomers = [b for b in S if b not in regs_now]
+        leaving = [b for b in regs_now if b not in S]
+        new_regs = []
+        for b in keep:
+            new_regs.append(b if b is not None else newcomers.pop())
+        thread = [b if b not in S else None for b in cur[c:TB]]
+        thread = [b if b is not None else leaving.pop() for b in thread]
+        return cur[:c] + thread + new_regs
+
+    done = [False] * n_ops
+    ready = {j for j in range(n_ops) if npred[j] == 0}
+    S, _ = best_regs(npred, ready, done)
+    start = low + [b for b in movable if b not in S] + sorted(S)
+    layout = list(start)
+    out: list[LOp] = []
+
+    index_of = {id(op): j for j, op in enumerate(ops)}
+
+    def mark(op):
+        done[index_of[id(op)]] = True
+
+    while True:
+        before = len(out)
+        drain(set(layout[TB:]), npred, ready, out)
+        for op in out[before:]:
+            if op.kind != "permute":
+                mark(op)
+        if not ready:
+            break
+        # only LDS ops are ready: those that can never run in registers go first
+        forced = [j for j in sorted(ready) if not (ops[j].kind == "dense" and len(ops[j].targets) <= kmax_reg
+                                                    and not set(ops[j].targets) & set(low))]
+        if forced:
+            j = forced[0]
+        else:
+            S, gain = best_regs(npred, ready, done)
+            if gain > COSTS["permute"]:
+                layout = layout_for(layout, S)
+                out.append(LOp("permute", tuple(layout), None))
+                continue
+            j = min(ready)
+        ready.discard(j)
+        out.append(ops[j])
+        done[j] = True
+        for s2 in succ[j]:
+            npred[s2] -= 1
+            if npred[s2] == 0:
+                ready.add(s2)
+    if len([o for o in out if o.kind != "permute"]) != n_ops:
+        raise RuntimeError("register planner lost ops")
+    return Pass(L, tuple(start), out)
+
+
 def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
                 blocks: bool = True, kmax: int = 5) -> PackedProgram:
     if fuse:
@@ -787,7 +992,7 @@ def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, 
     passes = schedule(ops, n_bits, L, coalesce)
     if blocks:
         passes = [Pass(p.L, p.lpos, fuse_blocks(p.ops, set(p.lpos), p.L, kmax)) for p in passes]
-    passes = [order_tile_bits(p, coalesce) for p in passes]
+    passes = [plan_registers(order_tile_bits(p, coalesce), coalesce) for p in passes]
     return PackedProgram(passes, n_bits)
 
 

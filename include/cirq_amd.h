@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define CQS_ABI_VERSION 1
+#define CQS_ABI_VERSION 2
 
 /* status codes */
 #define CQS_OK 0
@@ -45,6 +45,7 @@ extern "C" {
 #define CQS_OP_BLOCK 5     /* fused 2^k x 2^k block (k = 4 or 5) on MFMA, split-FP16 operands  */
 #define CQS_OP_GRAD_DENSE 6 /* adjoint only: partial of 2 Re<phi|dU|psi> for a dense dU (k <= 3) */
 #define CQS_OP_GRAD_DIAG 7  /* adjoint only: the same for a diagonal dU (DIAG bit encoding)     */
+#define CQS_OP_PERMUTE 8    /* re-layout of the tile: tile bit b moves to tile bit aux[t[0] + b] */
 /* In adjoint programs bits 8-9 of `kind` select the tile an op acts on: 0 both, 1 psi, 2 phi. */
 
 #define CQS_MAX_LOCAL 16 /* largest tile: 2^16 amplitudes are never used; L <= 14 in practice */
@@ -57,10 +58,11 @@ extern "C" {
  *              cl_mask/cl_val: controls on tile-local bits (the op acts only on groups whose
  *              tile index satisfies (j & cl_mask) == cl_val); co_*: controls on bits outside
  *              the tile (global positions, 64-bit masks split in two words).
- *   DIAG:      t[0..k-1] are global bit positions; bit m of the diagonal index is
- *              bit t[m] of the flat index.  Diagonal at mats[mat .. mat+2^k).
- *   PAIRPHASE: k pairs; global positions at aux[t[0] + 2p], aux[t[0] + 2p + 1]; the phase
- *              of pair p is mats[mat + p] and applies when both bits are 1.
+ *   DIAG:      t[0..k-1] name k bits, each either a tile-local bit index (< 64) or 64 + a flat
+ *              bit position outside the tile; bit m of the diagonal index is that bit of the
+ *              amplitude's flat index.  Diagonal at mats[mat .. mat+2^k).
+ *   PAIRPHASE: k pairs of bits, encoded as for DIAG, at aux[t[0] + 2p], aux[t[0] + 2p + 1]; the
+ *              phase of pair p is mats[mat + p] and applies when both bits are 1.
  *   SCALE:     every amplitude times mats[mat].
  *   BLOCK:     like DENSE (targets t[0..k-1], k = 4 or 5) but applied with f16 MFMA: the matrix is
  *              given as its real form R (R[2r+a][2c+b] = Re/Im parts of U[r][c], size 2^(k+1)),
@@ -68,6 +70,8 @@ extern "C" {
  *              half8 entries [part][ks][rb][lane] = R_part[rb*32 + lane%32][ks*16 + 8*(lane/32) + e]
  *              starting at mats + mat (16-byte aligned).  Amplitudes are scaled by 2^15 and split
  *              the same way; three products (hi*hi, lo*hi, hi*lo) accumulate in FP32.
+ *   PERMUTE:   k = L; tile element j moves to the index whose bit aux[t[0] + b] is bit b of j.
+ *              Later ops of the pass name tile bits in the new layout.  Forward passes only.
  *   slot:      parameter slot for gradient bookkeeping (-1 when unused); GRAD ops write their
  *              per-tile partial to grad_partial[slot * partial_stride + tile].
  */
@@ -85,10 +89,14 @@ typedef struct cqs_op {
 
 /*
  * One pass: every workgroup gathers the 2^L amplitudes whose flat indices differ only in
- * the L distinct bit positions lpos[0..L-1] (tile index bit b <-> flat bit lpos[b]; the lowest
- * tile bits map to threads, the highest log2(2^L / threads) bits to registers of one thread,
- * where 1- and 2-bit DENSE ops run without LDS), runs ops[op_begin .. op_begin+op_count)
- * on them in LDS and writes them back.  One pass reads and writes the state once.
+ * the L distinct bit positions lpos[0..L-1] (tile index bit b <-> flat bit lpos[b]), runs
+ * ops[op_begin .. op_begin+op_count) on them and writes them back through lpos_out (equal to
+ * lpos unless the pass holds PERMUTE ops; then it is the layout after the last one).  One pass
+ * reads and writes the state once.
+ * Thread t of T holds tile elements t + e*T: the lowest log2(T) tile bits are thread bits, the
+ * highest log2(2^L / T) are register bits.  DENSE ops with k <= 3 whose targets are all register
+ * bits, given in ascending order, run in registers; other DENSE and BLOCK ops exchange the tile
+ * through LDS.
  */
 typedef struct cqs_pass {
   int32_t L;
@@ -96,6 +104,7 @@ typedef struct cqs_pass {
   int32_t op_count;
   int32_t max_k; /* largest DENSE k in the pass (sizes the LDS matrix stage) */
   int32_t lpos[CQS_MAX_LOCAL];
+  int32_t lpos_out[CQS_MAX_LOCAL];
 } cqs_pass;
 
 /* ---------------------------------------------------------------- library */

@@ -182,6 +182,25 @@ def _gather(packed, ps):
     return L, lpos, base, base[:, None] + off[None, :]
 
 
+def _permute(r, tile, I, lpos, aux):
+    """PERMUTE: tile bit b moves to tile bit aux[t0 + b] (tiles and their flat indices alike)."""
+    L = len(lpos)
+    dst = [int(aux[int(r["t"][0]) + b]) for b in range(L)]
+    assert sorted(dst) == list(range(L))
+    j = np.arange(1 << L)
+    nj = np.zeros_like(j)
+    for b in range(L):
+        nj |= ((j >> b) & 1) << dst[b]
+    nt = np.empty_like(tile)
+    nI = np.empty_like(I)
+    nt[:, nj] = tile
+    nI[:, nj] = I
+    nl = [0] * L
+    for b in range(L):
+        nl[dst[b]] = lpos[b]
+    return nt, nI, nl
+
+
 def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) -> np.ndarray:
     """Executes the packed program on a copy of `state` (flat complex array)."""
     st = np.array(state, dtype=np.complex128)
@@ -191,7 +210,11 @@ def run_pass_model(packed, state: np.ndarray, mats: np.ndarray | None = None) ->
         L, lpos, base, I = _gather(packed, ps)
         tile = st[I]
         for r in packed.ops_arr[int(ps["op_begin"]) : int(ps["op_begin"]) + int(ps["op_count"])]:
+            if int(r["kind"]) == _lib.OP_PERMUTE:
+                tile, I, lpos = _permute(r, tile, I, lpos, packed.aux_arr)
+                continue
             tile = _apply_op(r, tile, I, base, L, lpos, mats, mats_c64, packed.aux_arr)
+        assert lpos == [int(x) for x in ps["lpos_out"][:L]], "lpos_out disagrees with the PERMUTE ops"
         st[I] = tile
     return st
 

@@ -250,7 +250,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(_lib.LIB_PATH))
     for name in declared:
         assert hasattr(lib, name), name
-    assert lib.cqs_abi_version() == 1
+    assert lib.cqs_abi_version() == 2
 
 
 # ------------------------------------------------------------------ MFMA block fusion (packing checked by the model)
