@@ -45,7 +45,65 @@ def parse():
     ap.add_argument("--adjoint-qubits", type=int, default=20)
     ap.add_argument("--adjoint-layers", type=int, default=10)
     ap.add_argument("--adjoint-reps", type=int, default=5)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
+    ap.add_argument("--sweep-circuits", type=int, default=256)
+    ap.add_argument("--sweep-points", type=int, default=16)
+    ap.add_argument("--sweep-looped-circuits", type=int, default=256,
+                    help="circuits whose first point the looped comparison runs (scaled to all points)")
     return ap.parse_args()
+
+
+def sweep_leg(args):
+    """Mixed-size batched sweep: 256 hardware-efficient ansatz circuits (10-22 qubits, 4-16 layers),
+    16 points each (4096), per point <P_t> of 32 random 3-local strings with N(0,1) weights plus
+    the adjoint gradient of their sum.  Batched (one launch per pass level over all points, HIP
+    graph replay) vs looped (one point at a time through the single-state path)."""
+    import torch
+
+    from cirq_amd import generators
+    from cirq_amd.adjoint import AdjointGradient
+    from cirq_amd.batch import BatchedExpectation
+    from cirq_amd.paulis import PauliSum
+
+    circuits, observables, values = generators.mixed_sweep_workload(
+        n_circuits=args.sweep_circuits, points=args.sweep_points, seed=0)
+    n_points = sum(v.shape[0] for v in values)
+    t0 = time.perf_counter()
+    be = BatchedExpectation(circuits, observables)
+    t_compile = time.perf_counter() - t0
+    be.run(values)  # plan, graph capture, warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs, grads = be.run(values)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    # looped: the first point of each of the first K circuits, one at a time
+    K = min(args.sweep_looped_circuits, len(circuits))
+    ags = [AdjointGradient(circuits[c], PauliSum.from_pauli_strings(observables[c])) for c in range(K)]
+    ags[0].value_and_grad(values[0][0])
+    torch.cuda.synchronize()
+    diffs = []
+    t0 = time.perf_counter()
+    for c in range(K):
+        v, g = ags[c].value_and_grad(values[c][0])
+        diffs.append(max(float(np.abs(g - grads[c][0]).max()), abs(v - float(evs[c][0].sum()))))
+    torch.cuda.synchronize()
+    t_loop_sample = time.perf_counter() - t0
+    looped_points = K
+    t_loop = t_loop_sample / looped_points * n_points
+    qubits = [len(c.all_qubits()) for c in circuits]
+    return {
+        "workload": f"{len(circuits)} HEA circuits x {args.sweep_points} points = {n_points} points, "
+        f"{min(qubits)}-{max(qubits)} qubits, 32 random 3-local strings each, <P_t> + adjoint gradients",
+        "batched_s": t_batch,
+        "batched_points_per_s": n_points / t_batch,
+        "looped_points_per_s": looped_points / t_loop_sample,
+        "looped_sample": f"first point of {looped_points} circuits ({t_loop_sample:.2f} s), scaled to {n_points}",
+        "speedup": t_loop / t_batch,
+        "max_abs_diff_batched_vs_looped": max(diffs),
+        "compile_s": t_compile,
+        "launches": len(be.host.launches),
+    }
 
 
 def adjoint_leg(args):
@@ -246,6 +304,8 @@ def main():
     }
     if not args.no_adjoint:
         out["extra"]["adjoint"] = adjoint_leg(args)
+    if not args.no_sweep:
+        out["extra"]["sweep"] = sweep_leg(args)
     if not args.no_cpu_baseline:
         from tests.helpers import to_oracle_ops
 

@@ -51,6 +51,47 @@ PASS_DTYPE = np.dtype(
 )
 assert PASS_DTYPE.itemsize == 144
 
+BATCH_PASS_DTYPE = np.dtype(
+    [
+        ("n_bits", np.int32),
+        ("L", np.int32),
+        ("op_begin", np.int32),
+        ("op_count", np.int32),
+        ("max_k", np.int32),
+        ("nouter", np.int32),
+        ("pad", np.int32, (2,)),
+        ("lpos", np.int32, (MAX_LOCAL,)),
+        ("lpos_out", np.int32, (MAX_LOCAL,)),
+        ("opos", np.int32, (48,)),
+    ]
+)
+assert BATCH_PASS_DTYPE.itemsize == 352
+
+BATCH_ITEM_DTYPE = np.dtype(
+    [
+        ("state", np.int64),
+        ("state2", np.int64),
+        ("mats", np.int64),
+        ("partial", np.int64),
+        ("pass", np.int32),
+        ("tile_begin", np.int32),
+    ]
+)
+assert BATCH_ITEM_DTYPE.itemsize == 40
+
+BATCH_TERMS_DTYPE = np.dtype(
+    [
+        ("state", np.int64),
+        ("state2", np.int64),
+        ("out", np.int64),
+        ("n_bits", np.int32),
+        ("term_begin", np.int32),
+        ("n_terms", np.int32),
+        ("block_begin", np.int32),
+    ]
+)
+assert BATCH_TERMS_DTYPE.itemsize == 40
+
 _lib = None
 
 _vp = ctypes.c_void_p
@@ -79,6 +120,13 @@ _SIGNATURES = {
     "cqs_indices_to_bits": ([_vp, _i64, _i32, _vp, _vp], ctypes.c_int),
     "cqs_adjoint_passes": ([_vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp], ctypes.c_int),
     "cqs_grad_reduce": ([_vp, _i32, _i64, _vp, _vp], ctypes.c_int),
+    "cqs_batch_validate": ([_vp, _i32, _i32, _vp, _i32, _i32, _i64, _i64, _i32], ctypes.c_int),
+    "cqs_batch_init": ([_vp, _vp, _i32, _i64, _vp, _i32, _vp], ctypes.c_int),
+    "cqs_batch_passes": ([_vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cqs_batch_adjoint": ([_vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cqs_batch_apply_paulisum": ([_vp, _vp, _i32, _i64, _vp, _vp, _vp, _i32, _vp], ctypes.c_int),
+    "cqs_batch_expect": ([_vp, _vp, _i32, _i64, _vp, _vp, _vp, _i32, _vp], ctypes.c_int),
+    "cqs_segment_sum": ([_vp, _vp, _i32, _i32, _vp, _vp], ctypes.c_int),
 }
 
 

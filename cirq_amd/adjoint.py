@@ -154,6 +154,14 @@ class AdjointProgram:
         self.tile_bits_rev = La
         self.partial_stride = 1 << (n - La)
         self._slot_exprs = [ref.exprs[k] for ref, k in self.grad_slots]
+        index = {sym: i for i, sym in enumerate(self.symbols)}
+        # (slot, symbol index, symbol, expression) for every symbol an expression depends on
+        self._slot_terms = [
+            (slot, index[sym], sym, e)
+            for slot, e in enumerate(self._slot_exprs)
+            for sym in sorted(getattr(e, "free_symbols", ()), key=lambda x: x.name)
+            if sym in index
+        ]
 
     @property
     def n_slots(self) -> int:
@@ -174,13 +182,17 @@ class AdjointProgram:
                 fill_param_mats(self.rev, self.evaluator, symvals)[0])
 
     def chain(self, gs: np.ndarray, symvals: np.ndarray) -> np.ndarray:
-        """d<H>/d symbol from per-slot derivatives d<H>/d t_slot."""
-        grad = np.zeros(len(self.symbols))
-        for i, s in enumerate(self.symbols):
-            for g, e in zip(gs, self._slot_exprs):
-                d = self.evaluator.dvalues(e, s, symvals)
-                if d is not None:
-                    grad[i] += g * d[0]
+        """d<H>/d symbol from per-slot derivatives d<H>/d t_slot (one point)."""
+        return self.chain_batch(np.asarray(gs)[None, :], symvals)[0]
+
+    def chain_batch(self, gs: np.ndarray, symvals: np.ndarray) -> np.ndarray:
+        """(B, n_symbols) gradients from (B, n_slots) slot derivatives; symvals (n_symbols, B)."""
+        B = symvals.shape[1]
+        grad = np.zeros((B, len(self.symbols)))
+        for slot, i, sym, e in self._slot_terms:
+            d = self.evaluator.dvalues(e, sym, symvals)
+            if d is not None:
+                grad[:, i] += gs[:, slot] * d
         return grad
 
     def shifted_mats(self, symvals: np.ndarray) -> np.ndarray:
@@ -225,6 +237,13 @@ class AdjointGradient:
         check(lib().cqs_init_basis(ptr(self.psi), self.n, 0, engine.stream_handle()), "cqs_init_basis")
         self.dev_fwd.run(self.psi)
 
+    def forward_state(self, values) -> torch.Tensor:
+        """The forward state psi(values) (a view of this object's buffer)."""
+        symvals = self.prog.symbol_values(values)
+        self.dev_fwd.set_mats(fill_param_mats(self.prog.fwd, self.prog.evaluator, symvals)[0])
+        self._forward()
+        return self.psi
+
     def value_and_grad(self, values) -> tuple[float, np.ndarray]:
         """(<H>, d<H>/d symbol in `symbols` order) at the given symbol values."""
         prog, n = self.prog, self.n
@@ -233,7 +252,8 @@ class AdjointGradient:
         self.dev_fwd.set_mats(mf)
         self.dev_rev.set_mats(mr)
         self._forward()
-        ev = engine.expect_terms(self.psi, n, False, prog.terms, self.ws).sum()
+        evs = engine.expect_terms(self.psi, n, False, prog.terms, self.ws)
+        ev = evs.sum()
         engine.apply_paulisum(self.psi, self.phi, n, prog.terms)
         G = prog.n_slots
         rv = prog.rev
@@ -248,6 +268,7 @@ class AdjointGradient:
         check(lib().cqs_grad_reduce(ptr(self.partial), G, prog.partial_stride, ptr(self.gslot), engine.stream_handle()),
               "cqs_grad_reduce")
         gs = self.gslot.cpu().numpy()[:G]
+        self.last_term_values = evs.real.cpu().numpy()  # <coef_t P_t> of every term of H
         return float(ev.real.item()), prog.chain(gs, symvals)
 
     def parameter_shift(self, values) -> tuple[float, np.ndarray]:

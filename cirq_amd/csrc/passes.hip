@@ -42,30 +42,9 @@ __global__ void k_scale(float2* __restrict__ a, uint64_t n, float2 s) {
 // straight into registers and back; LDS only carries the ops that mix elements across threads.
 template <int L>
 __global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_pass(float2* __restrict__ psi, TileProg P) {
-  constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
   extern __shared__ float2 lds[];
-  float2* tile = lds;
-  float2* mstage = lds + (1 << L);
-  const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
   const uint64_t base = deposit64((uint64_t)blockIdx.x, P.opos, P.nouter);
-  float2 v[PER];
-  {
-    uint64_t off_lo, off_hi[PER];
-    tile_offsets<L>(P.lpos, off_lo, off_hi);
-    if (act) {
-#pragma unroll
-      for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
-    }
-  }
-  run_program<PER, NT>(v, tile, mstage, P, base, act);
-  {
-    uint64_t off_lo, off_hi[PER];
-    tile_offsets<L>(P.lpos_out, off_lo, off_hi);
-    if (act) {
-#pragma unroll
-      for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = v[e];
-    }
-  }
+  pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds);
 }
 
 // amps[i] = ((i & mask) == value) ? amps[i] * s : 0

@@ -30,6 +30,17 @@ struct TileProg {
   const int32_t* aux;
 };
 
+// The part of a pass program the tile interpreter reads (TileProg carries the same fields).
+struct ProgView {
+  int32_t L;
+  int32_t op_count;
+  int32_t max_k;
+  float mscale;
+  const cqs_op* ops;
+  const float2* mats;
+  const int32_t* aux;
+};
+
 // Bit encoding of DIAG / PAIRPHASE positions: values < kOuter are tile-local indices, values
 // >= kOuter are kOuter + a global position outside the tile.
 constexpr int kOuter = 64;
@@ -171,8 +182,8 @@ __device__ __forceinline__ void stage_matrix(float2* dst, const float2* src, int
 }
 
 // Dense (VALU, k <= 3) or block (MFMA, k = 4, 5) op on the LDS tile.  Caller fences before.
-template <int NT>
-__device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, const TileProg& P, const cqs_op& op) {
+template <int NT, class Prog>
+__device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, const Prog& P, const cqs_op& op) {
   if (op.kind == CQS_OP_BLOCK) {
     const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat);
     if (op.k == 4)
@@ -197,8 +208,8 @@ __device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, cons
 
 // ---------------------------------------------------------------- elementwise ops in registers
 // v[e] holds tile element j = threadIdx.x + e*NT.
-template <int PER, int NT>
-__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const TileProg& P, const cqs_op& op, uint64_t base) {
+template <int PER, int NT, class Prog>
+__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, const cqs_op& op, uint64_t base) {
   constexpr int TB = __builtin_ctz(NT);
   const int tid = threadIdx.x;
   int fixed = 0;   // index bits known for the whole thread
@@ -222,8 +233,8 @@ __device__ __forceinline__ void ew_diag(float2 (&v)[PER], const TileProg& P, con
   }
 }
 
-template <int PER, int NT>
-__device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const TileProg& P, const cqs_op& op, uint64_t base) {
+template <int PER, int NT, class Prog>
+__device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const Prog& P, const cqs_op& op, uint64_t base) {
   const int32_t* pr = P.aux + op.t[0];
   const float2* ph = P.mats + op.mat;
   const int tid = threadIdx.x;
@@ -249,8 +260,8 @@ __device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const TileProg& P, co
   }
 }
 
-template <int PER>
-__device__ __forceinline__ void ew_scale(float2 (&v)[PER], const TileProg& P, const cqs_op& op) {
+template <int PER, class Prog>
+__device__ __forceinline__ void ew_scale(float2 (&v)[PER], const Prog& P, const cqs_op& op) {
   const float2 s = P.mats[op.mat];
 #pragma unroll
   for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], s);
@@ -300,8 +311,8 @@ __device__ __forceinline__ bool is_register_dense(const cqs_op& op) {
   return true;
 }
 
-template <int PER, int NT>
-__device__ __forceinline__ void reg_dense(float2 (&v)[PER], const TileProg& P, const cqs_op& op) {
+template <int PER, int NT, class Prog>
+__device__ __forceinline__ void reg_dense(float2 (&v)[PER], const Prog& P, const cqs_op& op) {
   constexpr int TB = __builtin_ctz(NT);
   constexpr int NR = __builtin_ctz(PER);
   const float2* M = P.mats + op.mat;
@@ -374,8 +385,8 @@ __device__ __forceinline__ void regs_to_tile_permuted(const float2 (&v)[PER], fl
 // holds them after the last op, in the layout of the last PERMUTE.  The program alternates
 // register phases (elementwise ops, register-bit dense ops, PERMUTE) and LDS phases (ops that mix
 // elements of different threads, LDS to LDS between barriers); v is dead during an LDS phase.
-template <int PER, int NT>
-__device__ __forceinline__ void run_program(float2 (&v)[PER], float2* tile, float2* mstage, const TileProg& P,
+template <int PER, int NT, class Prog>
+__device__ __forceinline__ void run_program(float2 (&v)[PER], float2* tile, float2* mstage, const Prog& P,
                                             uint64_t base, bool act) {
   int i = 0;
   for (;;) {
@@ -434,6 +445,35 @@ __device__ __forceinline__ void tile_offsets(const int32_t* pos, uint64_t& off_l
 #pragma unroll
     for (int b = 0; b < L - TBL; ++b) o |= (uint64_t)((e >> b) & 1) << pos[TBL + b];
     off_hi[e] = o;
+  }
+}
+
+// One workgroup's whole pass: load the tile from HBM (layout lpos) into registers, run the
+// program, store it (layout lpos_out).  `base` is the flat offset of the tile's outer bits.
+template <int L, class Prog>
+__device__ __forceinline__ void pass_tile(float2* __restrict__ psi, const Prog& P, const int32_t* lpos,
+                                          const int32_t* lpos_out, uint64_t base, float2* lds) {
+  constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
+  float2* tile = lds;
+  float2* mstage = lds + (1 << L);
+  const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
+  float2 v[PER];
+  {
+    uint64_t off_lo, off_hi[PER];
+    tile_offsets<L>(lpos, off_lo, off_hi);
+    if (act) {
+#pragma unroll
+      for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
+    }
+  }
+  run_program<PER, NT>(v, tile, mstage, P, base, act);
+  {
+    uint64_t off_lo, off_hi[PER];
+    tile_offsets<L>(lpos_out, off_lo, off_hi);
+    if (act) {
+#pragma unroll
+      for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = v[e];
+    }
   }
 }
 

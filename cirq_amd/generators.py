@@ -192,6 +192,26 @@ def random_pauli_sum(qubits, n_terms: int, locality: int, seed: int = 0, include
     return PauliSum.from_pauli_strings(terms)
 
 
+def mixed_sweep_workload(n_circuits: int = 256, points: int = 16, n_terms: int = 32, locality: int = 3,
+                         qubit_range=(10, 22), layer_range=(4, 16), seed: int = 0):
+    """A mixed-size sweep: n_circuits hardware-efficient ansatz circuits whose qubit and layer counts
+    are uniform in the given inclusive ranges, `points` rows of symbol values ~ U[-pi, pi] each, and
+    per circuit n_terms random `locality`-local Pauli strings with N(0, 1) weights.
+    Returns (circuits, observables: lists of PauliStrings, values: (points, n_symbols) arrays)."""
+    rng = np.random.RandomState(seed)
+    circuits, observables, values = [], [], []
+    for k in range(n_circuits):
+        n = int(rng.randint(qubit_range[0], qubit_range[1] + 1))
+        layers = int(rng.randint(layer_range[0], layer_range[1] + 1))
+        c, syms, _ = hardware_efficient_ansatz(n, layers, seed=k, prefix=f"c{k}_")
+        qs = sorted(c.all_qubits())
+        terms = list(random_pauli_sum(qs, n_terms, locality, seed=seed * 100003 + k, include_z=False))
+        circuits.append(c)
+        observables.append(terms)
+        values.append(rng.uniform(-np.pi, np.pi, (points, len(syms))))
+    return circuits, observables, values
+
+
 def random_3_regular_graph(n: int, seed: int = 0) -> list:
     """Edges of a random 3-regular graph on n nodes (configuration model with rejection)."""
     rng = np.random.RandomState(seed)

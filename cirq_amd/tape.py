@@ -998,7 +998,11 @@ def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, 
 
 # --------------------------------------------------------------------------------------------
 class ParamEvaluator:
-    """Evaluates parameter expressions for batches of resolvers (vectorised with sympy.lambdify)."""
+    """Evaluates parameter expressions for batches of resolvers (vectorised over resolvers).
+
+    A bare symbol and an affine function of one symbol (the common cases: rx(theta) has exponent
+    theta/pi) are evaluated directly; other expressions go through sympy.lambdify over their own
+    free symbols only."""
 
     def __init__(self, refs: Sequence[ParamRef]):
         self.symbols: list[sympy.Symbol] = []
@@ -1010,27 +1014,33 @@ class ParamEvaluator:
                         if s.name not in seen:
                             seen.add(s.name)
                             self.symbols.append(s)
+        self.index = {s: i for i, s in enumerate(self.symbols)}
         self._fns: dict = {}
         self._dfns: dict = {}
 
-    def _fn(self, e):
-        key = e
-        if key not in self._fns:
-            if isinstance(e, sympy.Basic) and e.free_symbols:
-                self._fns[key] = sympy.lambdify(self.symbols, e, "numpy")
-            else:
-                v = float(e)
-                self._fns[key] = lambda *a, v=v: v
-        return self._fns[key]
+    def _compile(self, e):
+        """(kind, data): ('const', v) | ('affine', (i, a, b)) | ('fn', (rows, f))."""
+        if not isinstance(e, sympy.Basic) or not e.free_symbols:
+            return "const", float(e)
+        syms = sorted(e.free_symbols, key=lambda x: x.name)
+        rows = [self.index[x] for x in syms]
+        if len(syms) == 1:
+            a = sympy.diff(e, syms[0])
+            if not a.free_symbols:
+                return "affine", (rows[0], float(a), float(e.subs(syms[0], 0)))
+        return "fn", (rows, sympy.lambdify(syms, e, "numpy"))
 
-    def _dfn(self, e, sym):
-        key = (e, sym)
-        if key not in self._dfns:
-            if isinstance(e, sympy.Basic) and sym in e.free_symbols:
-                self._dfns[key] = sympy.lambdify(self.symbols, sympy.diff(e, sym), "numpy")
-            else:
-                self._dfns[key] = None
-        return self._dfns[key]
+    @staticmethod
+    def _eval(c, symvals: np.ndarray) -> np.ndarray:
+        kind, d = c
+        B = symvals.shape[1]
+        if kind == "const":
+            return np.full(B, d)
+        if kind == "affine":
+            i, a, b = d
+            return a * symvals[i] + b
+        rows, f = d
+        return np.broadcast_to(np.asarray(f(*(symvals[r] for r in rows)), dtype=np.float64), (B,)).copy()
 
     def symbol_values(self, resolvers) -> np.ndarray:
         """(n_symbols, B) float64 array of resolved symbol values."""
@@ -1045,16 +1055,23 @@ class ParamEvaluator:
         return out
 
     def values(self, e, symvals: np.ndarray) -> np.ndarray:
-        B = symvals.shape[1]
-        v = self._fn(e)(*symvals)
-        return np.broadcast_to(np.asarray(v, dtype=np.float64), (B,)).copy()
+        c = self._fns.get(e)
+        if c is None:
+            c = self._fns[e] = self._compile(e)
+        return self._eval(c, symvals)
 
     def dvalues(self, e, sym, symvals: np.ndarray):
-        f = self._dfn(e, sym)
-        if f is None:
+        key = (e, sym)
+        c = self._dfns.get(key, False)
+        if c is False:
+            if isinstance(e, sympy.Basic) and sym in e.free_symbols:
+                c = self._compile(sympy.diff(e, sym))
+            else:
+                c = None
+            self._dfns[key] = c
+        if c is None:
             return None
-        B = symvals.shape[1]
-        return np.broadcast_to(np.asarray(f(*symvals), dtype=np.float64), (B,)).copy()
+        return self._eval(c, symvals)
 
 
 def fill_param_mats(prog: PackedProgram, ev: ParamEvaluator, symvals: np.ndarray) -> np.ndarray:

@@ -161,6 +161,85 @@ int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* pas
 /* out[g] = sum over t < partial_stride of grad_partial[g * partial_stride + t], g < n_slots. */
 int cqs_grad_reduce(const double* grad_partial, int32_t n_slots, int64_t partial_stride, double* out, void* stream);
 
+/* ---------------------------------------------------------- batched sweeps */
+
+/*
+ * Many states of different sizes, each running its own pass program with its own parameter
+ * values, advance together: one launch runs pass p of every state whose program has a pass p with
+ * tile size L.  Replaces the per-resolver loop of cirq-core/cirq/sim/simulator_base.py:275-320
+ * (simulate_sweep_iter: resolve, copy the initial state, simulate, one point at a time) and its
+ * expectation-value form cirq-core/cirq/sim/sparse_simulator.py:193-218.
+ * All descriptor arrays below are DEVICE arrays laid out as these structs; cqs_batch_validate
+ * checks HOST copies of them before the caller uploads.
+ */
+typedef struct cqs_batch_pass { /* one pass of one program (352 bytes) */
+  int32_t n_bits;
+  int32_t L;
+  int32_t op_begin; /* into the batch's ops array */
+  int32_t op_count;
+  int32_t max_k;
+  int32_t nouter; /* n_bits - L */
+  int32_t pad[2];
+  int32_t lpos[CQS_MAX_LOCAL];
+  int32_t lpos_out[CQS_MAX_LOCAL];
+  int32_t opos[CQS_MAX_BITS]; /* the n_bits - L positions outside the tile, ascending */
+} cqs_batch_pass;
+
+typedef struct cqs_batch_item { /* one state's share of one launch (40 bytes) */
+  int64_t state;      /* complex64 offset of the state (psi) in the arena */
+  int64_t state2;     /* complex64 offset of phi (adjoint launches) */
+  int64_t mats;       /* complex64 offset of this state's matrix buffer (16-byte aligned) */
+  int64_t partial;    /* adjoint: double offset of this state's [slot][tile] partials */
+  int32_t pass;       /* index into the batch passes */
+  int32_t tile_begin; /* first workgroup of this item; items are sorted by it */
+} cqs_batch_item;
+
+typedef struct cqs_batch_terms { /* one state's Pauli terms (40 bytes) */
+  int64_t state;       /* psi */
+  int64_t state2;      /* phi (cqs_batch_apply_paulisum) */
+  int64_t out;         /* cqs_batch_expect: first partial (complex double) of [term][block] */
+  int32_t n_bits;
+  int32_t term_begin;  /* into xmask / zmask / coef */
+  int32_t n_terms;
+  int32_t block_begin; /* first workgroup; each covers 2^min(chunk_bits, n_bits) amplitudes */
+} cqs_batch_terms;
+
+/* HOST-side consistency check of a launch's descriptors (positions, offsets within the arenas,
+ * contiguous tile numbering, L of every item's pass).  adjoint != 0 also checks state2. */
+int cqs_batch_validate(const cqs_batch_pass* passes, int32_t n_passes, int32_t n_ops, const cqs_batch_item* items,
+                       int32_t n_items, int32_t L, int64_t arena_elems, int64_t mats_elems, int32_t adjoint);
+
+/* |0...0> in every item's state; item tile_begin counts chunks of 2^min(chunk_bits, n_bits). */
+int cqs_batch_init(void* arena, const cqs_batch_item* items, int32_t n_items, int64_t n_chunks,
+                   const cqs_batch_pass* passes, int32_t chunk_bits, void* stream);
+
+/* One forward pass level: n_tiles workgroups of tile size L over the items. */
+int cqs_batch_passes(void* arena, const cqs_batch_item* items, int32_t n_items, int64_t n_tiles, int32_t L,
+                     const cqs_batch_pass* passes, const cqs_op* ops, const void* mats, const int32_t* aux,
+                     void* stream);
+
+/* One reverse-sweep level of adjoint programs (cqs_adjoint_passes for many states). */
+int cqs_batch_adjoint(void* arena, const cqs_batch_item* items, int32_t n_items, int64_t n_tiles, int32_t L,
+                      const cqs_batch_pass* passes, const cqs_op* ops, const void* mats, const int32_t* aux,
+                      double* partial, void* stream);
+
+/* phi = sum_t coef_t P_t psi for every item (cqs_apply_paulisum per state); masks and coef
+ * (complex double pairs) are DEVICE arrays. */
+int cqs_batch_apply_paulisum(void* arena, const cqs_batch_terms* items, int32_t n_items, int64_t n_blocks,
+                             const uint64_t* xmask, const uint64_t* zmask, const double* coef, int32_t chunk_bits,
+                             void* stream);
+
+/* Per item, term and block: partial[out + t * blocks_of_item + b] = sum over the block's indices i
+ * of (-1)^popcount(i & z_t) conj(psi[i ^ x_t]) psi[i] (complex double; coefficients are applied
+ * by the caller).  Reduce with cqs_segment_sum. */
+int cqs_batch_expect(const void* arena, const cqs_batch_terms* items, int32_t n_items, int64_t n_blocks,
+                     const uint64_t* xmask, const uint64_t* zmask, double* partial, int32_t chunk_bits,
+                     void* stream);
+
+/* out[s] = sum of in[seg[s] .. seg[s+1]) in a fixed order, for width = 1 (double) or 2 (complex
+ * double, offsets counted in complex elements).  seg: DEVICE int64[n_seg + 1]. */
+int cqs_segment_sum(const double* in, const int64_t* seg, int32_t n_seg, int32_t width, double* out, void* stream);
+
 /* ---------------------------------------------------------- reductions */
 
 /*

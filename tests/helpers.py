@@ -250,3 +250,69 @@ def fidelity(a: np.ndarray, b: np.ndarray) -> float:
     a = np.asarray(a, dtype=np.complex128).reshape(-1)
     b = np.asarray(b, dtype=np.complex128).reshape(-1)
     return float(abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real))
+
+
+def _pauli_raw(psi, x, z):
+    i = np.arange(psi.size, dtype=np.int64)
+    sign = np.where(np.array([bin(v).count("1") & 1 for v in (i & z)]), -1.0, 1.0)
+    return np.sum(sign * np.conj(psi[i ^ x]) * psi)
+
+
+def run_batch_model(h, mats_arena):
+    """Executes a planned BatchPrograms (cirq_amd.batch) launch by launch with the pass models:
+    returns (raw <P_t> per point and term, flat; gradient slots per point, flat)."""
+    from types import SimpleNamespace
+
+    arena = np.zeros(h.arena_elems, dtype=np.complex128)
+    point_of = {}
+    for i, (c, _) in enumerate(h.points):
+        point_of[int(h.term_items[i]["state"])] = i
+    grads = {i: np.zeros(h.progs[c].n_slots) for i, (c, _) in enumerate(h.points)}
+    raw = None
+
+    def before_adjoint():
+        out = []
+        for i, ti in enumerate(h.term_items):
+            n, st = int(ti["n_bits"]), int(ti["state"])
+            psi = arena[st : st + (1 << n)].copy()
+            phi = np.zeros_like(psi)
+            for t in range(int(ti["term_begin"]), int(ti["term_begin"]) + int(ti["n_terms"])):
+                x, z = int(h.xmask[t]), int(h.zmask[t])
+                out.append(_pauli_raw(psi, x, z))
+                j = np.arange(psi.size, dtype=np.int64)
+                src = j ^ x
+                sign = np.where(np.array([bin(v).count("1") & 1 for v in (src & z)]), -1.0, 1.0)
+                phi += h.coef[t] * sign * psi[src]
+            if h.gradients:
+                st2 = int(ti["state2"])
+                arena[st2 : st2 + (1 << n)] = phi.astype(np.complex64)
+        return np.array(out, dtype=np.complex128)
+
+    for kind, L, off, n_items, n_tiles in h.launches:
+        if kind == "adj" and raw is None:
+            raw = before_adjoint()
+        for it in h.items[off : off + n_items]:
+            B = h.passes[int(it["pass"])]
+            n, st = int(B["n_bits"]), int(it["state"])
+            if kind == "init":
+                arena[st : st + (1 << n)] = 0
+                arena[st] = 1
+                continue
+            row = np.zeros(1, dtype=_lib.PASS_DTYPE)
+            row["L"], row["op_begin"], row["op_count"], row["max_k"] = B["L"], B["op_begin"], B["op_count"], B["max_k"]
+            row["lpos"], row["lpos_out"] = B["lpos"], B["lpos_out"]
+            view = SimpleNamespace(n_bits=n, pass_arr=row, ops_arr=h.ops, aux_arr=h.aux, mats_arr=None)
+            mats = mats_arena[int(it["mats"]) :]
+            if kind == "fwd":
+                arena[st : st + (1 << n)] = run_pass_model(view, arena[st : st + (1 << n)], mats)
+            else:
+                st2 = int(it["state2"])
+                p = point_of[st]
+                psi, phi, g = run_adjoint_model(view, arena[st : st + (1 << n)], arena[st2 : st2 + (1 << n)], mats,
+                                                len(grads[p]))
+                arena[st : st + (1 << n)], arena[st2 : st2 + (1 << n)] = psi, phi
+                grads[p] += g
+    if raw is None:
+        raw = before_adjoint()
+    slots = np.concatenate([grads[i] for i in range(len(h.points))]) if h.gradients else None
+    return raw, slots
