@@ -45,12 +45,68 @@ def parse():
     ap.add_argument("--adjoint-qubits", type=int, default=20)
     ap.add_argument("--adjoint-layers", type=int, default=10)
     ap.add_argument("--adjoint-reps", type=int, default=5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 5 times (reference for HBM byte counters) and exit")
+    ap.add_argument("--no-dm", action="store_true", help="skip the noisy density-matrix leg")
+    ap.add_argument("--dm-qubits", type=int, default=16)
+    ap.add_argument("--dm-layers", type=int, default=20)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
     ap.add_argument("--sweep-circuits", type=int, default=256)
     ap.add_argument("--sweep-points", type=int, default=16)
     ap.add_argument("--sweep-looped-circuits", type=int, default=256,
                     help="circuits whose first point the looped comparison runs (scaled to all points)")
     return ap.parse_args()
+
+
+def dm_leg(args):
+    """Noisy density matrix: 16-qubit line (2^32 complex64 entries, 32 GiB), 20 layers of Rz, Ry per
+    qubit and CZ on alternating pairs, depolarize(1e-3) after every gate, amplitude_damp(1e-3) on
+    every qubit each layer; then Tr(rho sum_i Z_i)."""
+    import torch
+
+    import cirq_amd as cirq
+    from cirq_amd import engine, generators
+    from cirq_amd.paulis import pauli_masks
+
+    n = args.dm_qubits
+    circuit = generators.noisy_line_circuit(n, args.dm_layers, seed=0)
+    qubits = sorted(circuit.all_qubits())
+    sim = cirq.DensityMatrixSimulator()
+    comp = sim._compile(circuit, qubits)
+    qmap = {q: i for i, q in enumerate(qubits)}
+    terms = [pauli_masks(t, qmap, n) for t in generators.sum_z(qubits)]
+    rho = engine.new_state(2 * n, sim._dev())
+    stream = torch.cuda.current_stream()
+
+    def once():
+        engine.check(engine.lib().cqs_init_basis(engine.ptr(rho), 2 * n, 0, engine.stream_handle()), "init")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        comp.dev.run(rho)
+        e1.record(stream)
+        ev = engine.expect_terms(rho, n, True, terms, sim._ws).sum().real.item()
+        return e0.elapsed_time(e1), ev
+
+    once()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms, ev = once()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n_pass = int(comp.packed.n_passes)
+    bytes_per_pass = 2 * 8 * 4**n
+    del rho
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"noisy line {n} qubits x {args.dm_layers} layers, density matrix 2^{2 * n} complex64, "
+        f"{sum(1 for _ in circuit.all_operations())} operations incl. channels",
+        "layers_per_s": args.dm_layers / wall,
+        "passes": n_pass,
+        "passes_ms": ms,
+        "pass_avg_ms": ms / max(n_pass, 1),
+        "pass_hbm_gbs": bytes_per_pass / (ms / 1e3 / max(n_pass, 1)) / 1e9,
+        "tr_rho_sum_z": ev,
+    }
 
 
 def sweep_leg(args):
@@ -198,6 +254,16 @@ def main():
 
     import torch
 
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n, "copies": 5}))
+        return
+
     import cirq_amd as cirq
     from cirq_amd import engine, generators
     from cirq_amd.paulis import pauli_masks
@@ -306,6 +372,8 @@ def main():
         out["extra"]["adjoint"] = adjoint_leg(args)
     if not args.no_sweep:
         out["extra"]["sweep"] = sweep_leg(args)
+    if not args.no_dm:
+        out["extra"]["density_matrix"] = dm_leg(args)
     if not args.no_cpu_baseline:
         from tests.helpers import to_oracle_ops
 

@@ -200,6 +200,9 @@ class AdjointProgram:
         shifted by +1/2 (even rows) and -1/2 (odd rows) in exponent units."""
         base = fill_param_mats(self.fwd, self.evaluator, symvals)[0]
         out = np.broadcast_to(base, (2 * self.n_slots, base.size)).copy()
+        entries: dict = {}
+        for off, r, size in self.fwd.param_entries:
+            entries.setdefault(r.slot, []).append((off, r, size))
         for slot, (ref, k) in enumerate(self.grad_slots):
             if not shift_rule_ok(ref.gate, k):
                 raise ValueError(f"the two-term shift rule does not hold for parameter {k} of {ref.gate!r}")
@@ -207,9 +210,8 @@ class AdjointProgram:
             vv = [np.repeat(v, 2) for v in vals]
             vv[k] = vv[k] + np.array([0.5, -0.5])
             mb = ref.gate._matrix_batch(tuple(vv))
-            for off, r, size in self.fwd.param_entries:
-                if r.slot == ref.slot:  # every lowered op of this gate occurrence
-                    out[2 * slot : 2 * slot + 2, off : off + size] = r.transform(mb).reshape(2, -1)
+            for off, r, size in entries.get(ref.slot, ()):  # every lowered op of this gate occurrence
+                out[2 * slot : 2 * slot + 2, off : off + size] = r.transform(mb).reshape(2, -1)
         return out
 
 

@@ -1074,20 +1074,48 @@ class ParamEvaluator:
         return self._eval(c, symvals)
 
 
+def _batch_group(gate):
+    """Gates whose matrix batch depends only on their parameter values and this key can be
+    evaluated together: one _matrix_batch call over every gate of the group."""
+    from .gates import EigenGate, FSimGate, PhasedXPowGate, PhasedXZGate
+
+    if isinstance(gate, (EigenGate, PhasedXPowGate)):
+        return type(gate), float(gate.global_shift)
+    if isinstance(gate, (PhasedXZGate, FSimGate)):
+        return type(gate), 0.0
+    return None
+
+
 def fill_param_mats(prog: PackedProgram, ev: ParamEvaluator, symvals: np.ndarray) -> np.ndarray:
     """(B, mats) complex64: the program's matrix buffer for each of B resolvers."""
     B = symvals.shape[1]
     out = np.broadcast_to(prog.mats_arr, (B, prog.mats_arr.size)).copy()
+    refs: dict = {}
+    for _off, ref, _size in prog.param_entries:
+        refs.setdefault((id(ref.gate), ref.deriv), ref)
+    groups: dict = {}
+    for key, ref in refs.items():
+        g = _batch_group(ref.gate)
+        groups.setdefault(None if g is None else (g, ref.deriv), []).append(key)
     cache: dict = {}
+    for gk, keys in groups.items():
+        if gk is None:
+            for key in keys:
+                ref = refs[key]
+                vals = tuple(ev.values(e, symvals) for e in ref.exprs)
+                cache[key] = (ref.gate._matrix_batch(vals) if ref.deriv is None
+                              else ref.gate._dmatrix_batch(vals, ref.deriv))
+            continue
+        rep = refs[keys[0]]
+        G = len(keys)
+        vals = tuple(
+            np.concatenate([ev.values(refs[key].exprs[p], symvals) for key in keys]) for p in range(len(rep.exprs))
+        )
+        mb = rep.gate._matrix_batch(vals) if rep.deriv is None else rep.gate._dmatrix_batch(vals, rep.deriv)
+        mb = mb.reshape((G, B) + mb.shape[1:])
+        for i, key in enumerate(keys):
+            cache[key] = mb[i]
     for off, ref, size in prog.param_entries:
-        key = (id(ref.gate), ref.deriv)
-        if key not in cache:
-            vals = tuple(ev.values(e, symvals) for e in ref.exprs)
-            if ref.deriv is None:
-                cache[key] = (vals, ref.gate._matrix_batch(vals))
-            else:
-                cache[key] = (vals, ref.gate._dmatrix_batch(vals, ref.deriv))
-        vals, mb = cache[key]
-        payload = ref.transform(mb)
-        out[:, off : off + size] = payload.reshape(B, -1).astype(np.complex64)
+        payload = ref.transform(cache[(id(ref.gate), ref.deriv)])
+        out[:, off : off + size] = payload.reshape(B, -1)
     return out

@@ -23,7 +23,7 @@ def _log(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("tiles", [(5, 5, 4), (3, 3, 1), (2, 2, 0)])
+@pytest.mark.parametrize("tiles", [(5, 5, 4), (3, 3, 1), (2, 3, 0)])
 def test_adjoint_mixed_circuit_matches_finite_differences(gpu, tiles):
     _log(f"adjoint mixed circuit tiles={tiles}")
     circuit, syms = _mixed_circuit()
