@@ -118,6 +118,7 @@ _SIGNATURES = {
     "cqs_sample_draw": ([_vp, _vp, _i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp], ctypes.c_int),
     "cqs_marginal_probs": ([_vp, _i32, _i32, _vp, _i32, _vp, _vp], ctypes.c_int),
     "cqs_indices_to_bits": ([_vp, _i64, _i32, _vp, _vp], ctypes.c_int),
+    "cqs_permuted_probs": ([_vp, _i32, _i32, _vp, _vp, _vp], ctypes.c_int),
     "cqs_adjoint_passes": ([_vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp], ctypes.c_int),
     "cqs_grad_reduce": ([_vp, _i32, _i64, _vp, _vp], ctypes.c_int),
     "cqs_batch_validate": ([_vp, _i32, _i32, _vp, _i32, _i32, _i64, _i64, _i32], ctypes.c_int),

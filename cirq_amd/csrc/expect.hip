@@ -92,6 +92,82 @@ __global__ __launch_bounds__(64) void k_expect_final(const double* __restrict__ 
   }
 }
 
+// Terms without X or Y (x mask 0): sum_i (-1)^popcount(i & z) |psi_i|^2.  Each thread takes
+// 16 consecutive amplitudes i = 16g + j, so the sign splits into (-1)^popcount(g & z>>4), the same
+// for the 16, times (-1)^popcount(j & z&15), the same for every thread: one FP64 FMA per
+// amplitude and term.  |psi_i|^2 is exact in FP64 up to its final rounding.
+constexpr int kZTerms = 32;
+constexpr int kZBlocks = 2048;  // fixed grid => deterministic reduction order
+struct ZBatch {
+  int count;
+  int pad;
+  uint64_t z[kZTerms];
+};
+
+__global__ __launch_bounds__(256) void k_expect_z(const float2* __restrict__ amps, int n, ZBatch zb,
+                                                  double* __restrict__ partials) {
+  __shared__ double red[4][kZTerms];
+  double acc[kZTerms];
+#pragma unroll
+  for (int t = 0; t < kZTerms; ++t) acc[t] = 0.0;
+  const uint64_t G = 1ull << (n - 4);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < G; g += stride) {
+    const float4* src = reinterpret_cast<const float4*>(amps + 16 * g);
+    double p[16];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float4 v = src[q];
+      p[2 * q] = fma((double)v.x, (double)v.x, (double)v.y * (double)v.y);
+      p[2 * q + 1] = fma((double)v.z, (double)v.z, (double)v.w * (double)v.w);
+    }
+#pragma unroll
+    for (int t = 0; t < kZTerms; ++t) {
+      if (t < zb.count) {
+        const uint64_t z = zb.z[t];
+        const int zl = (int)(z & 15);
+        double w = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) w = fma((__popc(j & zl) & 1) ? -1.0 : 1.0, p[j], w);
+        acc[t] += (__popcll(g & (z >> 4)) & 1) ? -w : w;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < kZTerms; ++t) {
+    if (t < zb.count) {
+      const double sv = wave_sum(acc[t]);
+      if (lane == 0) red[wv][t] = sv;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < zb.count) {
+    const int t = threadIdx.x;
+    partials[(size_t)blockIdx.x * kZTerms + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+  }
+}
+
+struct ZFinal {
+  int count;
+  int nblocks;
+  int out_idx[kZTerms];
+  double cr[kZTerms], ci[kZTerms];
+};
+
+__global__ __launch_bounds__(64) void k_expect_z_final(const double* __restrict__ partials, ZFinal fb,
+                                                       double* __restrict__ out) {
+  for (int k = 0; k < fb.count; ++k) {
+    double sv = 0.0;
+    for (int b = threadIdx.x; b < fb.nblocks; b += 64) sv += partials[(size_t)b * kZTerms + k];
+    sv = wave_sum(sv);
+    if (threadIdx.x == 0) {
+      out[2 * fb.out_idx[k] + 0] = fb.cr[k] * sv;
+      out[2 * fb.out_idx[k] + 1] = fb.ci[k] * sv;
+    }
+  }
+}
+
 constexpr int kApplyTerms = 32;
 struct ApplyBatch {
   int count;
@@ -132,7 +208,7 @@ extern "C" {
 
 size_t cqs_expect_workspace_bytes(int32_t n_terms) {
   (void)n_terms;
-  return sizeof(double) * 2 * kTermsPerLaunch * kRedBlocks;
+  return sizeof(double) * std::max(2 * kTermsPerLaunch * kRedBlocks, kZTerms * kZBlocks);
 }
 
 int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms,
@@ -156,7 +232,30 @@ int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t
   const uint64_t N = 1ull << n_qubits;
   int grid = (int)std::min<uint64_t>(kRedBlocks, (N + 255) / 256);
   if (grid < 1) grid = 1;
+  const int zgrid = n_qubits >= 4 ? (int)std::min<uint64_t>(kZBlocks, ((N >> 4) + 255) / 256) : 0;
   for (int i = 0; i < n_terms;) {
+    if (!is_dm && n_qubits >= 4 && xmask[order[i]] == 0) {
+      ZBatch zb;
+      ZFinal zf;
+      zb.count = 0;
+      zb.pad = 0;
+      while (i < n_terms && zb.count < kZTerms && xmask[order[i]] == 0) {
+        const int t = order[i];
+        zb.z[zb.count] = zmask[t];
+        zf.out_idx[zb.count] = t;
+        zf.cr[zb.count] = coef[2 * t];
+        zf.ci[zb.count] = coef[2 * t + 1];
+        ++zb.count;
+        ++i;
+      }
+      for (int k = zb.count; k < kZTerms; ++k) zb.z[k] = 0;
+      zf.count = zb.count;
+      zf.nblocks = zgrid;
+      hipLaunchKernelGGL(k_expect_z, dim3(zgrid), dim3(256), 0, s, (const float2*)amps, (int)n_qubits, zb,
+                         partials);
+      hipLaunchKernelGGL(k_expect_z_final, dim3(1), dim3(64), 0, s, (const double*)partials, zf, out);
+      continue;
+    }
     TermBatch tb;
     FinalBatch fb;
     tb.x = xmask[order[i]];

@@ -13,6 +13,8 @@
 // (compose(f, g)[p] = f[p] + g[p ^ (f[p] & 1)]).  Chunks that cross a binade edge (a few dozen as
 // the sum climbs from 0 to 1) are summed element by element.  Every map is verified against the
 // exact incoming value before use, so a misclassified chunk only costs time.
+#include <string.h>
+
 #include "common.hpp"
 
 namespace cqs {
@@ -577,6 +579,72 @@ __global__ __launch_bounds__(256) void k_marginal(Loader ld, int n, AxisBits ab,
   if (threadIdx.x == 0) out[row] = S;
 }
 
+
+// out[j] = p(map(j)) where output bit b is input flat bit perm[b] and p = |amp|^2 as NumPy rounds
+// it (state vector) or Re rho[k, k] (density matrix).  One workgroup per tile of 2^T output
+// indices over the output bits S[0..T) (ascending); loads walk the tile with the bits that feed
+// input bits 0..3 lowest (contiguous reads), stores with output bits 0..3 lowest (contiguous
+// writes), and LDS holds the tile in between.
+struct PermTile {
+  int n, T, nouter, is_dm;
+  int S[16];
+  int A[16];  // A[k]: tile bit (index into S) of load-order bit k
+  int O[CQS_MAX_BITS];
+  int perm[CQS_MAX_BITS];
+};
+
+constexpr int kPermT = 12;
+
+__global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ amps, float* __restrict__ out,
+                                                    PermTile pt) {
+  __shared__ float tile[1 << kPermT];
+  const uint64_t N = 1ull << pt.n;
+  uint64_t base_out = 0, base_in = 0;
+  for (int b = 0; b < pt.nouter; ++b)
+    if ((blockIdx.x >> b) & 1) {
+      base_out |= 1ull << pt.O[b];
+      base_in |= 1ull << pt.perm[pt.O[b]];
+    }
+  const int T = pt.T, cnt = 1 << T;
+  // thread part of the load-order index: bits 0..7 (or fewer when T < 8)
+  const int tb = T < 8 ? T : 8;
+  int lt = 0;
+  uint64_t it = 0;
+  for (int k = 0; k < tb; ++k)
+    if ((threadIdx.x >> k) & 1) {
+      lt |= 1 << pt.A[k];
+      it |= 1ull << pt.perm[pt.S[pt.A[k]]];
+    }
+  const bool act = (int)threadIdx.x < cnt;
+  for (int e = 0; e < (cnt >> tb); ++e) {
+    int le = 0;
+    uint64_t ie = 0;
+    for (int k = tb; k < T; ++k)
+      if ((e >> (k - tb)) & 1) {
+        le |= 1 << pt.A[k];
+        ie |= 1ull << pt.perm[pt.S[pt.A[k]]];
+      }
+    if (act) {
+      const uint64_t i = base_in | it | ie;
+      float v;
+      if (pt.is_dm) {
+        v = amps[i * (N + 1)].x;
+      } else {
+        const float2 a = amps[i];
+        v = fmaf(a.x, a.x, a.y * a.y);
+      }
+      tile[lt | le] = v;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < cnt; j += 256) {
+    uint64_t o = base_out;
+    for (int b = 0; b < T; ++b)
+      if ((j >> b) & 1) o |= 1ull << pt.S[b];
+    out[o] = tile[j];
+  }
+}
+
 }  // namespace cqs
 
 using namespace cqs;
@@ -676,6 +744,65 @@ int cqs_sample_draw(const void* amps, const float* probs_in, int32_t n_qubits, i
   const unsigned g = (unsigned)((reps + 255) / 256);
   hipLaunchKernelGGL(k_draw, dim3(g), dim3(256), 0, as_stream(stream), ld, (const SampleHdr*)(ws + l.o_hdr),
                      l.NC, l.Cl, (const double*)(ws + l.o_cstart), uniforms, reps, out_index);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const int32_t* qubit_axes, float* out,
+                       void* stream) {
+  CQS_REQUIRE(amps != nullptr && out != nullptr && qubit_axes != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_qubits >= 1 && n_qubits <= (is_dm ? 20 : 40), "n_qubits=%d out of range", n_qubits);
+  int32_t perm[64];
+  CQS_REQUIRE(axes_to_perm(qubit_axes, n_qubits, perm), "qubit_axes is not a permutation of 0..%d", n_qubits - 1);
+  const int n = n_qubits;
+  PermTile pt;
+  memset(&pt, 0, sizeof(pt));
+  pt.n = n;
+  pt.is_dm = is_dm ? 1 : 0;
+  for (int b = 0; b < n; ++b) pt.perm[b] = perm[b];
+  const int T = n < kPermT ? n : kPermT;
+  // tile bits: output bits 0..3, the output bits feeding input bits 0..3, then the lowest others
+  bool in_s[64] = {false};
+  int inv[64];
+  for (int b = 0; b < n; ++b) inv[perm[b]] = b;
+  int cnt = 0;
+  auto take = [&](int b) {
+    if (!in_s[b] && cnt < T) {
+      in_s[b] = true;
+      ++cnt;
+    }
+  };
+  for (int b = 0; b < 4 && b < n; ++b) take(b);
+  for (int b = 0; b < 4 && b < n; ++b) take(inv[b]);
+  for (int b = 0; b < n; ++b) take(b);
+  int k = 0;
+  int tile_of[64];
+  for (int b = 0; b < n; ++b)
+    if (in_s[b]) {
+      tile_of[b] = k;
+      pt.S[k++] = b;
+    }
+  pt.T = T;
+  // load order: tile bits feeding input bits 0, 1, 2, ... first
+  bool used[16] = {false};
+  k = 0;
+  for (int q = 0; q < n && k < T; ++q) {
+    const int b = inv[q];
+    if (in_s[b] && !used[tile_of[b]] && q < 4) {
+      used[tile_of[b]] = true;
+      pt.A[k++] = tile_of[b];
+    }
+  }
+  for (int t = 0; t < T; ++t)
+    if (!used[t]) pt.A[k++] = t;
+  pt.nouter = n - T;
+  k = 0;
+  for (int b = 0; b < n; ++b)
+    if (!in_s[b]) pt.O[k++] = b;
+  const uint64_t grid = 1ull << pt.nouter;
+  CQS_REQUIRE(grid < (1ull << 31), "too many tiles");
+  hipLaunchKernelGGL(k_perm_probs, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), (const float2*)amps, out,
+                     pt);
   CQS_HIP(hipGetLastError());
   return CQS_OK;
 }

@@ -158,8 +158,16 @@ class Sampler:
         axes: optional order of all qubits (a permutation) for sampling every qubit."""
         reps = int(uniforms.shape[0])
         w = self.ws.get(lib().cqs_sample_workspace_bytes(n_qubits))
-        src = None if probs is not None else state
         ax = None if axes is None else np.ascontiguousarray(axes, dtype=np.int32)
+        if ax is not None and probs is None:
+            if np.array_equal(ax, np.arange(n_qubits)):
+                ax = None
+            else:
+                # transpose the probabilities once (contiguous reads and writes) instead of
+                # gathering permuted amplitudes in every sampling kernel
+                probs = self.permuted_probs(state, n_qubits, is_dm, ax)
+                ax = None
+        src = None if probs is not None else state
         check(
             lib().cqs_sample_prepare(ptr(src), ptr(probs), n_qubits, int(is_dm), ptr(ax), ptr(w), stream_handle()),
             "cqs_sample_prepare",
@@ -173,6 +181,16 @@ class Sampler:
             "cqs_sample_draw",
         )
         return idx[:reps]
+
+    def permuted_probs(self, state, n_qubits: int, is_dm: bool, axes) -> torch.Tensor:
+        """float32 probabilities of every basis state with the qubits in `axes` order."""
+        if getattr(self, "_pbuf", None) is None or self._pbuf.numel() != (1 << n_qubits):
+            self._pbuf = None
+            self._pbuf = torch.empty(1 << n_qubits, dtype=torch.float32, device=self.device)
+        ax = np.ascontiguousarray(axes, dtype=np.int32)
+        check(lib().cqs_permuted_probs(ptr(state), n_qubits, int(is_dm), ax.ctypes.data, ptr(self._pbuf),
+                                       stream_handle()), "cqs_permuted_probs")
+        return self._pbuf
 
     def bits(self, idx: torch.Tensor, m: int) -> np.ndarray:
         reps = int(idx.shape[0])

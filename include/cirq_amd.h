@@ -318,6 +318,17 @@ int cqs_marginal_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
                        int32_t m, float* out, void* stream);
 
 /*
+ * out[j] (DEVICE float32[2^n_qubits]) = the probability of the basis state whose big-endian digits,
+ * qubit qubit_axes[0] first, spell j: p = float32(|amp|^2) rounded as NumPy rounds it (state
+ * vector) or Re rho[k, k] (density matrix).  The transpose Cirq's product state applies when it
+ * merges factors before sampling (cirq-core/cirq/sim/simulation_product_state.py:68-81, 154-172),
+ * done once with contiguous reads and writes; pass out as probs_in to cqs_sample_prepare / _draw
+ * (with qubit_axes NULL).  qubit_axes: HOST int32[n_qubits], a permutation.
+ */
+int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const int32_t* qubit_axes, float* out,
+                       void* stream);
+
+/*
  * Index -> big-endian digits: out[r*m + k] = bit (m-1-k) of index[r]
  * (cirq-core/cirq/value/digits.py:125-170 as used by cirq-core/cirq/sim/state_vector.py:228-232).
  */

@@ -74,6 +74,40 @@ def test_sampling_bit_exact_with_numpy_choice(gpu):
         assert np.array_equal(bits[:200], want_bits)
 
 
+def test_permuted_probabilities_and_sampling(gpu):
+    """cqs_permuted_probs equals NumPy's float32 p of the transposed state, bit for bit, and
+    sampling every qubit in a permuted order matches numpy's choice on that p."""
+    _log("permuted probabilities")
+    rng = np.random.RandomState(9)
+    for n, seed in [(5, 1), (13, 2), (17, 3)]:
+        c = generators.grid_circuit(3, 6, 6, seed=seed, n_qubits=n)
+        sim = cirq.Simulator()
+        st = sim.simulate(c)._state
+        psi = st.cpu().numpy()
+        axes = list(rng.permutation(n))
+        want_p = (psi.reshape((2,) * n).transpose(axes).reshape(-1) * np.conj(
+            psi.reshape((2,) * n).transpose(axes).reshape(-1))).real.astype(np.float32)
+        got_p = sim._sampler.permuted_probs(st, n, False, axes).cpu().numpy()
+        assert np.array_equal(got_p, want_p), f"n={n}: {np.sum(got_p != want_p)} probabilities differ"
+        p = want_p.astype(np.float64)
+        p = p / p.sum()
+        reps = 5000
+        want = np.random.RandomState(seed).choice(len(p), size=reps, p=want_p / want_p.sum())
+        u = np.random.RandomState(seed).random_sample(reps)
+        idx = sim._sampler.draw(st, n, False, u, axes=axes).cpu().numpy()
+        assert np.array_equal(idx, want)
+    # density-matrix diagonal
+    c = cirq.Circuit(cirq.H.on_each(*cirq.LineQubit.range(4)), cirq.CZ(cirq.LineQubit(0), cirq.LineQubit(3)),
+                     cirq.X(cirq.LineQubit(1)) ** 0.37, cirq.depolarize(0.1).on(cirq.LineQubit(2)))
+    dsim = cirq.DensityMatrixSimulator()
+    rho_t = dsim.simulate(c)._state
+    rho = rho_t.cpu().numpy().reshape(16, 16)
+    axes = [2, 0, 3, 1]
+    diag = np.real(np.diag(rho)).astype(np.float32).reshape((2,) * 4).transpose(axes).reshape(-1)
+    got = dsim._sampler.permuted_probs(rho_t, 4, True, axes).cpu().numpy()
+    assert np.array_equal(got, diag)
+
+
 def test_marginal_sampling_bit_exact(gpu):
     _log("marginal sampling")
     c = generators.grid_circuit(3, 4, 6, seed=4)
@@ -140,6 +174,28 @@ def test_expectations_match_oracle(gpu):
     want = [sum(O.expectation_from_state_vector(psi, t, 12) for t in terms_of(psum, qs))]
     want += [sum(O.expectation_from_state_vector(psi, t, 12) for t in terms_of(cirq.PauliSum.wrap(z), qs)) for z in zz]
     assert np.max(np.abs(np.array(got) - np.array(want))) < 1e-5
+
+
+@pytest.mark.parametrize("n", [4, 5, 9, 16, 21])
+def test_z_only_expectations_exact(gpu, n):
+    """Terms without X/Y take the FP64 sign-split kernel: compare with an FP64 NumPy sum of the
+    same complex64 state (40 random Z strings, more than one launch's worth, plus the norm)."""
+    _log(f"Z-only expectations n={n}")
+    rng = np.random.RandomState(n)
+    psi = (rng.randn(2**n) + 1j * rng.randn(2**n)).astype(np.complex64)
+    psi /= np.linalg.norm(psi)
+    st = torch.from_numpy(psi).to(gpu)
+    zs = [int(rng.randint(0, 2**n)) for _ in range(40)] + [0]
+    coefs = rng.randn(len(zs)) + 1j * rng.randn(len(zs))
+    terms = [(0, z, c) for z, c in zip(zs, coefs)]
+    got = engine.expect_terms(st, n, False, terms, engine.Workspace(gpu)).cpu().numpy()
+    p = np.abs(psi.astype(np.complex128)) ** 2
+    idx = np.arange(2**n)
+    for k, (z, c) in enumerate(zip(zs, coefs)):
+        sign = 1 - 2 * (np.array([bin(v).count("1") for v in (idx & z)]) & 1)
+        want = c * np.sum(sign * p)
+        assert abs(got[k] - want) < 1e-12 * max(1.0, abs(c)) + 1e-13
+    assert abs(engine.norm2(st, n, engine.Workspace(gpu)) - np.sum(p)) < 1e-12
 
 
 def test_density_matrix_matches_oracle(gpu):
