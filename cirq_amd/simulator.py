@@ -34,8 +34,10 @@ from .tape import (
     Measurement,
     ParamEvaluator,
     UnsupportedOperation,
+    _has_channel,
     compile_ops,
     fill_param_mats,
+    kraus_of,
 )
 
 
@@ -148,6 +150,15 @@ class _Compiled:
     def mats_for(self, resolvers) -> np.ndarray:
         symvals = self.evaluator.symbol_values(resolvers)
         return fill_param_mats(self.packed, self.evaluator, symvals)
+
+
+class _QubitsOnly:
+    """The qubit bookkeeping of a _Compiled, for states that several programs act on."""
+
+    def __init__(self, qubits, dm: bool = False):
+        self.qubits = tuple(qubits)
+        self.n = len(self.qubits)
+        self.n_bits = 2 * self.n if dm else self.n
 
 
 class _SimulatorBase:
@@ -352,10 +363,8 @@ class _SimulatorBase:
         prefix, suffix = _split_prefix(circuit, self._can_be_in_run_prefix)
         meas_ops = list(suffix.all_operations())
         if not all(_is_measurement_op(op) for op in meas_ops):
-            raise NotImplementedError(
-                "mid-circuit measurements or noise after measurements need per-repetition simulation, "
-                "which this simulator does not implement"
-            )
+            yield from self._run_trajectories(prefix, suffix, qubits, resolvers, repetitions)
+            return
         if repetitions == 0:
             for r in resolvers:
                 yield ResultDict(params=r, records={op.gate.key: np.empty([0, 1, 1]) for op in meas_ops})
@@ -374,6 +383,83 @@ class _SimulatorBase:
             bits = self._sample_factors(st, comp, factors, measured, repetitions)
             yield ResultDict(params=r, records=self._records(bits, measured, meas_ops))
 
+    def _run_trajectories(self, prefix, suffix, qubits, resolvers, repetitions: int):
+        """Per-repetition simulation of a general suffix after a deterministic prefix, as
+        SimulatorBase._run does (cirq-core/cirq/sim/simulator_base.py:248-273): the prefix state is
+        computed once, every repetition runs the suffix on a copy, and records of a key measured
+        several times are padded to the same count."""
+        comp = self._compile(prefix, qubits)
+        self._check_resolved(resolvers, comp)
+        actions = self._actions(suffix, qubits)
+        keys = []
+        for op in suffix.all_operations():
+            if _is_measurement_op(op) and op.gate.key not in keys:
+                keys.append(op.gate.key)
+        for r, st0 in zip(resolvers, self._run_program(comp, resolvers, None)):
+            per_key: dict = {k: [] for k in keys}
+            for i in range(repetitions):
+                st = st0.clone() if i < repetitions - 1 else st0
+                records: dict = {}
+                for _ in self._run_actions(st, qubits, actions, r, records):
+                    pass
+                for k in keys:
+                    per_key[k].append(records.get(k, []))
+            out = {}
+            for k, reps in per_key.items():
+                width = max((len(x) for rec in reps for x in rec), default=1)
+                largest = max((len(rec) for rec in reps), default=0)
+                xs = np.zeros((repetitions, largest, width), dtype=np.uint8)
+                for i, rec in enumerate(reps):
+                    for j, bits in enumerate(rec):
+                        xs[i, j, : len(bits)] = bits
+                out[k] = xs
+            yield ResultDict(params=r, records=out)
+
+    # ---------------------------------------------------------------- moment steps
+    def simulate_moment_steps(self, circuit, param_resolver=None, qubit_order=QubitOrder.DEFAULT,
+                              initial_state=None) -> Iterator:
+        """One step result per moment (cirq-core/cirq/sim/simulator.py:607-639): measurements in a
+        moment collapse the state and are reported in that step."""
+        r = ParamResolver(param_resolver) if not isinstance(param_resolver, ParamResolver) else param_resolver
+        circuit = self._noisy(circuit)
+        qubits = QubitOrder.as_qubit_order(qubit_order).order_for(circuit.all_qubits())
+        actions = self._actions(circuit, qubits, per_moment=True)
+        for kind, a in actions:
+            if kind == "program":
+                self._check_resolved([r], a)
+        st = self._prepare_state(qubits, initial_state)
+        if len(circuit) == 0:  # one step with the initial state (SimulatorBase._core_iterator)
+            yield self._step_result(st, qubits, {})
+            return
+        records: dict = {}
+        for _ in self._run_actions(st, qubits, actions, r, records):
+            step = self._step_result(st, qubits, {k: (v[0] if len(v) == 1 else np.stack(v))
+                                                  for k, v in records.items()})
+            records = {}
+            yield step
+
+    def _step_result(self, st, qubits, measurements):
+        cls = DensityMatrixStepResult if self._dm else StateVectorStepResult
+        return cls(self, st, qubits, measurements)
+
+    def _sample_state(self, st, qubits, measured, repetitions: int, seed) -> np.ndarray:
+        prng = parse_random_state(seed) if seed is not None else self._prng
+        idx = {q: i for i, q in enumerate(qubits)}
+        axes = [idx[q] for q in measured]
+        n = len(qubits)
+        u = prng.random_sample(repetitions)
+        if axes == list(range(n)):
+            ind = self._sampler.draw(st, n, self._dm, u)
+        elif len(axes) == n:
+            ind = self._sampler.draw(st, n, self._dm, u, axes=axes)
+        else:
+            probs = self._sampler.marginal(st, n, self._dm, axes)
+            ind = self._sampler.draw(None, len(axes), False, u, probs=probs)
+        return self._sampler.bits(ind, len(axes))
+
+    def _prepare_state(self, qubits, initial_state) -> torch.Tensor:
+        return self._prepare(_QubitsOnly(qubits, self._dm), initial_state)
+
     # ---------------------------------------------------------------- SimulatesFinalState
     def simulate(self, program, param_resolver=None, qubit_order=QubitOrder.DEFAULT, initial_state=None):
         return self.simulate_sweep(program, param_resolver, qubit_order, initial_state)[0]
@@ -383,11 +469,21 @@ class _SimulatorBase:
 
     def _measure_terminal(self, st, comp: _Compiled, meas: list) -> dict:
         """Applies measurements at the end of a simulation (collapse + renormalise)."""
-        out = {}
-        idx = {q: i for i, q in enumerate(comp.qubits)}
+        raw = self._measure(st, comp.qubits, meas)
+        return {k: (v[0] if len(v) == 1 else np.stack(v)) for k, v in raw.items()}
+
+    def _measure(self, st, qubits, meas: list) -> dict:
+        """Measures in order, collapsing the state after each (measure_state_vector,
+        cirq-core/cirq/sim/state_vector.py:235-322; measure_density_matrix,
+        cirq-core/cirq/sim/density_matrix_utils.py:97-182): float32 marginal in NumPy's order,
+        normalised, one prng.choice, projection and renormalisation.  Returns key -> bit arrays."""
+        out: dict = {}
+        n = len(qubits)
+        n_bits = 2 * n if self._dm else n
+        idx = {q: i for i, q in enumerate(qubits)}
         for m in meas:
             axes = [idx[q] for q in m.qubits]
-            probs = self._sampler.marginal(st, comp.n, self._dm, axes)
+            probs = self._sampler.marginal(st, n, self._dm, axes)
             p = probs.cpu().numpy().astype(np.float32)
             p = np.clip(p, 0, None)
             p /= p.sum()
@@ -396,25 +492,143 @@ class _SimulatorBase:
             mask = value = 0
             for a, b in zip(axes, bits):
                 if self._dm:
-                    rb, cb = 2 * comp.n - 1 - a, comp.n - 1 - a
+                    rb, cb = 2 * n - 1 - a, n - 1 - a
                     mask |= (1 << rb) | (1 << cb)
                     value |= (b << rb) | (b << cb)
                 else:
-                    bb = comp.n - 1 - a
+                    bb = n - 1 - a
                     mask |= 1 << bb
                     value |= b << bb
             s = 1.0 / p[r] if self._dm else 1.0 / math.sqrt(p[r])
-            engine.collapse(st, comp.n_bits, mask, value, s)
+            engine.collapse(st, n_bits, mask, value, s)
             inv = m.invert_mask
             out.setdefault(m.key, []).append(
                 np.array([b ^ (1 if k < len(inv) and inv[k] else 0) for k, b in enumerate(bits)], dtype=np.uint8)
             )
-        return {k: (v[0] if len(v) == 1 else np.stack(v)) for k, v in out.items()}
+        return out
+
+    # ---------------------------------------------------------------- trajectories
+    def _stochastic(self, op) -> bool:
+        """Ops that consume the prng while the state evolves: measurements, and channels on a state
+        vector (sampled as in cirq-core/cirq/sim/state_vector_simulation_state.py:183-257)."""
+        return _is_measurement_op(op) or (not self._dm and _has_channel(op))
+
+    def _actions(self, circuit, qubits, per_moment: bool = False) -> list:
+        """The circuit as a list of actions: ('program', _Compiled) for runs of deterministic ops,
+        ('op', op) for stochastic ones in circuit order, ('step', None) after every moment when
+        per_moment is set."""
+        key = ("actions", id(circuit), tuple(qubits), per_moment)
+        hit = self._cache.get(key)
+        if hit is not None and hit[0]() is circuit:
+            return hit[1]
+        actions: list = []
+        pending: list = []
+
+        def flush():
+            if pending:
+                c = Circuit.from_moments(*pending)
+                nb = 2 * len(qubits) if self._dm else len(qubits)
+                L, cb = self._tile_params(nb)
+                actions.append(("program", _Compiled(c, qubits, self._dm, self._dev(), L, cb, self._fuse,
+                                                     self._blocks)))
+                pending.clear()
+
+        for moment in circuit:
+            det = [op for op in moment if not self._stochastic(op)]
+            sto = [op for op in moment if self._stochastic(op)]
+            if det:
+                pending.append(Moment(det))
+            if sto or per_moment:
+                flush()
+                actions += [("op", op) for op in sto]
+            if per_moment:
+                actions.append(("step", None))
+        flush()
+        try:
+            ref = weakref.ref(circuit)
+        except TypeError:
+            ref = lambda c=circuit: c
+        self._cache[key] = (ref, actions)
+        return actions
+
+    def _apply_matrix(self, st, qubits, op_qubits, matrix):
+        """st <- matrix (Cirq order over op_qubits) applied to a state vector."""
+        from .tape import LOp
+
+        idx = {q: i for i, q in enumerate(qubits)}
+        n = len(qubits)
+        bits = tuple(n - 1 - idx[q] for q in reversed(op_qubits))
+        packed = compile_ops([LOp("dense", bits, np.asarray(matrix, dtype=np.complex128))], n,
+                             L=min(n, 12), coalesce=min(4, n), fuse=False, blocks=False)
+        engine.DeviceProgram(packed, st.device).run(st)
+
+    def _apply_stochastic(self, st, qubits, op, records: dict):
+        if _is_measurement_op(op):
+            g = op.gate
+            m = Measurement(g.key, tuple(op.qubits), g.full_invert_mask())
+            for k, v in self._measure(st, qubits, [m]).items():
+                records.setdefault(k, []).extend(v)
+            return
+        obj = getattr(op, "gate", None) or op
+        mix = getattr(obj, "_mixture_", None)
+        mixture = mix() if mix is not None else None
+        if mixture is not None and mixture is not NotImplemented:
+            probabilities, unitaries = zip(*mixture)
+            index = self._prng.choice(range(len(unitaries)), p=probabilities)
+            self._apply_matrix(st, qubits, op.qubits, unitaries[index])
+            return
+        kraus = kraus_of(obj)
+        n = len(qubits)
+        p = self._prng.random()
+        fallback_weight, fallback_index = 0.0, 0
+        weight, buf = None, None
+        for index, K in enumerate(kraus):
+            buf = st.clone()
+            self._apply_matrix(buf, qubits, op.qubits, K)
+            weight = engine.norm2(buf, n, self._ws)
+            if weight > fallback_weight:
+                fallback_index, fallback_weight = index, weight
+            p -= weight
+            if p < 0:
+                break
+        if p >= 0 or weight == 0:
+            buf = st.clone()
+            self._apply_matrix(buf, qubits, op.qubits, kraus[fallback_index])
+            weight = fallback_weight
+        engine.scale(buf, n, 1.0 / math.sqrt(weight))
+        st.copy_(buf)
+
+    def _run_actions(self, st, qubits, actions, resolver, records: dict):
+        """Runs actions on st; yields at every ('step', None)."""
+        for kind, a in actions:
+            if kind == "program":
+                if a.has_params:
+                    a.dev.set_mats(a.mats_for([resolver])[0])
+                a.dev.run(st)
+            elif kind == "op":
+                self._apply_stochastic(st, qubits, a, records)
+            else:
+                yield
+
+    def _needs_trajectory(self, circuit) -> bool:
+        if circuit.has_measurements() and not circuit.are_all_measurements_terminal():
+            return True
+        return not self._dm and any(_has_channel(op) for op in circuit.all_operations())
 
     def simulate_sweep_iter(self, program, params, qubit_order=QubitOrder.DEFAULT, initial_state=None):
         resolvers = list(to_resolvers(params))
         circuit = self._noisy(program)
         qubits = QubitOrder.as_qubit_order(qubit_order).order_for(circuit.all_qubits())
+        if self._needs_trajectory(circuit):
+            actions = self._actions(circuit, qubits)
+            for r in resolvers:
+                st = self._prepare_state(qubits, initial_state)
+                records: dict = {}
+                for _ in self._run_actions(st, qubits, actions, r, records):
+                    pass
+                meas = {k: (v[0] if len(v) == 1 else np.stack(v)) for k, v in records.items()}
+                yield self._trial_result(r, meas, st, _QubitsOnly(qubits))
+            return
         body, meas = _split_terminal_measurements(circuit)
         comp = self._compile(body, qubits)
         self._check_resolved(resolvers, comp)
@@ -578,6 +792,40 @@ class StateVectorTrialResult:
 
     def __repr__(self):
         return f"StateVectorTrialResult(params={self.params!r}, measurements={self.measurements!r})"
+
+
+class StateVectorStepResult:
+    """The state after one moment (SparseSimulatorStep, cirq-core/cirq/sim/sparse_simulator.py:221-283)."""
+
+    def __init__(self, sim, state: torch.Tensor, qubits, measurements: dict):
+        self._sim = sim
+        self._state = state.clone()
+        self._qubits = tuple(qubits)
+        self.qubit_map = {q: i for i, q in enumerate(self._qubits)}
+        self.measurements = measurements
+
+    def state_vector(self, copy: bool = True) -> np.ndarray:
+        return self._state.cpu().numpy()
+
+    def dirac_notation(self, decimals: int = 2) -> str:
+        v = self.state_vector()
+        n = len(self._qubits)
+        return " + ".join(f"({np.round(v[i], decimals)})|{i:0{n}b}⟩" for i in np.flatnonzero(np.abs(v) > 10 ** -decimals))
+
+    def sample(self, qubits, repetitions: int = 1, seed=None) -> np.ndarray:
+        """uint8[repetitions, len(qubits)] (sample_state_vector, cirq-core/cirq/sim/state_vector.py:170-232)."""
+        return self._sim._sample_state(self._state, self._qubits, list(qubits), repetitions, seed)
+
+
+class DensityMatrixStepResult(StateVectorStepResult):
+    """The density matrix after one moment (cirq-core/cirq/sim/density_matrix_simulator.py:238-330)."""
+
+    def density_matrix(self, copy: bool = True) -> np.ndarray:
+        n = len(self._qubits)
+        return self._state.cpu().numpy().reshape(2**n, 2**n)
+
+    def state_vector(self, copy: bool = True):
+        raise NotImplementedError("a density-matrix step has no state vector")
 
 
 class DensityMatrixTrialResult:

@@ -1,0 +1,122 @@
+"""GPU tests of moment steps, mid-circuit measurements and noisy state-vector trajectories (the
+behaviour of cirq-core/cirq/sim/sparse_simulator_test.py:215-300,566-630 restated)."""
+
+import sys
+
+import numpy as np
+import pytest
+
+import cirq_amd as cirq
+from oracle import cirq_np as O
+from tests.helpers import to_oracle_ops
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_moment_steps_states(gpu):
+    _log("moment steps")
+    q0, q1 = cirq.LineQubit.range(2)
+    circuit = cirq.Circuit(cirq.H(q0), cirq.H(q1), cirq.H(q0), cirq.H(q1))
+    steps = list(cirq.Simulator().simulate_moment_steps(circuit))
+    assert len(steps) == 2
+    np.testing.assert_allclose(steps[0].state_vector(), [0.5] * 4, atol=1e-6)
+    np.testing.assert_allclose(steps[1].state_vector(), [1, 0, 0, 0], atol=1e-6)
+    steps = list(cirq.Simulator().simulate_moment_steps(cirq.Circuit()))
+    assert len(steps) == 1 and np.allclose(steps[0].state_vector(), [1]) and not steps[0].qubit_map
+
+
+def test_moment_steps_match_oracle_prefixes(gpu):
+    _log("moment steps vs oracle prefixes")
+    from tests.test_host import _random_circuit
+
+    c, qs = _random_circuit(7, 12, np.random.RandomState(3))
+    sim = cirq.Simulator()
+    moments = list(c)
+    for i, step in enumerate(sim.simulate_moment_steps(c, qubit_order=qs)):
+        want = O.simulate_state_vector(to_oracle_ops(cirq.Circuit.from_moments(*moments[: i + 1]), qs), 7)
+        assert np.max(np.abs(step.state_vector() - want)) < 1e-5
+
+
+def test_moment_steps_sample_and_intermediate_measurement(gpu):
+    _log("moment steps sample / intermediate measurement")
+    q0, q1 = cirq.LineQubit.range(2)
+    for i, step in enumerate(cirq.Simulator().simulate_moment_steps(cirq.Circuit(cirq.H(q0), cirq.CNOT(q0, q1)))):
+        s = step.sample([q0, q1], repetitions=10)
+        allowed = ([1, 0], [0, 0]) if i == 0 else ([1, 1], [0, 0])
+        assert all(list(x) in [list(a) for a in allowed] for x in s)
+    for seed in range(4):
+        circuit = cirq.Circuit(cirq.H(q0), cirq.measure(q0), cirq.H(q0))
+        for i, step in enumerate(cirq.Simulator(seed=seed).simulate_moment_steps(circuit)):
+            if i == 1:
+                result = int(step.measurements["q(0)"][0])
+                expected = np.zeros(2)
+                expected[result] = 1
+                np.testing.assert_allclose(step.state_vector(), expected, atol=1e-6)
+            if i == 2:
+                np.testing.assert_allclose(step.state_vector(), [np.sqrt(0.5), np.sqrt(0.5) * (-1) ** result],
+                                           atol=1e-6)
+
+
+def test_run_measurement_not_terminal(gpu):
+    _log("run with non-terminal measurements")
+    q0, q1 = cirq.LineQubit.range(2)
+    sim = cirq.Simulator()
+    for b0 in (0, 1):
+        for b1 in (0, 1):
+            circuit = cirq.Circuit((cirq.X**b0)(q0), (cirq.X**b1)(q1), cirq.measure(q0), cirq.measure(q1),
+                                   cirq.H(q0), cirq.H(q1))
+            result = sim.run(circuit, repetitions=3)
+            np.testing.assert_equal(result.measurements["q(0)"], [[b0]] * 3)
+            np.testing.assert_equal(result.measurements["q(1)"], [[b1]] * 3)
+
+
+def test_mid_circuit_measurement_correlations_and_repeated_keys(gpu):
+    _log("mid-circuit correlations")
+    q0, q1 = cirq.LineQubit.range(2)
+    circuit = cirq.Circuit(cirq.H(q0), cirq.measure(q0, key="a"), cirq.CNOT(q0, q1), cirq.measure(q1, key="b"),
+                           cirq.H(q0), cirq.measure(q0, key="c"), cirq.measure(q0, key="c"))
+    r = cirq.Simulator(seed=5).run(circuit, repetitions=400)
+    a, b, c = r.records["a"], r.records["b"], r.records["c"]
+    assert a.shape == (400, 1, 1) and c.shape == (400, 2, 1)
+    assert np.array_equal(a[:, 0, 0], b[:, 0, 0])
+    assert np.array_equal(c[:, 0, 0], c[:, 1, 0])  # a repeated measurement repeats its outcome
+    assert 120 < a.sum() < 280 and 120 < c[:, 0, 0].sum() < 280
+
+
+def test_state_vector_noise_trajectories(gpu):
+    _log("noisy trajectories")
+    q0 = cirq.LineQubit(0)
+    r = cirq.Simulator(seed=1).run(cirq.Circuit(cirq.bit_flip(0.5)(q0), cirq.measure(q0)), repetitions=100)
+    assert 20 < np.count_nonzero(r.measurements["q(0)"]) < 80
+    r = cirq.Simulator(seed=23).run(cirq.Circuit(cirq.H(q0), cirq.phase_flip(0.5)(q0), cirq.H(q0), cirq.measure(q0)),
+                                    repetitions=100)
+    assert 20 < np.count_nonzero(r.measurements["q(0)"]) < 80
+    # amplitude damping (Kraus, not a mixture): from |1>, P(1) = 1 - gamma
+    circuit = cirq.Circuit(cirq.X(q0), cirq.amplitude_damp(0.3)(q0), cirq.measure(q0))
+    r = cirq.Simulator(seed=2).run(circuit, repetitions=2000)
+    frac = np.mean(r.measurements["q(0)"])
+    assert abs(frac - 0.7) < 0.04
+    # the trajectory average of the final state's populations matches the density matrix
+    c2 = cirq.Circuit(cirq.H(q0), cirq.depolarize(0.3)(q0), cirq.rx(0.4)(q0))
+    rho = cirq.DensityMatrixSimulator().simulate(c2).final_density_matrix
+    sim = cirq.Simulator(seed=3)
+    pops = np.mean([np.abs(sim.simulate(c2).final_state_vector) ** 2 for _ in range(400)], axis=0)
+    assert np.max(np.abs(pops - np.real(np.diag(rho)))) < 0.06
+
+
+def test_density_matrix_mid_circuit_measurement(gpu):
+    _log("density matrix mid-circuit measurement")
+    q0, q1 = cirq.LineQubit.range(2)
+    circuit = cirq.Circuit(cirq.H(q0), cirq.measure(q0, key="m"), cirq.CNOT(q0, q1), cirq.amplitude_damp(0.1)(q1))
+    res = cirq.DensityMatrixSimulator(seed=4).simulate(circuit)
+    m = int(res.measurements["m"][0])
+    rho = res.final_density_matrix
+    assert abs(np.trace(rho) - 1) < 1e-5
+    # q0 is now classical: populations only where q0 == m
+    diag = np.real(np.diag(rho)).reshape(2, 2)
+    assert np.allclose(diag[1 - m], 0, atol=1e-6)
