@@ -58,7 +58,7 @@ int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* pas
                 (unsigned long long)tiles);
     const int mk = ps.max_k;
     const size_t stage = mk >= 3 ? (size_t(1) << (2 * mk)) : 0;
-    const size_t shmem = ((size_t(2) << ps.L) + stage) * sizeof(float2);
+    const size_t shmem = (2 * (size_t)tile_words(ps.L) + stage) * sizeof(float2);
     CQS_REQUIRE(shmem <= 160 * 1024 - 1024, "adjoint pass needs %zu bytes of LDS", shmem);
     if (shmem > 64 * 1024)
       CQS_HIP(hipFuncSetAttribute((const void*)kAdjKernels[ps.L], hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -23,17 +23,16 @@ __device__ __forceinline__ void ew_on_tile(float2* tile, const Prog& P, const cq
   if (!act) return;
   float2 v[PER];
 #pragma unroll
-  for (int e = 0; e < PER; ++e) v[e] = tile[threadIdx.x + e * NT];
-  cqs_op o = op;
-  o.kind = op_kind(op);
-  if (o.kind == CQS_OP_DIAG)
-    ew_diag<PER, NT>(v, P, o, base);
-  else if (o.kind == CQS_OP_PAIRPHASE)
-    ew_pairs<PER, NT>(v, P, o, base);
-  else if (o.kind == CQS_OP_SCALE)
-    ew_scale<PER>(v, P, o);
+  for (int e = 0; e < PER; ++e) v[e] = tile[pad(threadIdx.x + e * NT)];
+  const int kind = op_kind(op);
+  if (kind == CQS_OP_DIAG)
+    ew_diag<PER, NT>(v, P, op, base);
+  else if (kind == CQS_OP_PAIRPHASE)
+    ew_pairs<PER, NT>(v, P, op, base);
+  else if (kind == CQS_OP_SCALE)
+    ew_scale<PER>(v, P, op);
 #pragma unroll
-  for (int e = 0; e < PER; ++e) tile[threadIdx.x + e * NT] = v[e];
+  for (int e = 0; e < PER; ++e) tile[pad(threadIdx.x + e * NT)] = v[e];
 }
 
 // sum over the tile of 2 Re(conj(phi) dU psi) for a dense dU (k <= 3) on tile bits op.t.
@@ -69,7 +68,7 @@ __device__ __forceinline__ double grad_dense(const float2* psi, const float2* ph
     for (int s = 0; s < K; ++s) j0 = insert0(j0, ts[s]);
     float2 x[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) x[c] = psi[j0 + pat[c]];
+    for (int c = 0; c < D; ++c) x[c] = psi[pad(j0 + pat[c])];
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       double yr = 0.0, yi = 0.0;
@@ -79,7 +78,7 @@ __device__ __forceinline__ double grad_dense(const float2* psi, const float2* ph
         yr += (double)m.x * x[c].x - (double)m.y * x[c].y;
         yi += (double)m.x * x[c].y + (double)m.y * x[c].x;
       }
-      const float2 f = phi[j0 + pat[r]];
+      const float2 f = phi[pad(j0 + pat[r])];
       acc += (double)f.x * yr + (double)f.y * yi;  // Re(conj(f) y)
     }
   }
@@ -110,7 +109,7 @@ __device__ __forceinline__ double grad_diag(const float2* psi, const float2* phi
     int idx = fixed;
     for (int m = 0; m < op.k; ++m)
       if ((emask >> m) & 1) idx |= ((e >> (op.t[m] - TB)) & 1) << m;
-    const float2 d = Dg[idx], x = psi[tid + e * NT], f = phi[tid + e * NT];
+    const float2 d = Dg[idx], x = psi[pad(tid + e * NT)], f = phi[pad(tid + e * NT)];
     const double yr = (double)d.x * x.x - (double)d.y * x.y;
     const double yi = (double)d.x * x.y + (double)d.y * x.x;
     acc += (double)f.x * yr + (double)f.y * yi;
@@ -139,16 +138,16 @@ __device__ __forceinline__ void adjoint_tile(float2* __restrict__ psi, float2* _
                                              int64_t stride, int64_t tile, float2* lds, double* red) {
   constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
   float2* tA = lds;
-  float2* tB = lds + (1 << L);
-  float2* mstage = lds + 2 * (1 << L);
+  float2* tB = lds + tile_words(L);
+  float2* mstage = lds + 2 * tile_words(L);
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
   uint64_t off_lo, off_hi[PER];
   tile_offsets<L>(lpos, off_lo, off_hi);
   if (act) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      tA[threadIdx.x + e * NT] = psi[base + off_lo + off_hi[e]];
-      tB[threadIdx.x + e * NT] = phi[base + off_lo + off_hi[e]];
+      tA[pad(threadIdx.x + e * NT)] = psi[base + off_lo + off_hi[e]];
+      tB[pad(threadIdx.x + e * NT)] = phi[base + off_lo + off_hi[e]];
     }
   }
   for (int i = 0; i < P.op_count; ++i) {
@@ -176,23 +175,21 @@ __device__ __forceinline__ void adjoint_tile(float2* __restrict__ psi, float2* _
       continue;
     }
     if (!ok) continue;
-    cqs_op o = op;
-    o.kind = kind;
     if (kind == CQS_OP_DENSE || kind == CQS_OP_BLOCK) {
-      if (sel != kSelPhi) run_mixing_op<NT>(tA, mstage, P, o);
+      if (sel != kSelPhi) run_mixing_op<NT>(tA, mstage, P, op);
       if (sel == kSelBoth) __syncthreads();  // the matrix stage is shared by both tiles
-      if (sel != kSelPsi) run_mixing_op<NT>(tB, mstage, P, o);
+      if (sel != kSelPsi) run_mixing_op<NT>(tB, mstage, P, op);
     } else {
-      if (sel != kSelPhi) ew_on_tile<PER, NT>(tA, P, o, base, act);
-      if (sel != kSelPsi) ew_on_tile<PER, NT>(tB, P, o, base, act);
+      if (sel != kSelPhi) ew_on_tile<PER, NT>(tA, P, op, base, act);
+      if (sel != kSelPsi) ew_on_tile<PER, NT>(tB, P, op, base, act);
     }
   }
   __syncthreads();
   if (act) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      psi[base + off_lo + off_hi[e]] = tA[threadIdx.x + e * NT];
-      phi[base + off_lo + off_hi[e]] = tB[threadIdx.x + e * NT];
+      psi[base + off_lo + off_hi[e]] = tA[pad(threadIdx.x + e * NT)];
+      phi[base + off_lo + off_hi[e]] = tB[pad(threadIdx.x + e * NT)];
     }
   }
 }

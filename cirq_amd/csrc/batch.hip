@@ -198,7 +198,7 @@ static const BatchAdjKernel kBatchAdj[15] = {
 
 // LDS of a launch: every item's pass may need the 4^3 matrix stage, so size for it.
 static size_t batch_lds(int L, bool adjoint) {
-  return ((size_t(adjoint ? 2 : 1) << L) + 64) * sizeof(float2);
+  return ((adjoint ? 2 : 1) * (size_t)tile_words(L) + 64) * sizeof(float2);
 }
 
 template <class K>

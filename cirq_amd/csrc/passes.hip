@@ -191,7 +191,7 @@ int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n
     CQS_REQUIRE(tiles <= 0x7fffffffull, "too many tiles (n=%d, L=%d)", n_bits, ps.L);
     const int mk = ps.max_k;
     const size_t stage = mk >= 3 ? (size_t(1) << (2 * mk)) : 0;
-    const size_t shmem = ((size_t(1) << ps.L) + stage) * sizeof(float2);
+    const size_t shmem = ((size_t)tile_words(ps.L) + stage) * sizeof(float2);
     if (int rc = ensure_pass_lds(ps.L, shmem)) return rc;
     hipLaunchKernelGGL(kPassKernels[ps.L], dim3((unsigned)tiles), dim3(threads_for_L(ps.L)), shmem, s,
                        (float2*)amps, P);

@@ -41,6 +41,13 @@ struct ProgView {
   const int32_t* aux;
 };
 
+// LDS tiles hold one empty slot after every 32 amplitudes: lanes that walk a tile with a stride
+// of 32 amplitudes or more (MFMA block columns when the block's targets are low tile bits, dense
+// groups, PERMUTE writes) then fall into different banks instead of one.
+constexpr int kPadShift = 5;
+__device__ __forceinline__ int pad(int j) { return j + (j >> kPadShift); }
+__host__ __device__ constexpr int tile_words(int L) { return (1 << L) + ((1 << L) >> kPadShift); }
+
 // Bit encoding of DIAG / PAIRPHASE positions: values < kOuter are tile-local indices, values
 // >= kOuter are kOuter + a global position outside the tile.
 constexpr int kOuter = 64;
@@ -86,13 +93,13 @@ __device__ __forceinline__ void tile_dense(float2* tile, int L, const cqs_op& op
     if ((j0 & cm) != cv) continue;
     float2 v[D];
 #pragma unroll
-    for (int m = 0; m < D; ++m) v[m] = tile[j0 + pat[m]];
+    for (int m = 0; m < D; ++m) v[m] = tile[pad(j0 + pat[m])];
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
       for (int c = 0; c < D; ++c) acc = cfma(M[r * D + c], v[c], acc);
-      tile[j0 + pat[r]] = acc;
+      tile[pad(j0 + pat[r])] = acc;
     }
   }
 }
@@ -146,7 +153,7 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_o
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = ks * 8 + 4 * h + e;
-        const float2 v = valid ? tile[jc | rowoff(r)] : make_float2(0.f, 0.f);
+        const float2 v = valid ? tile[pad(jc | rowoff(r))] : make_float2(0.f, 0.f);
         const float xr = v.x * scale, xi = v.y * scale;
         const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
         bhi[2 * e] = hr;
@@ -170,7 +177,7 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_o
 #pragma unroll
         for (int a = 0; a < 16; a += 2) {
           const int i = rb * 32 + (a & 3) + 8 * (a >> 2) + 4 * h;  // even: real part of row i/2
-          tile[jc | rowoff(i >> 1)] = make_float2(acc[rb][a] * inv_scale, acc[rb][a + 1] * inv_scale);
+          tile[pad(jc | rowoff(i >> 1))] = make_float2(acc[rb][a] * inv_scale, acc[rb][a + 1] * inv_scale);
         }
     }
   }
@@ -184,7 +191,7 @@ __device__ __forceinline__ void stage_matrix(float2* dst, const float2* src, int
 // Dense (VALU, k <= 3) or block (MFMA, k = 4, 5) op on the LDS tile.  Caller fences before.
 template <int NT, class Prog>
 __device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, const Prog& P, const cqs_op& op) {
-  if (op.kind == CQS_OP_BLOCK) {
+  if ((op.kind & 0xff) == CQS_OP_BLOCK) {  // bits 8-9 may carry an adjoint tile selector
     const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat);
     if (op.k == 4)
       tile_block_mfma<4, NT>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
@@ -346,14 +353,14 @@ template <int PER, int NT>
 __device__ __forceinline__ void tile_to_regs(float2 (&v)[PER], const float2* tile, bool act) {
   if (act) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) v[e] = tile[threadIdx.x + e * NT];
+    for (int e = 0; e < PER; ++e) v[e] = tile[pad(threadIdx.x + e * NT)];
   }
 }
 template <int PER, int NT>
 __device__ __forceinline__ void regs_to_tile(const float2 (&v)[PER], float2* tile, bool act) {
   if (act) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) tile[threadIdx.x + e * NT] = v[e];
+    for (int e = 0; e < PER; ++e) tile[pad(threadIdx.x + e * NT)] = v[e];
   }
 }
 
@@ -376,7 +383,7 @@ __device__ __forceinline__ void regs_to_tile_permuted(const float2 (&v)[PER], fl
 #pragma unroll
       for (int r = 0; r < NR; ++r)
         if ((e >> r) & 1) j |= eb[r];
-      tile[j] = v[e];
+      tile[pad(j)] = v[e];
     }
   }
 }
@@ -455,7 +462,7 @@ __device__ __forceinline__ void pass_tile(float2* __restrict__ psi, const Prog& 
                                           const int32_t* lpos_out, uint64_t base, float2* lds) {
   constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
   float2* tile = lds;
-  float2* mstage = lds + (1 << L);
+  float2* mstage = lds + tile_words(L);
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
   float2 v[PER];
   {
