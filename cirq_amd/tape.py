@@ -114,10 +114,16 @@ def kraus_of(obj) -> list[np.ndarray]:
 
 def _unitary_batch_fn(gate):
     """Returns (exprs, fn(vals)->(B,d,d)) for a gate, or raises UnsupportedOperation."""
+    from .gates import Gate
+
     exprs = tuple(gate._param_exprs()) if hasattr(gate, "_param_exprs") else ()
-    if hasattr(gate, "_matrix_batch"):
+    mb = getattr(type(gate), "_matrix_batch", None)
+    if mb is not None and mb is not Gate._matrix_batch:
         return exprs, gate._matrix_batch
-    u = getattr(gate, "_unitary_", lambda: NotImplemented)()
+    try:
+        u = getattr(gate, "_unitary_", lambda: NotImplemented)()
+    except NotImplementedError:
+        u = NotImplemented
     if u is NotImplemented or u is None:
         raise UnsupportedOperation(f"{gate!r} has no unitary")
     u = np.asarray(u, dtype=np.complex128)

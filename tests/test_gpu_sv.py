@@ -260,7 +260,15 @@ def test_norm_and_inverse_round_trip_at_26_qubits(gpu):
 
 def test_unsupported_ops_raise_type_error(gpu):
     q = cirq.LineQubit.range(2)
+
+    class Opaque(cirq.Gate):  # no unitary, mixture or Kraus operators
+        def _num_qubits_(self):
+            return 1
+
     with pytest.raises(TypeError):
-        cirq.Simulator().simulate(cirq.Circuit(cirq.depolarize(0.1).on(q[0])))
+        cirq.Simulator().simulate(cirq.Circuit(Opaque().on(q[0])))
     with pytest.raises(ValueError):
         cirq.Simulator().run(cirq.Circuit(cirq.H(q[0])))
+    # a channel on a state vector is sampled as one trajectory (a normalised state)
+    psi = cirq.Simulator(seed=1).simulate(cirq.Circuit(cirq.depolarize(0.1).on(q[0]))).final_state_vector
+    assert abs(np.linalg.norm(psi) - 1) < 1e-6
