@@ -282,10 +282,12 @@ __device__ __forceinline__ void reg_dense_k(float2 (&v)[PER], const float2* __re
   constexpr int D = 1 << K;
   constexpr int RS[3] = {R0, R1, R2};
   constexpr int TMASK = (1 << R0) | (K > 1 ? 1 << R1 : 0) | (K > 2 ? 1 << R2 : 0);
+  const bool ctl = cm != 0;  // uniform: the per-group test below is skipped by a scalar branch
+  const int tm = (int)threadIdx.x & cm;
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     if (e & TMASK) continue;
-    if (((threadIdx.x + e * NT) & cm) != cv) continue;
+    if (ctl && (tm | ((e * NT) & cm)) != cv) continue;
     int idx[D];
 #pragma unroll
     for (int m = 0; m < D; ++m) {
@@ -326,18 +328,20 @@ __device__ __forceinline__ void reg_dense(float2 (&v)[PER], const Prog& P, const
   const int cm = op.cl_mask, cv = op.cl_val;
   const int r0 = op.t[0] - TB, r1 = op.t[1] - TB, r2 = op.t[2] - TB;
   const int code = op.k == 1 ? r0 : (op.k == 2 ? 16 + r0 * 4 + r1 : 32 + r0 * 16 + r1 * 4 + r2);
+#define CQS_RK(K, A, B, C) reg_dense_k<PER, NT, K, A, B, C>(v, M, cm, cv);
 #define CQS_R1(A) \
-  case A: if constexpr (NR > A) reg_dense_k<PER, NT, 1, A, 0, 0>(v, M, cm, cv); return;
+  case A: if constexpr (NR > A) { CQS_RK(1, A, 0, 0) } return;
 #define CQS_R2(A, B) \
-  case 16 + A * 4 + B: if constexpr (NR > B) reg_dense_k<PER, NT, 2, A, B, 0>(v, M, cm, cv); return;
+  case 16 + A * 4 + B: if constexpr (NR > B) { CQS_RK(2, A, B, 0) } return;
 #define CQS_R3(A, B, C) \
-  case 32 + A * 16 + B * 4 + C: if constexpr (NR > C) reg_dense_k<PER, NT, 3, A, B, C>(v, M, cm, cv); return;
+  case 32 + A * 16 + B * 4 + C: if constexpr (NR > C) { CQS_RK(3, A, B, C) } return;
   switch (code) {
     CQS_R1(0) CQS_R1(1) CQS_R1(2) CQS_R1(3)
     CQS_R2(0, 1) CQS_R2(0, 2) CQS_R2(0, 3) CQS_R2(1, 2) CQS_R2(1, 3) CQS_R2(2, 3)
     CQS_R3(0, 1, 2) CQS_R3(0, 1, 3) CQS_R3(0, 2, 3) CQS_R3(1, 2, 3)
     default: return;
   }
+#undef CQS_RK
 #undef CQS_R1
 #undef CQS_R2
 #undef CQS_R3

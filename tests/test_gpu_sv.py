@@ -211,6 +211,21 @@ def test_density_matrix_matches_oracle(gpu):
     assert isinstance(ev, float) and abs(ev - want_ev) < 1e-5
 
 
+def test_noisy_line_density_matrix_8_qubits(gpu):
+    """The noisy-line density-matrix workload (Rz, Ry, alternating CZ, depolarize after every gate,
+    amplitude damping each layer) at 8 qubits x 20 layers: a 16-bit index over several passes."""
+    _log("noisy line 8q x 20")
+    nc = generators.noisy_line_circuit(8, 20, seed=0)
+    qs = sorted(nc.all_qubits())
+    sim = cirq.DensityMatrixSimulator()
+    got = sim.simulate(nc).final_density_matrix
+    want = O.simulate_density_matrix(to_oracle_ops(nc, qs), 8)
+    assert np.linalg.norm(got - want) < 1e-5
+    ev = sim.simulate_expectation_values(nc, [generators.sum_z(qs)])[0]
+    want_ev = sum(O.expectation_from_density_matrix(want, t, 8) for t in terms_of(generators.sum_z(qs), qs))
+    assert abs(ev - want_ev) < 1e-5
+
+
 @pytest.mark.parametrize("seed", range(3))
 def test_density_matrix_random_channels(gpu, seed):
     rng = np.random.RandomState(200 + seed)
