@@ -435,7 +435,7 @@ class _SimulatorBase:
         for _ in self._run_actions(st, qubits, actions, r, records):
             step = self._step_result(st, qubits, {k: (v[0] if len(v) == 1 else np.stack(v))
                                                   for k, v in records.items()})
-            records = {}
+            records.clear()  # the running generator keeps writing into this dict
             yield step
 
     def _step_result(self, st, qubits, measurements):
