@@ -13,12 +13,12 @@ namespace cqs {
 
 constexpr int kSelBoth = 0, kSelPsi = 1, kSelPhi = 2;
 
-__device__ __forceinline__ int op_kind(const cqs_op& op) { return op.kind & 0xff; }
-__device__ __forceinline__ int op_sel(const cqs_op& op) { return (op.kind >> 8) & 3; }
+__device__ __forceinline__ int op_kind(sop_t& op) { return op.kind & 0xff; }
+__device__ __forceinline__ int op_sel(sop_t& op) { return (op.kind >> 8) & 3; }
 
 // Elementwise op on one LDS tile (this thread's elements).
 template <int PER, int NT, class Prog>
-__device__ __forceinline__ void ew_on_tile(float2* tile, const Prog& P, const cqs_op& op, uint64_t base,
+__device__ __forceinline__ void ew_on_tile(float2* tile, const Prog& P, sop_t& op, uint64_t base,
                                            bool act) {
   if (!act) return;
   float2 v[PER];
@@ -37,8 +37,7 @@ __device__ __forceinline__ void ew_on_tile(float2* tile, const Prog& P, const cq
 
 // sum over the tile of 2 Re(conj(phi) dU psi) for a dense dU (k <= 3) on tile bits op.t.
 template <int K, int NT>
-__device__ __forceinline__ double grad_dense(const float2* psi, const float2* phi, int L, const cqs_op& op,
-                                             const float2* M) {
+__device__ __forceinline__ double grad_dense(const float2* psi, const float2* phi, int L, sop_t& op, smat_t* M) {
   constexpr int D = 1 << K;
   int ts[K];
 #pragma unroll
@@ -74,7 +73,7 @@ __device__ __forceinline__ double grad_dense(const float2* psi, const float2* ph
       double yr = 0.0, yi = 0.0;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
-        const float2 m = M[r * D + c];
+        const float2 m = ldm(M, r * D + c);
         yr += (double)m.x * x[c].x - (double)m.y * x[c].y;
         yi += (double)m.x * x[c].y + (double)m.y * x[c].x;
       }
@@ -87,7 +86,7 @@ __device__ __forceinline__ double grad_dense(const float2* psi, const float2* ph
 
 // sum over the tile of 2 Re(conj(phi_j) d(j) psi_j) for a diagonal derivative d.
 template <int PER, int NT, class Prog>
-__device__ __forceinline__ double grad_diag(const float2* psi, const float2* phi, const Prog& P, const cqs_op& op,
+__device__ __forceinline__ double grad_diag(const float2* psi, const float2* phi, const Prog& P, sop_t& op,
                                             uint64_t base, bool act) {
   if (!act) return 0.0;
   constexpr int TB = __builtin_ctz(NT);
@@ -151,7 +150,7 @@ __device__ __forceinline__ void adjoint_tile(float2* __restrict__ psi, float2* _
     }
   }
   for (int i = 0; i < P.op_count; ++i) {
-    const cqs_op& op = P.ops[i];
+    sop_t& op = scalar_ptr(P.ops)[i];
     __syncthreads();
     const bool ok = outer_controls_ok(op, base);
     const int kind = op_kind(op), sel = op_sel(op);
@@ -160,7 +159,7 @@ __device__ __forceinline__ void adjoint_tile(float2* __restrict__ psi, float2* _
       if (!ok) {
         // the gate does not act on this tile: its derivative contributes nothing here
       } else if (kind == CQS_OP_GRAD_DENSE) {
-        const float2* M = P.mats + op.mat;
+        smat_t* M = scalar_ptr(P.mats) + op.mat;
         if (op.k == 1)
           g = grad_dense<1, NT>(tA, tB, L, op, M);
         else if (op.k == 2)

@@ -48,11 +48,29 @@ constexpr int kPadShift = 5;
 __device__ __forceinline__ int pad(int j) { return j + (j >> kPadShift); }
 __host__ __device__ constexpr int tile_words(int L) { return (1 << L) + ((1 << L) >> kPadShift); }
 
+// Ops, matrices and aux entries are the same for every lane of a workgroup: they are read through
+// the constant address space, so that uniform addresses become scalar loads (the scalar data
+// cache) instead of one vector-memory load per lane with a full memory latency per op.  These
+// pointers are only ever read.
+#define CQS_SCALAR __attribute__((address_space(4)))
+typedef const CQS_SCALAR cqs_op sop_t;
+typedef const CQS_SCALAR float2 smat_t;
+typedef const CQS_SCALAR int32_t saux_t;
+__device__ __forceinline__ sop_t* scalar_ptr(const cqs_op* p) { return (sop_t*)p; }
+__device__ __forceinline__ smat_t* scalar_ptr(const float2* p) { return (smat_t*)p; }
+__device__ __forceinline__ saux_t* scalar_ptr(const int32_t* p) { return (saux_t*)p; }
+// element i of a matrix, from the constant address space or (staged) from LDS
+__device__ __forceinline__ float2 ldm(smat_t* p, int i) {
+  const CQS_SCALAR float* f = (const CQS_SCALAR float*)p;
+  return make_float2(f[2 * i], f[2 * i + 1]);
+}
+__device__ __forceinline__ float2 ldm(const float2* p, int i) { return p[i]; }
+
 // Bit encoding of DIAG / PAIRPHASE positions: values < kOuter are tile-local indices, values
 // >= kOuter are kOuter + a global position outside the tile.
 constexpr int kOuter = 64;
 
-__device__ __forceinline__ bool outer_controls_ok(const cqs_op& op, uint64_t base) {
+__device__ __forceinline__ bool outer_controls_ok(sop_t& op, uint64_t base) {
   const uint64_t m = ((uint64_t)op.co_mask_hi << 32) | op.co_mask_lo;
   const uint64_t v = ((uint64_t)op.co_val_hi << 32) | op.co_val_lo;
   return (base & m) == v;
@@ -61,8 +79,8 @@ __device__ __forceinline__ bool outer_controls_ok(const cqs_op& op, uint64_t bas
 // ---------------------------------------------------------------- dense ops on the LDS tile
 // 2^K x 2^K matrix on tile bits op.t[0..K); M row-major, bit m of the matrix index is tile bit
 // op.t[m].  Each thread owns whole groups, so no barrier is needed inside.
-template <int K, int NT>
-__device__ __forceinline__ void tile_dense(float2* tile, int L, const cqs_op& op, const float2* M) {
+template <int K, int NT, class MPtr>
+__device__ __forceinline__ void tile_dense(float2* tile, int L, sop_t& op, MPtr M) {
   constexpr int D = 1 << K;
   int ts[K];
 #pragma unroll
@@ -98,7 +116,7 @@ __device__ __forceinline__ void tile_dense(float2* tile, int L, const cqs_op& op
     for (int r = 0; r < D; ++r) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int c = 0; c < D; ++c) acc = cfma(M[r * D + c], v[c], acc);
+      for (int c = 0; c < D; ++c) acc = cfma(ldm(M, r * D + c), v[c], acc);
       tile[pad(j0 + pat[r])] = acc;
     }
   }
@@ -114,7 +132,7 @@ typedef float floatx16_t __attribute__((ext_vector_type(16)));
 // different waves never overlap and the block needs no barrier inside.  X is split as
 // hi + lo FP16 after scaling by `scale`; U's split comes from the host in A-operand order.
 template <int K, int NT>
-__device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_op& op, const half8_t* A,
+__device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, const half8_t* A,
                                                 float scale, float inv_scale) {
   constexpr int D = 1 << K;
   constexpr int R = 2 * D;
@@ -184,13 +202,13 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, const cqs_o
 }
 
 template <int NT>
-__device__ __forceinline__ void stage_matrix(float2* dst, const float2* src, int count) {
-  for (int i = threadIdx.x; i < count; i += NT) dst[i] = src[i];
+__device__ __forceinline__ void stage_matrix(float2* dst, smat_t* src, int count) {
+  for (int i = threadIdx.x; i < count; i += NT) dst[i] = ldm(src, i);
 }
 
 // Dense (VALU, k <= 3) or block (MFMA, k = 4, 5) op on the LDS tile.  Caller fences before.
 template <int NT, class Prog>
-__device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, const Prog& P, const cqs_op& op) {
+__device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, const Prog& P, sop_t& op) {
   if ((op.kind & 0xff) == CQS_OP_BLOCK) {  // bits 8-9 may carry an adjoint tile selector
     const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat);
     if (op.k == 4)
@@ -199,7 +217,7 @@ __device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, cons
       tile_block_mfma<5, NT>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
     return;
   }
-  const float2* M = P.mats + op.mat;
+  smat_t* M = scalar_ptr(P.mats) + op.mat;
   switch (op.k) {
     case 1: tile_dense<1, NT>(tile, P.L, op, M); break;
     case 2: tile_dense<2, NT>(tile, P.L, op, M); break;
@@ -214,62 +232,98 @@ __device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, cons
 }
 
 // ---------------------------------------------------------------- elementwise ops in registers
-// v[e] holds tile element j = threadIdx.x + e*NT.
+// v[e] holds tile element j = threadIdx.x + e*NT: tile bits below log2(NT) are thread bits (the
+// same for all of a thread's elements), the rest are register bits (bits of e, the same for every
+// thread).  Both ops below split their bit conditions that way, so the per-element work is one
+// complex multiply with everything else decided once per op.
+
+// Diagonal over k bits (tile-local or outer): index bits from thread / outer bits form `fixed`
+// (per lane); index bits from register bit r contribute w[r] (uniform).
 template <int PER, int NT, class Prog>
-__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, const cqs_op& op, uint64_t base) {
+__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, sop_t& op, uint64_t base) {
   constexpr int TB = __builtin_ctz(NT);
+  constexpr int NR = __builtin_ctz(PER);
   const int tid = threadIdx.x;
-  int fixed = 0;   // index bits known for the whole thread
-  int emask = 0;   // index bits that come from element bits
+  int fixed = 0;
+  int w[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int r = 0; r < (NR > 0 ? NR : 1); ++r) w[r] = 0;
+  int any_reg = 0;
   for (int m = 0; m < op.k; ++m) {
     const int t = op.t[m];
-    if (t >= kOuter)
+    if (t >= kOuter) {
       fixed |= (int)((base >> (t - kOuter)) & 1ull) << m;
-    else if (t < TB)
+    } else if (t < TB) {
       fixed |= ((tid >> t) & 1) << m;
-    else
-      emask |= 1 << m;
+    } else {
+      any_reg = 1;
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        if (t - TB == r) w[r] |= 1 << m;
+    }
   }
   const float2* Dg = P.mats + op.mat;
+  if (!any_reg) {
+    const float2 d = Dg[fixed];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], d);
+    return;
+  }
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
-    int idx = fixed;
-    for (int m = 0; m < op.k; ++m)
-      if ((emask >> m) & 1) idx |= ((e >> (op.t[m] - TB)) & 1) << m;
-    v[e] = cmul(v[e], Dg[idx]);
+    int sub = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if ((e >> r) & 1) sub |= w[r];
+    v[e] = cmul(v[e], Dg[fixed | sub]);
   }
 }
 
+// Products of phases on bit pairs (a CZ layer).  Pairs on thread / outer bits only multiply one
+// per-lane phase; pairs touching register bits multiply the elements whose e has those bits.
 template <int PER, int NT, class Prog>
-__device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const Prog& P, const cqs_op& op, uint64_t base) {
-  const int32_t* pr = P.aux + op.t[0];
-  const float2* ph = P.mats + op.mat;
+__device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const Prog& P, sop_t& op, uint64_t base) {
+  constexpr int TB = __builtin_ctz(NT);
+  saux_t* pr = scalar_ptr(P.aux) + op.t[0];
+  smat_t* ph = scalar_ptr(P.mats) + op.mat;
   const int tid = threadIdx.x;
+  float2 cph = make_float2(1.f, 0.f);
   for (int p = 0; p < op.k; ++p) {
-    const int a = pr[2 * p], b = pr[2 * p + 1];
-    int need = 0;
+    const int ab[2] = {pr[2 * p], pr[2 * p + 1]};
     bool en = true;
-    if (a >= kOuter)
-      en = en && ((base >> (a - kOuter)) & 1ull);
-    else
-      need |= 1 << a;
-    if (b >= kOuter)
-      en = en && ((base >> (b - kOuter)) & 1ull);
-    else
-      need |= 1 << b;
-    if (!en) continue;
-    const float2 f = ph[p];
+    int tneed = 0, rneed = 0;
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-      const int j = tid + e * NT;
-      if ((j & need) == need) v[e] = cmul(v[e], f);
+    for (int q = 0; q < 2; ++q) {
+      const int x = ab[q];
+      if (x >= kOuter)
+        en = en && ((base >> (x - kOuter)) & 1ull);
+      else if (x < TB)
+        tneed |= 1 << x;
+      else
+        rneed |= 1 << (x - TB);
     }
+    if (!en) continue;
+    const bool tc = (tid & tneed) == tneed;
+    const float2 f = ldm(ph, p);
+    if (rneed == 0) {
+      const float2 g = cmul(cph, f);
+      cph = tc ? g : cph;
+      continue;
+    }
+#pragma unroll
+    for (int e = 0; e < PER; ++e)
+      if ((e & rneed) == rneed) {  // uniform test; the lane condition is a select
+        const float2 g = cmul(v[e], f);
+        v[e] = tc ? g : v[e];
+      }
   }
+#pragma unroll
+  for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], cph);
 }
 
 template <int PER, class Prog>
-__device__ __forceinline__ void ew_scale(float2 (&v)[PER], const Prog& P, const cqs_op& op) {
-  const float2 s = P.mats[op.mat];
+__device__ __forceinline__ void ew_scale(float2 (&v)[PER], const Prog& P, sop_t& op) {
+  const float2 s = ldm(scalar_ptr(P.mats), op.mat);
 #pragma unroll
   for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], s);
 }
@@ -278,7 +332,7 @@ __device__ __forceinline__ void ew_scale(float2 (&v)[PER], const Prog& P, const 
 // bit r of the element index e; targets come sorted (R0 < R1 < R2) with matrix index bit m <->
 // target m.  Local controls are tested on the group's base element (controls are never targets).
 template <int PER, int NT, int K, int R0, int R1, int R2>
-__device__ __forceinline__ void reg_dense_k(float2 (&v)[PER], const float2* __restrict__ M, int cm, int cv) {
+__device__ __forceinline__ void reg_dense_k(float2 (&v)[PER], smat_t* M, int cm, int cv) {
   constexpr int D = 1 << K;
   constexpr int RS[3] = {R0, R1, R2};
   constexpr int TMASK = (1 << R0) | (K > 1 ? 1 << R1 : 0) | (K > 2 ? 1 << R2 : 0);
@@ -301,16 +355,16 @@ __device__ __forceinline__ void reg_dense_k(float2 (&v)[PER], const float2* __re
     for (int c = 0; c < D; ++c) x[c] = v[idx[c]];
 #pragma unroll
     for (int r = 0; r < D; ++r) {
-      float2 acc = cmul(M[r * D], x[0]);
+      float2 acc = cmul(ldm(M, r * D), x[0]);
 #pragma unroll
-      for (int c = 1; c < D; ++c) acc = cfma(M[r * D + c], x[c], acc);
+      for (int c = 1; c < D; ++c) acc = cfma(ldm(M, r * D + c), x[c], acc);
       v[idx[r]] = acc;
     }
   }
 }
 
 template <int PER, int NT>
-__device__ __forceinline__ bool is_register_dense(const cqs_op& op) {
+__device__ __forceinline__ bool is_register_dense(sop_t& op) {
   constexpr int TB = __builtin_ctz(NT);
   constexpr int NR = __builtin_ctz(PER);
   if (op.kind != CQS_OP_DENSE || op.k < 1 || op.k > 3 || op.k > NR) return false;
@@ -321,10 +375,10 @@ __device__ __forceinline__ bool is_register_dense(const cqs_op& op) {
 }
 
 template <int PER, int NT, class Prog>
-__device__ __forceinline__ void reg_dense(float2 (&v)[PER], const Prog& P, const cqs_op& op) {
+__device__ __forceinline__ void reg_dense(float2 (&v)[PER], const Prog& P, sop_t& op) {
   constexpr int TB = __builtin_ctz(NT);
   constexpr int NR = __builtin_ctz(PER);
-  const float2* M = P.mats + op.mat;
+  smat_t* M = scalar_ptr(P.mats) + op.mat;
   const int cm = op.cl_mask, cv = op.cl_val;
   const int r0 = op.t[0] - TB, r1 = op.t[1] - TB, r2 = op.t[2] - TB;
   const int code = op.k == 1 ? r0 : (op.k == 2 ? 16 + r0 * 4 + r1 : 32 + r0 * 16 + r1 * 4 + r2);
@@ -348,7 +402,7 @@ __device__ __forceinline__ void reg_dense(float2 (&v)[PER], const Prog& P, const
 }
 
 template <int PER, int NT>
-__device__ __forceinline__ bool is_mixing(const cqs_op& op) {
+__device__ __forceinline__ bool is_mixing(sop_t& op) {
   return (op.kind == CQS_OP_DENSE && !is_register_dense<PER, NT>(op)) || op.kind == CQS_OP_BLOCK;
 }
 
@@ -370,8 +424,8 @@ __device__ __forceinline__ void regs_to_tile(const float2 (&v)[PER], float2* til
 
 // Element j = tid + e*NT goes to tile index sum_b bit_b(j) << dst[b].
 template <int PER, int NT>
-__device__ __forceinline__ void regs_to_tile_permuted(const float2 (&v)[PER], float2* tile, int L,
-                                                      const int32_t* __restrict__ dst, bool act) {
+__device__ __forceinline__ void regs_to_tile_permuted(const float2 (&v)[PER], float2* tile, int L, saux_t* dst,
+                                                      bool act) {
   constexpr int TB = __builtin_ctz(NT);
   constexpr int NR = __builtin_ctz(PER);
   const int tb = L < TB ? L : TB;
@@ -401,11 +455,11 @@ __device__ __forceinline__ void run_program(float2 (&v)[PER], float2* tile, floa
                                             uint64_t base, bool act) {
   int i = 0;
   for (;;) {
-    for (; i < P.op_count && !is_mixing<PER, NT>(P.ops[i]); ++i) {
-      const cqs_op& op = P.ops[i];
+    for (; i < P.op_count && !is_mixing<PER, NT>(scalar_ptr(P.ops)[i]); ++i) {
+      sop_t& op = scalar_ptr(P.ops)[i];
       if (op.kind == CQS_OP_PERMUTE) {
         __syncthreads();  // every thread has read its elements before any is overwritten
-        regs_to_tile_permuted<PER, NT>(v, tile, P.L, P.aux + op.t[0], act);
+        regs_to_tile_permuted<PER, NT>(v, tile, P.L, scalar_ptr(P.aux) + op.t[0], act);
         __syncthreads();
         tile_to_regs<PER, NT>(v, tile, act);
         continue;
@@ -422,9 +476,10 @@ __device__ __forceinline__ void run_program(float2 (&v)[PER], float2* tile, floa
     }
     if (i >= P.op_count) return;
     regs_to_tile<PER, NT>(v, tile, act);
-    for (; i < P.op_count && is_mixing<PER, NT>(P.ops[i]); ++i) {
+    for (; i < P.op_count && is_mixing<PER, NT>(scalar_ptr(P.ops)[i]); ++i) {
       __syncthreads();
-      if (outer_controls_ok(P.ops[i], base)) run_mixing_op<NT>(tile, mstage, P, P.ops[i]);
+      sop_t& op = scalar_ptr(P.ops)[i];
+      if (outer_controls_ok(op, base)) run_mixing_op<NT>(tile, mstage, P, op);
     }
     __syncthreads();
     tile_to_regs<PER, NT>(v, tile, act);
