@@ -175,9 +175,9 @@ __device__ __forceinline__ void adjoint_tile(float2* __restrict__ psi, float2* _
     }
     if (!ok) continue;
     if (kind == CQS_OP_DENSE || kind == CQS_OP_BLOCK) {
-      if (sel != kSelPhi) run_mixing_op<NT>(tA, mstage, P, op);
+      if (sel != kSelPhi) run_mixing_op<NT>(tA, mstage, P, op, base);
       if (sel == kSelBoth) __syncthreads();  // the matrix stage is shared by both tiles
-      if (sel != kSelPsi) run_mixing_op<NT>(tB, mstage, P, op);
+      if (sel != kSelPsi) run_mixing_op<NT>(tB, mstage, P, op, base);
     } else {
       if (sel != kSelPhi) ew_on_tile<PER, NT>(tA, P, op, base, act);
       if (sel != kSelPsi) ew_on_tile<PER, NT>(tB, P, op, base, act);

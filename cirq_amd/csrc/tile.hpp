@@ -70,6 +70,16 @@ __device__ __forceinline__ float2 ldm(const float2* p, int i) { return p[i]; }
 // >= kOuter are kOuter + a global position outside the tile.
 constexpr int kOuter = 64;
 
+// Fused DENSE / BLOCK ops may hold several matrices, one per value of up to 3 outer bits: t[5] is
+// count << 24 | the bits' flat positions (7 bits each); the tile's outer bits pick the variant.
+__device__ __forceinline__ int variant_of(sop_t& op, uint64_t base) {
+  const int code = op.t[5];
+  const int cnt = (code >> 24) & 3;
+  int v = 0;
+  for (int i = 0; i < cnt; ++i) v |= (int)((base >> ((code >> (7 * i)) & 127)) & 1ull) << i;
+  return v;
+}
+
 __device__ __forceinline__ bool outer_controls_ok(sop_t& op, uint64_t base) {
   const uint64_t m = ((uint64_t)op.co_mask_hi << 32) | op.co_mask_lo;
   const uint64_t v = ((uint64_t)op.co_val_hi << 32) | op.co_val_lo;
@@ -208,16 +218,18 @@ __device__ __forceinline__ void stage_matrix(float2* dst, smat_t* src, int count
 
 // Dense (VALU, k <= 3) or block (MFMA, k = 4, 5) op on the LDS tile.  Caller fences before.
 template <int NT, class Prog>
-__device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, const Prog& P, sop_t& op) {
+__device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, const Prog& P, sop_t& op,
+                                              uint64_t base) {
+  const int var = variant_of(op, base);
   if ((op.kind & 0xff) == CQS_OP_BLOCK) {  // bits 8-9 may carry an adjoint tile selector
-    const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat);
+    const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat + var * (2 << (2 * op.k)));
     if (op.k == 4)
       tile_block_mfma<4, NT>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
     else if (op.k == 5)
       tile_block_mfma<5, NT>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
     return;
   }
-  smat_t* M = scalar_ptr(P.mats) + op.mat;
+  smat_t* M = scalar_ptr(P.mats) + op.mat + var * (1 << (2 * op.k));
   switch (op.k) {
     case 1: tile_dense<1, NT>(tile, P.L, op, M); break;
     case 2: tile_dense<2, NT>(tile, P.L, op, M); break;
@@ -375,10 +387,10 @@ __device__ __forceinline__ bool is_register_dense(sop_t& op) {
 }
 
 template <int PER, int NT, class Prog>
-__device__ __forceinline__ void reg_dense(float2 (&v)[PER], const Prog& P, sop_t& op) {
+__device__ __forceinline__ void reg_dense(float2 (&v)[PER], const Prog& P, sop_t& op, uint64_t base) {
   constexpr int TB = __builtin_ctz(NT);
   constexpr int NR = __builtin_ctz(PER);
-  smat_t* M = scalar_ptr(P.mats) + op.mat;
+  smat_t* M = scalar_ptr(P.mats) + op.mat + variant_of(op, base) * (1 << (2 * op.k));
   const int cm = op.cl_mask, cv = op.cl_val;
   const int r0 = op.t[0] - TB, r1 = op.t[1] - TB, r2 = op.t[2] - TB;
   const int code = op.k == 1 ? r0 : (op.k == 2 ? 16 + r0 * 4 + r1 : 32 + r0 * 16 + r1 * 4 + r2);
@@ -466,7 +478,7 @@ __device__ __forceinline__ void run_program(float2 (&v)[PER], float2* tile, floa
       }
       if (!outer_controls_ok(op, base)) continue;
       if (op.kind == CQS_OP_DENSE)
-        reg_dense<PER, NT>(v, P, op);
+        reg_dense<PER, NT>(v, P, op, base);
       else if (op.kind == CQS_OP_DIAG)
         ew_diag<PER, NT>(v, P, op, base);
       else if (op.kind == CQS_OP_PAIRPHASE)
@@ -479,7 +491,7 @@ __device__ __forceinline__ void run_program(float2 (&v)[PER], float2* tile, floa
     for (; i < P.op_count && is_mixing<PER, NT>(scalar_ptr(P.ops)[i]); ++i) {
       __syncthreads();
       sop_t& op = scalar_ptr(P.ops)[i];
-      if (outer_controls_ok(op, base)) run_mixing_op<NT>(tile, mstage, P, op);
+      if (outer_controls_ok(op, base)) run_mixing_op<NT>(tile, mstage, P, op, base);
     }
     __syncthreads();
     tile_to_regs<PER, NT>(v, tile, act);

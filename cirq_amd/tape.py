@@ -51,6 +51,7 @@ class LOp:
     pref: ParamRef | None = None
     sel: int = 0  # adjoint programs: 0 both states, 1 psi only, 2 phi only
     slot: int = -1  # adjoint GRAD ops: gradient slot
+    vbits: tuple = ()  # fused dense / block ops: outer bits selecting one of 2^len matrices (mat[v])
 
     @property
     def bits(self) -> frozenset:
@@ -397,22 +398,69 @@ def _valu_cost(op: LOp) -> float:
     return COSTS.get(op.kind, 0.3)
 
 
-class _Block:
-    __slots__ = ("bits", "mat", "members")
+MAX_VARIANT_BITS = 3  # outer bits a fused op may depend on (2^3 matrices per op)
 
-    def __init__(self, bits, mat, members):
-        self.bits = list(bits)
-        self.mat = mat
+
+class _Block:
+    __slots__ = ("bits", "obits", "mats", "members")
+
+    def __init__(self, bits, obits, mats, members):
+        self.bits = list(bits)  # local bits, matrix index bit m <-> bits[m]
+        self.obits = list(obits)  # outer bits; variant index bit i <-> obits[i]
+        self.mats = mats  # (2^len(obits), 2^len(bits), 2^len(bits))
         self.members = list(members)
 
 
+def _variant_matrix(op: LOp, local_bits: list, outer: list, assignment: int) -> np.ndarray:
+    """op's action on local_bits when the outer bits `outer` take the values of `assignment`."""
+    val = {b: (assignment >> i) & 1 for i, b in enumerate(outer)}
+    if op.kind == "dense":
+        lc = [(b, v) for b, v in op.controls if b not in val]
+        if any(val[b] != int(v) for b, v in op.controls if b in val):
+            return np.eye(1 << len(local_bits), dtype=np.complex128)
+        return embed(op.mat, list(op.targets), tuple(lc), local_bits)
+    if op.kind in ("diag", "pair"):
+        if op.kind == "diag":
+            tb, dvals = list(op.targets), np.asarray(op.mat)
+        else:
+            tb, dvals = list(op.targets), np.array([1, 1, 1, op.mat], dtype=np.complex128)
+        lt = [b for b in tb if b not in val]
+        sub = np.empty(1 << len(lt), dtype=np.complex128)
+        for x in range(1 << len(lt)):
+            idx, li = 0, 0
+            for m, b in enumerate(tb):
+                if b in val:
+                    bit = val[b]
+                else:
+                    bit = (x >> li) & 1
+                    li += 1
+                idx |= bit << m
+            sub[x] = dvals[idx]
+        return embed(np.diag(sub), lt, (), local_bits)
+    raise ValueError(op.kind)
+
+
+def _rebase(blk: _Block, bits: list, obits: list) -> np.ndarray:
+    """blk's matrices over (bits, obits): embedded on the local bits, repeated over new outer bits."""
+    out = np.empty((1 << len(obits), 1 << len(bits), 1 << len(bits)), dtype=np.complex128)
+    for a in range(1 << len(obits)):
+        sub = 0
+        for i, b in enumerate(blk.obits):
+            sub |= ((a >> obits.index(b)) & 1) << i
+        out[a] = embed(blk.mats[sub], blk.bits, (), bits)
+    return out
+
+
 def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = True) -> list[LOp]:
-    """Fuses a pass's fixed ops on local bits into blocks of at most kmax bits.
+    """Fuses a pass's fixed ops on local bits into blocks of at most kmax local bits.
 
     Several blocks stay open at once (one per disjoint bit group), so a block keeps absorbing
-    gates on its bits across many cycles until an op links it to bits it cannot take.  A closed
-    block runs as: its original ops (VALU), one fused dense op (k <= 3, VALU) or an MFMA block
-    (k = 4, 5; smaller groups padded), whichever the cost table rates cheapest."""
+    gates on its bits across many cycles until an op links it to bits it cannot take.  Ops that
+    also touch bits outside the tile (a CZ to an outer qubit, a gate controlled by one) are
+    absorbed too: the block then holds one matrix per value of up to MAX_VARIANT_BITS outer bits,
+    and each tile applies the one its outer bits select.  A closed block runs as: its original
+    ops, one fused dense op (k <= 3, VALU) or an MFMA block (k = 4, 5; smaller groups padded),
+    whichever the cost table rates cheapest."""
     out: list[LOp] = []
     open_blocks: list[_Block] = []
     owner: dict = {}
@@ -431,52 +479,63 @@ def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = 
         fused = COSTS[f"dense{nb}"] if nb <= 3 else 1e9
         mf = (COSTS["mfma4"] if nb <= 4 else COSTS["mfma5"]) if use_mfma else 1e9
         best = min(separate, fused, mf)
+        vb = tuple(blk.obits)
+        single = blk.mats[0] if not vb else blk.mats
         if best == separate:
             out.extend(members)
         elif best == fused:
-            out.append(LOp("dense", tuple(blk.bits), blk.mat))
+            out.append(LOp("dense", tuple(blk.bits), single, vbits=vb))
         else:
             bits = list(blk.bits)
-            mat = blk.mat
+            mats = blk.mats
             if nb < 4:
                 extra = [b for b in sorted(local) if b not in bits][: 4 - nb]
-                mat = embed(mat, bits, (), bits + extra)
+                mats = np.stack([embed(m, bits, (), bits + extra) for m in mats])
                 bits = bits + extra
-            out.append(LOp("block", tuple(bits), mat))
+            out.append(LOp("block", tuple(bits), mats[0] if not vb else mats, vbits=vb))
 
     for op in ops:
-        fixed = op.fixed and op.kind in ("dense", "diag", "pair")
-        bits = list(op.targets) + [b for b, _ in op.controls]
-        absorbable = fixed and all(b in local for b in bits) and bits
+        fixed = op.fixed and op.kind in ("dense", "diag", "pair") and not op.vbits
+        bits = list(dict.fromkeys(list(op.targets) + [b for b, _ in op.controls]))
+        lb = [b for b in bits if b in local]
+        ob = [b for b in bits if b not in local]
+        absorbable = fixed and lb and len(lb) <= kmax and len(ob) <= MAX_VARIANT_BITS
+        if absorbable and op.kind == "dense" and any(b not in local for b in op.targets):
+            absorbable = False
         if not absorbable:
             for blk in {id(owner[b]): owner[b] for b in op.bits if b in owner}.values():
                 close(blk)
             out.append(op)
             continue
         touched = []
-        for b in bits:
+        for b in lb:
             blk = owner.get(b)
             if blk is not None and blk not in touched:
                 touched.append(blk)
-        union = []
+        ubits = []
+        uobits = []
         for blk in touched:
-            union += blk.bits
-        union += [b for b in bits if b not in union]
-        if len(union) > kmax:
+            ubits += blk.bits
+            uobits += [b for b in blk.obits if b not in uobits]
+        ubits += [b for b in lb if b not in ubits]
+        uobits += [b for b in ob if b not in uobits]
+        if len(ubits) > kmax or len(uobits) > MAX_VARIANT_BITS:
             for blk in touched:
                 close(blk)
             touched = []
-            union = list(dict.fromkeys(bits))
-        mat = np.eye(1 << len(union), dtype=np.complex128)
+            ubits, uobits = list(lb), list(ob)
+        mats = np.broadcast_to(np.eye(1 << len(ubits), dtype=np.complex128),
+                               (1 << len(uobits), 1 << len(ubits), 1 << len(ubits))).copy()
         members = []
         for blk in touched:
-            mat = embed(blk.mat, blk.bits, (), union) @ mat
+            mats = np.einsum("vij,vjk->vik", _rebase(blk, ubits, uobits), mats)
             members += blk.members
             open_blocks.remove(blk)
-        mat = embed(_op_matrix(op), list(op.targets), op.controls, union) @ mat
-        nb = _Block(union, mat, members + [op])
+        for a in range(1 << len(uobits)):
+            mats[a] = _variant_matrix(op, ubits, uobits, a) @ mats[a]
+        nb = _Block(ubits, uobits, mats, members + [op])
         open_blocks.append(nb)
-        for b in union:
+        for b in ubits:
             owner[b] = nb
     for blk in list(open_blocks):
         close(blk)
@@ -586,6 +645,19 @@ def schedule(ops: list[LOp], n_bits: int, L: int, coalesce: int) -> list[Pass]:
 
 
 # --------------------------------------------------------------------------------------------
+def variant_code(vbits) -> int:
+    """t[5] of a DENSE / BLOCK op whose matrix depends on outer bits: count << 24 | positions, 7
+    bits each; the op's matrices follow each other at `mat`, variant index bit i = flat bit
+    vbits[i] of the tile's outer index."""
+    vbits = tuple(int(b) for b in vbits)
+    if len(vbits) > MAX_VARIANT_BITS or any(not 0 <= b < 128 for b in vbits):
+        raise ValueError(f"bad variant bits {vbits}")
+    code = len(vbits) << 24
+    for i, b in enumerate(vbits):
+        code |= b << (7 * i)
+    return code
+
+
 class PackedProgram:
     """Host/device arrays of a scheduled program in the C ABI layout."""
 
@@ -655,7 +727,11 @@ class PackedProgram:
                         r["cl_mask"], r["cl_val"] = clm, clv
                         d = 2**k
                         payload = op.mat if op.mat is not None else np.zeros((d, d))
-                        r["mat"] = add_mat(payload)
+                        want = ((1 << len(op.vbits)), d, d) if op.vbits else (d, d)
+                        if np.shape(payload) != want:  # the kernel reads exactly this many matrices
+                            raise ValueError(f"dense payload {np.shape(payload)} != {want}")
+                        r["mat"] = add_mat(payload)  # variants, if any, one after the other
+                        r["t"][5] = variant_code(op.vbits)
                         if op.pref is not None:
                             self.param_entries.append((int(r["mat"]), op.pref, d * d))
                             r["slot"] = op.pref.slot
@@ -693,7 +769,11 @@ class PackedProgram:
                         r["t"][m] = local[b]
                     if mat_off % 2:  # 16-byte alignment of the half8 operands
                         add_mat(np.zeros(1))
-                    r["mat"] = add_mat(block_payload(op.mat))
+                    variants = op.mat if op.vbits else [op.mat]
+                    if len(variants) != 1 << len(op.vbits) or np.shape(variants[0]) != (1 << k, 1 << k):
+                        raise ValueError("block payload does not match its variant bits")
+                    r["mat"] = add_mat(np.concatenate([block_payload(m) for m in variants]))
+                    r["t"][5] = variant_code(op.vbits)
                 elif op.kind == "diag":
                     k = len(op.targets)
                     r["kind"] = _lib.OP_DIAG
@@ -813,13 +893,13 @@ def sorted_targets(op: LOp) -> LOp:
     if order == list(range(k)):
         return op
     P = np.array([sum(((i >> m) & 1) << order[m] for m in range(k)) for i in range(1 << k)])
-    mat = None if op.mat is None else op.mat[np.ix_(P, P)]
+    mat = None if op.mat is None else np.asarray(op.mat)[..., P, :][..., :, P]
     ref = op.pref
     if ref is not None:
         base = ref.transform
         ref = ParamRef(ref.gate, lambda b, base=base, P=P: base(b)[..., P, :][..., :, P], ref.exprs, ref.slot,
                        ref.deriv)
-    return LOp(op.kind, tuple(t[m] for m in order), mat, op.controls, ref, op.sel, op.slot)
+    return LOp(op.kind, tuple(t[m] for m in order), mat, op.controls, ref, op.sel, op.slot, op.vbits)
 
 
 _ELEMENTWISE = ("diag", "pair", "pairs", "scale")

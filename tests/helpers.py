@@ -131,20 +131,32 @@ def _apply_op(r, tile, gi, base, L, lpos, mats, mats_c64, aux):
         k = int(r["k"])
         t = [int(x) for x in r["t"][:k]]
         D = 1 << k
-        if kind == _lib.OP_DENSE:
-            M = mats[int(r["mat"]) : int(r["mat"]) + D * D].reshape(D, D)
-        else:
+        # matrix variants selected by outer bits (t[5] = count << 24 | 7-bit positions)
+        code = int(r["t"][5])
+        vpos = [(code >> (7 * i)) & 127 for i in range((code >> 24) & 3)]
+        variant = np.zeros_like(base)
+        for i, b in enumerate(vpos):
+            variant |= ((base >> b) & 1) << i
+        stride = D * D if kind == _lib.OP_DENSE else 2 * D * D
+
+        def matrix(v):
+            o = int(r["mat"]) + v * stride
+            if kind == _lib.OP_DENSE:
+                return mats[o : o + D * D].reshape(D, D)
             from cirq_amd.tape import block_matrix_from_payload
 
-            raw = np.asarray(mats_c64[int(r["mat"]) : int(r["mat"]) + 2 * D * D], dtype=np.complex64)
-            M = block_matrix_from_payload(raw, k)
+            return block_matrix_from_payload(np.asarray(mats_c64[o : o + 2 * D * D], dtype=np.complex64), k)
+
         j = np.arange(1 << L)
         j0 = j[np.all([(j >> b) & 1 == 0 for b in t], axis=0)]
         pat = np.array([sum(((m >> b) & 1) << t[b] for b in range(k)) for m in range(D)])
         idx = j0[:, None] + pat[None, :]  # (G, D)
         grp_ok = (j0 & int(r["cl_mask"])) == int(r["cl_val"])
         V = tile[:, idx]  # (ntiles, G, D)
-        out = V @ M.T
+        out = np.empty_like(V)
+        for v in np.unique(variant):
+            sel = variant == v
+            out[sel] = V[sel] @ matrix(int(v)).T
         ok = tile_ok[:, None, None] & grp_ok[None, :, None]
         tile = tile.copy()
         tile[:, idx] = np.where(ok, out, V)
