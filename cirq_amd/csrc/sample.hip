@@ -150,19 +150,21 @@ __global__ __launch_bounds__(256) void k_sum_buffers(Loader ld, uint64_t len, fl
 }
 
 // Sequential float32 accumulation of the buffer sums (NumPy's reduce loop).
-__global__ void k_total32(const float* __restrict__ bsum, uint64_t nb, SampleHdr* hdr) {
-  if (threadIdx.x != 0) return;
+// The buffer sums are added one after the other in float32 (NumPy's order); the block stages
+// 8192 of them at a time in LDS so that the one adding thread never waits on global memory.
+__global__ __launch_bounds__(256) void k_total32(const float* __restrict__ bsum, uint64_t nb, SampleHdr* hdr) {
+  constexpr int kStage = 8192;
+  __shared__ float st[kStage];
   float S = 0.f;
-  uint64_t c = 0;
-  for (; c + 8 <= nb; c += 8) {
-    float v[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = bsum[c + j];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) S = S + v[j];
+  for (uint64_t c0 = 0; c0 < nb; c0 += kStage) {
+    const int cnt = (int)((nb - c0) < (uint64_t)kStage ? (nb - c0) : (uint64_t)kStage);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += 256) st[i] = bsum[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < cnt; ++i) S = S + st[i];
   }
-  for (; c < nb; ++c) S = S + bsum[c];
-  hdr->T32 = S;
+  if (threadIdx.x == 0) hdr->T32 = S;
 }
 
 __device__ __forceinline__ double px(const Loader& ld, float T32, uint64_t i) {
@@ -361,14 +363,21 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
                                                     const ChunkFn* __restrict__ scfn,
                                                     double* __restrict__ scstart, int* __restrict__ scmode,
                                                     double* __restrict__ cstart) {
+  constexpr int kStage = 1024;  // superchunk maps staged in LDS, so the serial walk reads LDS only
   __shared__ double xs[kChunk];
+  __shared__ ChunkFn sf[kStage];
   const float T32 = hdr->T32;
   const int t = threadIdx.x;
   double S = 0.0;
   int nseq = 0;
   for (uint64_t sc = 0; sc < NSC; ++sc) {
+    if (sc % kStage == 0) {
+      __syncthreads();
+      for (uint64_t i = t; i < kStage && sc + i < NSC; i += 64) sf[i] = scfn[sc + i];
+      __syncthreads();
+    }
     if (t == 0) scstart[sc] = S;
-    const ChunkFn F = scfn[sc];
+    const ChunkFn F = sf[sc % kStage];
     if (F.ok && try_apply(S, F.E, F.d0, F.d1)) {
       if (t == 0) scmode[sc] = 1;
       continue;
@@ -597,7 +606,7 @@ constexpr int kPermT = 12;
 
 __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ amps, float* __restrict__ out,
                                                     PermTile pt) {
-  __shared__ float tile[1 << kPermT];
+  __shared__ float tile[(1 << kPermT) + ((1 << kPermT) >> 5)];  // one pad slot per 32: fewer bank conflicts
   const uint64_t N = 1ull << pt.n;
   uint64_t base_out = 0, base_in = 0;
   for (int b = 0; b < pt.nouter; ++b)
@@ -633,15 +642,23 @@ __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ a
         const float2 a = amps[i];
         v = fmaf(a.x, a.x, a.y * a.y);
       }
-      tile[lt | le] = v;
+      const int w = lt | le;
+      tile[w + (w >> 5)] = v;
     }
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < cnt; j += 256) {
-    uint64_t o = base_out;
-    for (int b = 0; b < T; ++b)
-      if ((j >> b) & 1) o |= 1ull << pt.S[b];
-    out[o] = tile[j];
+  // store order: tile index j = tid + e*256; the thread part of the output offset once per thread
+  uint64_t ot = base_out;
+  for (int b = 0; b < tb; ++b)
+    if ((threadIdx.x >> b) & 1) ot |= 1ull << pt.S[b];
+  for (int e = 0; e < (cnt >> tb); ++e) {
+    uint64_t oe = 0;
+    for (int b = tb; b < T; ++b)
+      if ((e >> (b - tb)) & 1) oe |= 1ull << pt.S[b];
+    if (act) {
+      const int j = (int)threadIdx.x + (e << tb);
+      out[ot | oe] = tile[j + (j >> 5)];
+    }
   }
 }
 
@@ -710,7 +727,7 @@ int cqs_sample_prepare(const void* amps, const float* probs_in, int32_t n_qubits
   hipStream_t s = as_stream(stream);
   const uint64_t buflen = l.N / l.NB;
   hipLaunchKernelGGL(k_sum_buffers, dim3((unsigned)l.NB), dim3(256), 0, s, ld, buflen, csum32);
-  hipLaunchKernelGGL(k_total32, dim3(1), dim3(64), 0, s, csum32, l.NB, hdr);
+  hipLaunchKernelGGL(k_total32, dim3(1), dim3(256), 0, s, csum32, l.NB, hdr);
   const unsigned gch = (unsigned)((l.NC + kChunksPerWG - 1) / kChunksPerWG);
   hipLaunchKernelGGL(k_chunk_approx, dim3(gch), dim3(256), 0, s, ld, (const SampleHdr*)hdr, l.NC, l.Cl, capx);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)l.NSC), dim3(256), 0, s, (const double*)capx, l.SCl, cin,
