@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-dm", action="store_true", help="skip the noisy density-matrix leg")
     ap.add_argument("--dm-qubits", type=int, default=16)
     ap.add_argument("--dm-layers", type=int, default=20)
+    ap.add_argument("--no-dm-adjoint", action="store_true", help="skip the noisy density-matrix gradient leg")
+    ap.add_argument("--dm-adjoint-qubits", type=int, default=12)
+    ap.add_argument("--dm-adjoint-layers", type=int, default=10)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
     ap.add_argument("--sweep-circuits", type=int, default=256)
     ap.add_argument("--sweep-points", type=int, default=16)
@@ -201,6 +204,73 @@ def adjoint_leg(args):
         "value_diff": abs(v - v2),
         "forward_passes": int(ag.prog.fwd.n_passes),
         "reverse_passes": int(ag.prog.rev.n_passes),
+    }
+
+
+def dm_adjoint_leg(args):
+    """Noisy-circuit gradients: the density-matrix workload with every Rz / Ry angle a symbol
+    (12 qubits x 10 layers = 240 symbols, depolarize(1e-3) after every gate, amplitude_damp(1e-3)
+    per qubit per layer), H = sum_i Z_i.  One density-matrix adjoint sweep (weak channels undone in
+    place), the same sweep checkpointing at every channel (sqrt spacing), and the parameter-shift
+    rule through the density-matrix simulator (2 simulations + Tr(H rho) per symbol; the first 8
+    symbols timed and scaled), whose gradients pin the adjoint ones."""
+    import torch
+
+    import cirq_amd
+    from cirq_amd import generators
+    from cirq_amd.dm_adjoint import DMAdjointGradient
+    from cirq_amd.study import ParamResolver
+
+    n, layers = args.dm_adjoint_qubits, args.dm_adjoint_layers
+    circuit, syms, vals = generators.noisy_line_ansatz(n, layers, seed=0)
+    H = generators.sum_z(sorted(circuit.all_qubits()))
+    G = len(syms)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    ga = DMAdjointGradient(circuit, H)
+    t_adj, (v, g) = timed(lambda: ga.value_and_grad(vals), 3)
+    gc = DMAdjointGradient(circuit, H, invert_channels=False)
+    t_ck, (vc, g_ck) = timed(lambda: gc.value_and_grad(vals), 1)
+    K = min(8, G)
+    base = dict(zip(syms, vals))
+    resolvers = []
+    for k in range(K):
+        for sgn in (1, -1):
+            r = dict(base)
+            r[syms[k]] = vals[k] + sgn * np.pi / 2
+            resolvers.append(ParamResolver(r))
+    sim = cirq_amd.DensityMatrixSimulator()
+    t_ps_sample, evs = timed(lambda: sim.simulate_expectation_values_sweep(circuit, H, resolvers), 1)
+    ev = np.array([float(np.real(e[0])) for e in evs])
+    g_ps = (ev[0::2] - ev[1::2]) / 2
+    t_ps = t_ps_sample / (2 * K) * (2 * G)
+    return {
+        "workload": f"noisy line ansatz {n} qubits x {layers} layers, density matrix 2^{2 * n} complex64, "
+        f"{G} symbols, H = sum Z_i",
+        "adjoint_ms": t_adj * 1e3,
+        "adjoint_grads_per_s": G / t_adj,
+        "steps": ga.prog.n_steps,
+        "channels_undone": len(ga.prog.undone),
+        "forward_passes": int(ga.prog.fwd.n_passes),
+        "reverse_passes": int(ga.prog.rev.n_passes),
+        "checkpointed_ms": t_ck * 1e3,
+        "checkpointed_steps": gc.prog.n_steps,
+        "checkpoint_every": gc.prog.checkpoint_every,
+        "param_shift_ms": t_ps * 1e3,
+        "param_shift_sample": f"first {K} symbols ({t_ps_sample:.3f} s for {2 * K} simulations), scaled to {G}",
+        "speedup": t_ps / t_adj,
+        "speedup_checkpointed": t_ps / t_ck,
+        "max_abs_grad_diff_vs_param_shift": float(np.abs(g[:K] - g_ps).max()),
+        "max_abs_grad_diff_checkpointed": float(np.abs(g - g_ck).max()),
+        "value_diff_checkpointed": abs(v - vc),
     }
 
 
@@ -370,6 +440,8 @@ def main():
     }
     if not args.no_adjoint:
         out["extra"]["adjoint"] = adjoint_leg(args)
+    if not args.no_dm_adjoint:
+        out["extra"]["dm_adjoint"] = dm_adjoint_leg(args)
     if not args.no_sweep:
         out["extra"]["sweep"] = sweep_leg(args)
     if not args.no_dm:

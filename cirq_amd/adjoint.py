@@ -19,7 +19,7 @@ import torch
 from . import _lib, engine
 from ._lib import check, lib, ptr
 from .gates import is_parameterized
-from .paulis import PauliSum, pauli_masks
+from .paulis import PauliSum, adjoint_scale, pauli_masks
 from .qubits import QubitOrder
 from .study import ParamResolver
 from .tape import (
@@ -29,6 +29,7 @@ from .tape import (
     PackedProgram,
     ParamEvaluator,
     ParamRef,
+    SlotChain,
     compile_ops,
     embed,
     fill_param_mats,
@@ -49,13 +50,18 @@ def _dagger(kind: str, x):
 
 
 def _inverse(op: LOp, sel: int) -> LOp:
+    """U^dagger on the states `sel` selects.  The one-state inverses of a parameterised op and its
+    GRAD ops must share a pass: GRAD ops fold the controls into their matrix and need them local,
+    so the inverses ask for the same local bits.  (Split across passes, another op's gradient
+    could run while only one of the two states had this inverse applied.)"""
     if op.kind == "scale" and abs(abs(complex(op.mat)) - 1) > 1e-9:
         raise ValueError("adjoint gradients need a unitary circuit (non-unit scale factor)")
     ref = op.pref
     if ref is not None:
         base, kind = ref.transform, op.kind
         ref = ParamRef(ref.gate, lambda b, base=base, kind=kind: _dagger(kind, base(b)), ref.exprs, ref.slot)
-    return LOp(op.kind, op.targets, _dagger(op.kind, op.mat), op.controls, ref, sel)
+    need = tuple(b for b, _ in op.controls) if sel != _lib.SEL_BOTH else ()
+    return LOp(op.kind, op.targets, _dagger(op.kind, op.mat), op.controls, ref, sel, need=need)
 
 
 def _grad_op(op: LOp, k: int, slot: int) -> LOp:
@@ -127,6 +133,8 @@ class AdjointProgram:
                 if q not in qmap:
                     raise KeyError(q)
         self.terms = [pauli_masks(t, qmap, n) for t in psum]
+        self.phi_scale = adjoint_scale(c for _, _, c in self.terms)
+        self.phi_terms = [(x, z, c / self.phi_scale) for x, z, c in self.terms]  # phi = (H / s) psi
         self.evaluator = ParamEvaluator(low.param_refs)
         self.symbols = list(self.evaluator.symbols)
         L = min(tile_bits if tile_bits is not None else 12, n)
@@ -154,14 +162,7 @@ class AdjointProgram:
         self.tile_bits_rev = La
         self.partial_stride = 1 << (n - La)
         self._slot_exprs = [ref.exprs[k] for ref, k in self.grad_slots]
-        index = {sym: i for i, sym in enumerate(self.symbols)}
-        # (slot, symbol index, symbol, expression) for every symbol an expression depends on
-        self._slot_terms = [
-            (slot, index[sym], sym, e)
-            for slot, e in enumerate(self._slot_exprs)
-            for sym in sorted(getattr(e, "free_symbols", ()), key=lambda x: x.name)
-            if sym in index
-        ]
+        self._chain = SlotChain(self._slot_exprs, self.symbols, self.evaluator)
 
     @property
     def n_slots(self) -> int:
@@ -187,13 +188,7 @@ class AdjointProgram:
 
     def chain_batch(self, gs: np.ndarray, symvals: np.ndarray) -> np.ndarray:
         """(B, n_symbols) gradients from (B, n_slots) slot derivatives; symvals (n_symbols, B)."""
-        B = symvals.shape[1]
-        grad = np.zeros((B, len(self.symbols)))
-        for slot, i, sym, e in self._slot_terms:
-            d = self.evaluator.dvalues(e, sym, symvals)
-            if d is not None:
-                grad[:, i] += gs[:, slot] * d
-        return grad
+        return self._chain(gs, symvals)
 
     def shifted_mats(self, symvals: np.ndarray) -> np.ndarray:
         """(2 * n_slots, mats) forward matrix buffers with parameter `slot` of one gate occurrence
@@ -256,7 +251,7 @@ class AdjointGradient:
         self._forward()
         evs = engine.expect_terms(self.psi, n, False, prog.terms, self.ws)
         ev = evs.sum()
-        engine.apply_paulisum(self.psi, self.phi, n, prog.terms)
+        engine.apply_paulisum(self.psi, self.phi, n, prog.phi_terms)
         G = prog.n_slots
         rv = prog.rev
         check(
@@ -269,7 +264,7 @@ class AdjointGradient:
         )
         check(lib().cqs_grad_reduce(ptr(self.partial), G, prog.partial_stride, ptr(self.gslot), engine.stream_handle()),
               "cqs_grad_reduce")
-        gs = self.gslot.cpu().numpy()[:G]
+        gs = self.gslot.cpu().numpy()[:G] * prog.phi_scale
         self.last_term_values = evs.real.cpu().numpy()  # <coef_t P_t> of every term of H
         return float(ev.real.item()), prog.chain(gs, symvals)
 
@@ -302,7 +297,7 @@ class AdjointGradient:
 
 class _ExpectationFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, values: torch.Tensor, grad_engine: AdjointGradient):
+    def forward(ctx, values: torch.Tensor, grad_engine):
         value, grad = grad_engine.value_and_grad(values.detach().cpu().numpy())
         ctx.save_for_backward(torch.as_tensor(grad, dtype=values.dtype, device=values.device))
         return torch.tensor(value, dtype=values.dtype, device=values.device)
@@ -314,11 +309,19 @@ class _ExpectationFn(torch.autograd.Function):
 
 
 class ExpectationLayer(torch.nn.Module):
-    """torch module: symbol values (1-D tensor, in `.symbols` order) -> <H>, differentiable."""
+    """torch module: symbol values (1-D tensor, in `.symbols` order) -> <H>, differentiable.
 
-    def __init__(self, circuit, observable, qubit_order=QubitOrder.DEFAULT, **kw):
+    density_matrix=True evolves rho instead (the circuit may hold channels) and differentiates
+    Tr(H rho) with the checkpointed density-matrix sweep of cirq_amd.dm_adjoint."""
+
+    def __init__(self, circuit, observable, qubit_order=QubitOrder.DEFAULT, *, density_matrix: bool = False, **kw):
         super().__init__()
-        self.engine = AdjointGradient(circuit, observable, qubit_order, **kw)
+        if density_matrix:
+            from .dm_adjoint import DMAdjointGradient
+
+            self.engine = DMAdjointGradient(circuit, observable, qubit_order, **kw)
+        else:
+            self.engine = AdjointGradient(circuit, observable, qubit_order, **kw)
         self.symbols = self.engine.symbols
 
     def forward(self, values: torch.Tensor) -> torch.Tensor:

@@ -19,7 +19,7 @@ import torch
 from . import _lib, engine
 from ._lib import check, lib, ptr
 from .adjoint import AdjointProgram
-from .paulis import PauliSum, pauli_masks
+from .paulis import PauliSum, adjoint_scale, pauli_masks
 from .tape import fill_param_mats
 
 CHUNK_BITS = 12  # amplitudes per workgroup of the init / expectation / H psi kernels: 2^12
@@ -90,17 +90,23 @@ class BatchPrograms:
         self.ops = np.concatenate(ops)
         self.aux = np.concatenate(aux).astype(np.int32)
         self.passes = np.concatenate(rows)
-        xm, zm, cf, tbase = [], [], [], []
+        xm, zm, cf, acf, tbase = [], [], [], [], []
+        self.phi_scale = []
         for masks in self.term_masks:
             tbase.append(len(xm))
+            s = adjoint_scale(co for _, _, co in masks)
+            self.phi_scale.append(s)
             for x, z, co in masks:
                 xm.append(x)
                 zm.append(z)
                 cf.append(complex(co))
+                acf.append(complex(co) / s)
         self.term_base = tbase
         self.xmask = np.array(xm or [0], dtype=np.uint64)
         self.zmask = np.array(zm or [0], dtype=np.uint64)
         self.coef = np.array(cf or [0j], dtype=np.complex128)
+        # phi = (H_c / s_c) psi for the adjoint sweeps; the slot gradients are multiplied by s_c
+        self.apply_coef = np.array(acf or [0j], dtype=np.complex128)
         self.counts = None
 
     def plan(self, counts: tuple):
@@ -224,7 +230,7 @@ class BatchPrograms:
             evs.append((raw_ev[t : t + B * nt].reshape(B, nt) * co[None, :]).real)
             t += B * nt
             if self.gradients:
-                slots = slot_grads[gpos : gpos + B * prog.n_slots].reshape(B, prog.n_slots)
+                slots = slot_grads[gpos : gpos + B * prog.n_slots].reshape(B, prog.n_slots) * self.phi_scale[c]
                 gpos += B * prog.n_slots
                 grads.append(prog.chain_batch(slots, symvals[c]))
             else:
@@ -253,7 +259,7 @@ class BatchedExpectation:
         self.passes = torch.from_numpy(h.passes.view(np.uint8).copy()).to(self.device)
         self.xm = torch.from_numpy(h.xmask.view(np.int64).copy()).to(self.device)
         self.zm = torch.from_numpy(h.zmask.view(np.int64).copy()).to(self.device)
-        self.cf = torch.from_numpy(h.coef.view(np.float64).copy()).to(self.device)
+        self.cf = torch.from_numpy(h.apply_coef.view(np.float64).copy()).to(self.device)
         self._graph = None
 
     def _plan(self, counts: tuple):

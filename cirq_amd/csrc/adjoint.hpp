@@ -23,7 +23,7 @@ __device__ __forceinline__ void ew_on_tile(float2* tile, const Prog& P, sop_t& o
   if (!act) return;
   float2 v[PER];
 #pragma unroll
-  for (int e = 0; e < PER; ++e) v[e] = tile[pad(threadIdx.x + e * NT)];
+  for (int e = 0; e < PER; ++e) v[e] = tile[pad_nat<NT>(e)];
   const int kind = op_kind(op);
   if (kind == CQS_OP_DIAG)
     ew_diag<PER, NT>(v, P, op, base);
@@ -32,7 +32,7 @@ __device__ __forceinline__ void ew_on_tile(float2* tile, const Prog& P, sop_t& o
   else if (kind == CQS_OP_SCALE)
     ew_scale<PER>(v, P, op);
 #pragma unroll
-  for (int e = 0; e < PER; ++e) tile[pad(threadIdx.x + e * NT)] = v[e];
+  for (int e = 0; e < PER; ++e) tile[pad_nat<NT>(e)] = v[e];
 }
 
 // sum over the tile of 2 Re(conj(phi) dU psi) for a dense dU (k <= 3) on tile bits op.t.
@@ -108,7 +108,7 @@ __device__ __forceinline__ double grad_diag(const float2* psi, const float2* phi
     int idx = fixed;
     for (int m = 0; m < op.k; ++m)
       if ((emask >> m) & 1) idx |= ((e >> (op.t[m] - TB)) & 1) << m;
-    const float2 d = Dg[idx], x = psi[pad(tid + e * NT)], f = phi[pad(tid + e * NT)];
+    const float2 d = Dg[idx], x = psi[pad_nat<NT>(e)], f = phi[pad_nat<NT>(e)];
     const double yr = (double)d.x * x.x - (double)d.y * x.y;
     const double yi = (double)d.x * x.y + (double)d.y * x.x;
     acc += (double)f.x * yr + (double)f.y * yi;
@@ -145,8 +145,8 @@ __device__ __forceinline__ void adjoint_tile(float2* __restrict__ psi, float2* _
   if (act) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      tA[pad(threadIdx.x + e * NT)] = psi[base + off_lo + off_hi[e]];
-      tB[pad(threadIdx.x + e * NT)] = phi[base + off_lo + off_hi[e]];
+      tA[pad_nat<NT>(e)] = psi[base + off_lo + off_hi[e]];
+      tB[pad_nat<NT>(e)] = phi[base + off_lo + off_hi[e]];
     }
   }
   for (int i = 0; i < P.op_count; ++i) {
@@ -187,8 +187,8 @@ __device__ __forceinline__ void adjoint_tile(float2* __restrict__ psi, float2* _
   if (act) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      psi[base + off_lo + off_hi[e]] = tA[pad(threadIdx.x + e * NT)];
-      phi[base + off_lo + off_hi[e]] = tB[pad(threadIdx.x + e * NT)];
+      psi[base + off_lo + off_hi[e]] = tA[pad_nat<NT>(e)];
+      phi[base + off_lo + off_hi[e]] = tB[pad_nat<NT>(e)];
     }
   }
 }

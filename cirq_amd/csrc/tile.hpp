@@ -47,6 +47,13 @@ struct ProgView {
 constexpr int kPadShift = 5;
 __device__ __forceinline__ int pad(int j) { return j + (j >> kPadShift); }
 __host__ __device__ constexpr int tile_words(int L) { return (1 << L) + ((1 << L) >> kPadShift); }
+// pad(threadIdx.x + e * NT) for NT a multiple of 2^kPadShift: one per-thread base plus a constant
+// per element (which the LDS instructions take as an immediate offset).
+template <int NT>
+__device__ __forceinline__ int pad_nat(int e) {
+  static_assert(NT % (1 << kPadShift) == 0, "natural layout needs NT a multiple of the pad period");
+  return pad((int)threadIdx.x) + e * (NT + (NT >> kPadShift));
+}
 
 // Ops, matrices and aux entries are the same for every lane of a workgroup: they are read through
 // the constant address space, so that uniform addresses become scalar loads (the scalar data
@@ -423,14 +430,14 @@ template <int PER, int NT>
 __device__ __forceinline__ void tile_to_regs(float2 (&v)[PER], const float2* tile, bool act) {
   if (act) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) v[e] = tile[pad(threadIdx.x + e * NT)];
+    for (int e = 0; e < PER; ++e) v[e] = tile[pad_nat<NT>(e)];
   }
 }
 template <int PER, int NT>
 __device__ __forceinline__ void regs_to_tile(const float2 (&v)[PER], float2* tile, bool act) {
   if (act) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) tile[pad(threadIdx.x + e * NT)] = v[e];
+    for (int e = 0; e < PER; ++e) tile[pad_nat<NT>(e)] = v[e];
   }
 }
 

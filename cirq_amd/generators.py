@@ -153,6 +153,31 @@ def noisy_line_circuit(n: int, layers: int, seed: int = 0, p: float = 1e-3, gamm
     return c
 
 
+def noisy_line_ansatz(n: int, layers: int, seed: int = 0, p: float = 1e-3, gamma: float = 1e-3):
+    """noisy_line_circuit with every Rz / Ry angle a symbol.  Returns (circuit, symbols, values),
+    the values drawn as noisy_line_circuit draws its angles."""
+    rng = np.random.RandomState(seed)
+    qs = LineQubit.range(n)
+    c = Circuit()
+    syms, vals = [], []
+    for layer in range(layers):
+        th = [sympy.Symbol(f"th{layer}_{i}") for i in range(n)]
+        ph = [sympy.Symbol(f"ph{layer}_{i}") for i in range(n)]
+        vals += list(rng.uniform(0, 2 * np.pi, n)) + list(rng.uniform(0, 2 * np.pi, n))
+        syms += th + ph
+        c.append(Moment(g.rz(th[i]).on(q) for i, q in enumerate(qs)))
+        c.append(Moment(depolarize(p).on(q) for q in qs))
+        c.append(Moment(g.ry(ph[i]).on(q) for i, q in enumerate(qs)))
+        c.append(Moment(depolarize(p).on(q) for q in qs))
+        start = layer % 2
+        pairs = [(qs[i], qs[i + 1]) for i in range(start, n - 1, 2)]
+        if pairs:
+            c.append(Moment(g.CZ.on(a, b) for a, b in pairs))
+            c.append(Moment(depolarize(p).on(q) for ab in pairs for q in ab))
+        c.append(Moment(amplitude_damp(gamma).on(q) for q in qs))
+    return c, syms, np.array(vals)
+
+
 def sum_z(qubits) -> PauliSum:
     return PauliSum.from_pauli_strings([PZ(q) for q in qubits])
 

@@ -8,6 +8,8 @@ into observables.
 
 from __future__ import annotations
 
+import math
+
 import numbers
 from typing import Iterable, Mapping
 
@@ -369,6 +371,15 @@ class PauliSum:
 def pauli_sum_like(val) -> PauliSum:
     """Interprets a PauliSumLike (number, PauliString or PauliSum)."""
     return PauliSum.wrap(val)
+
+
+def adjoint_scale(coefs) -> float:
+    """Power of two >= sum |c| (and >= 1).  Adjoint sweeps start from phi = (H / s) psi (or
+    vec(H) / s) and multiply their gradients by s: every op is linear in phi, the division by a
+    power of two is exact, and phi's entries stay within the unit range that the split-FP16 MFMA
+    path scales for (|amplitude| <= 1; a larger entry would overflow FP16 there)."""
+    total = float(sum(abs(complex(c)) for c in coefs))
+    return float(2.0 ** max(0, math.ceil(math.log2(total)))) if total > 1 else 1.0
 
 
 def pauli_masks(term: PauliString, qubit_index: Mapping, n: int):

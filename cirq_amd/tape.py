@@ -52,6 +52,8 @@ class LOp:
     sel: int = 0  # adjoint programs: 0 both states, 1 psi only, 2 phi only
     slot: int = -1  # adjoint GRAD ops: gradient slot
     vbits: tuple = ()  # fused dense / block ops: outer bits selecting one of 2^len matrices (mat[v])
+    fence: bool = False  # scheduling barrier: runs after every earlier op, before every later one
+    need: tuple = ()  # bits that must be local in the op's pass besides its targets
 
     @property
     def bits(self) -> frozenset:
@@ -134,12 +136,16 @@ def _unitary_batch_fn(gate):
 class Lowering:
     """Lowers a circuit into LOps for a state vector (dm=False) or density matrix (dm=True)."""
 
-    def __init__(self, qubits: Sequence, dm: bool):
+    def __init__(self, qubits: Sequence, dm: bool, *, split_dm_diag: bool = False):
         self.qubits = tuple(qubits)
         self.index = {q: i for i, q in enumerate(self.qubits)}
         self.n = len(self.qubits)
         self.dm = dm
+        # density matrix: a diagonal gate as separate row (d) and column (conj d) diagonals, each
+        # linear in the gate's matrix (what gradients need), instead of one d (x) conj(d) diagonal
+        self.split_dm_diag = split_dm_diag
         self.ops: list[LOp] = []
+        self.nonunitary: list[int] = []  # indices into ops of channels and non-unit scale factors
         self.measurements: list[Measurement] = []
         self.param_refs: list[ParamRef] = []
         self.n_slots = 0
@@ -195,6 +201,8 @@ class Lowering:
     def _add_scale(self, c: complex):
         if self.dm:
             c = abs(c) ** 2
+        if abs(abs(c) - 1) > 1e-12:
+            self.nonunitary.append(len(self.ops))
         self.ops.append(LOp("scale", (), np.complex128(c)))
 
     def _lower_gate(self, gate, qubits):
@@ -266,6 +274,14 @@ class Lowering:
 
         d = None if m is None else to_diag(m[None])[0]
         # pure CZPow-like phase on two qubits: diag(1, 1, 1, phi)
+        if self.dm and self.split_dm_diag:
+            rbits = tuple(self.row_bit(q) for q in reversed(allq))
+            cbits = tuple(self.col_bit(q) for q in reversed(allq))
+            self.ops.append(LOp("diag", rbits, d, (), _sub_ref(ref, to_diag)))
+            self.ops.append(
+                LOp("diag", cbits, None if d is None else np.conj(d), (), _sub_ref(ref, lambda b: np.conj(to_diag(b))))
+            )
+            return
         if self.dm:
             rbits = tuple(self.row_bit(q) for q in reversed(allq))
             cbits = tuple(self.col_bit(q) for q in reversed(allq))
@@ -292,6 +308,7 @@ class Lowering:
         sop = sum(np.kron(K, np.conj(K)) for K in ks)
         cols = tuple(self.col_bit(q) for q in reversed(qubits))
         rows = tuple(self.row_bit(q) for q in reversed(qubits))
+        self.nonunitary.append(len(self.ops))
         self.ops.append(LOp("dense", cols + rows, sop))
 
 
@@ -598,7 +615,9 @@ def schedule(ops: list[LOp], n_bits: int, L: int, coalesce: int) -> list[Pass]:
     of 2^coalesce amplitudes) and walks the remaining ops in order: a dense op joins the pass if
     its targets fit into the L local bits, diagonal ops always join; an op that cannot join blocks
     its bits for the rest of the pass so that per-bit order is preserved (ops on disjoint bits
-    commute).
+    commute).  A fence op (one that does not commute with earlier ops on other bits, such as a
+    channel adjoint applied to one of the two states of an adjoint sweep) joins only when no
+    earlier op was deferred, and once deferred it holds back every later op.
     """
     L = min(L, n_bits)
     c = min(coalesce, L)
@@ -609,20 +628,26 @@ def schedule(ops: list[LOp], n_bits: int, L: int, coalesce: int) -> list[Pass]:
         S = set(forced)
         blocked: set = set()
         take, defer = [], []
+        fenced = False
         for op in rest:
             bits = op.bits
+            if fenced or (op.fence and defer):
+                defer.append(op)
+                fenced = True
+                continue
             if bits & blocked:
                 defer.append(op)
                 blocked |= bits
                 continue
             if op.kind in ("dense", "grad_dense"):
-                need = set(op.targets) - S
+                need = (set(op.targets) | set(op.need)) - S
                 if len(S) + len(need) <= L:
                     S |= need
                     take.append(op)
                 else:
                     defer.append(op)
                     blocked |= bits
+                    fenced = op.fence
             else:
                 take.append(op)
         return S, take, defer
@@ -1071,15 +1096,21 @@ def plan_registers(p: Pass, coalesce: int) -> Pass:
     return Pass(L, tuple(start), out)
 
 
-def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
-                blocks: bool = True, kmax: int = 5) -> PackedProgram:
+def plan_passes(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
+                blocks: bool = True, kmax: int = 5) -> list:
+    """The tile passes of a forward program (before packing)."""
     if fuse:
         ops = fuse_single_bit(ops)
     passes = schedule(ops, n_bits, L, coalesce)
     if blocks:
         passes = [Pass(p.L, p.lpos, fuse_blocks(p.ops, set(p.lpos), p.L, kmax)) for p in passes]
-    passes = [plan_registers(order_tile_bits(p, coalesce), coalesce) for p in passes]
-    return PackedProgram(passes, n_bits)
+    return [plan_registers(order_tile_bits(p, coalesce), coalesce) for p in passes]
+
+
+def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
+                blocks: bool = True, kmax: int = 5) -> PackedProgram:
+    return PackedProgram(plan_passes(ops, n_bits, L=L, coalesce=coalesce, fuse=fuse, blocks=blocks, kmax=kmax),
+                         n_bits)
 
 
 # --------------------------------------------------------------------------------------------
@@ -1158,6 +1189,46 @@ class ParamEvaluator:
         if c is None:
             return None
         return self._eval(c, symvals)
+
+
+class SlotChain:
+    """Chain rule from gradient slots (one per differentiated parameter occurrence) to symbols:
+    grad[b, i] = sum over slots s of g[b, s] * d expr_s / d symbol_i at point b.  Derivatives
+    that are constants (expressions affine in the symbol, the usual theta / pi) form a sparse
+    matrix built once; the others are evaluated per call."""
+
+    def __init__(self, slot_exprs: Sequence, symbols: Sequence, ev: ParamEvaluator):
+        index = {sym: i for i, sym in enumerate(symbols)}
+        self.n_symbols = len(symbols)
+        self.ev = ev
+        rows, cols, vals = [], [], []
+        self.dynamic: list = []  # (slot, symbol index, expr, symbol)
+        for slot, e in enumerate(slot_exprs):
+            for sym in sorted(getattr(e, "free_symbols", ()), key=lambda x: x.name):
+                if sym not in index:
+                    continue
+                d = sympy.diff(e, sym)
+                if not d.free_symbols:
+                    rows.append(slot)
+                    cols.append(index[sym])
+                    vals.append(float(d))
+                else:
+                    self.dynamic.append((slot, index[sym], e, sym))
+        self.rows = np.array(rows, dtype=np.int64)
+        self.cols = np.array(cols, dtype=np.int64)
+        self.vals = np.array(vals, dtype=np.float64)
+
+    def __call__(self, gs: np.ndarray, symvals: np.ndarray) -> np.ndarray:
+        """gs: (B, n_slots); symvals: (n_symbols, B).  Returns (B, n_symbols)."""
+        gs = np.asarray(gs, dtype=np.float64)
+        out = np.zeros((gs.shape[0], self.n_symbols))
+        if self.rows.size:
+            np.add.at(out.T, self.cols, (gs[:, self.rows] * self.vals).T)
+        for slot, i, e, sym in self.dynamic:
+            d = self.ev.dvalues(e, sym, symvals)
+            if d is not None:
+                out[:, i] += gs[:, slot] * d
+        return out
 
 
 def _batch_group(gate):

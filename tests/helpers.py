@@ -294,7 +294,7 @@ def run_batch_model(h, mats_arena):
                 j = np.arange(psi.size, dtype=np.int64)
                 src = j ^ x
                 sign = np.where(np.array([bin(v).count("1") & 1 for v in (src & z)]), -1.0, 1.0)
-                phi += h.coef[t] * sign * psi[src]
+                phi += h.apply_coef[t] * sign * psi[src]
             if h.gradients:
                 st2 = int(ti["state2"])
                 arena[st2 : st2 + (1 << n)] = phi.astype(np.complex64)
@@ -328,3 +328,63 @@ def run_batch_model(h, mats_arena):
         raw = before_adjoint()
     slots = np.concatenate([grads[i] for i in range(len(h.points))]) if h.gradients else None
     return raw, slots
+
+
+def vec_observable(row_terms, n_bits: int) -> np.ndarray:
+    """vec(H) as a 2n-bit density-matrix vector: the Pauli sum (masks on the row bits) applied to
+    vec(I), with the same phase convention as the expectation kernels."""
+    n = n_bits // 2
+    eye = np.zeros(1 << n_bits, dtype=np.complex128)
+    eye[np.arange(1 << n) * ((1 << n) + 1)] = 1
+    j = np.arange(1 << n_bits, dtype=np.int64)
+    out = np.zeros_like(eye)
+    for x, z, c in row_terms:
+        src = j ^ x
+        sign = np.where(np.array([bin(v).count("1") & 1 for v in (src & z)]), -1.0, 1.0)
+        out += c * sign * eye[src]
+    return out
+
+
+def run_dm_adjoint_model(prog, values):
+    """DMAdjointGradient.value_and_grad step by step on the pass models: the forward pass with
+    checkpoints, vec(H), then every segment recomputed from its checkpoint and walked backwards."""
+    from types import SimpleNamespace
+
+    symvals = prog.symbol_values(values)
+    mf, mr = prog.mats(symvals)
+    nb = prog.n_bits
+
+    def sub(packed, a, b):
+        return SimpleNamespace(n_bits=nb, pass_arr=packed.pass_arr[a:b], ops_arr=packed.ops_arr,
+                               aux_arr=packed.aux_arr, mats_arr=None)
+
+    def step(j, st):
+        a, b = prog.fwd_range[j]
+        return run_pass_model(sub(prog.fwd, a, b), st, mf) if b > a else st
+
+    rho0 = np.zeros(1 << nb, dtype=np.complex128)
+    rho0[0] = 1
+    segs = prog.segments()
+    ckpt = []
+    rho = rho0.copy()
+    for g, (a, b) in enumerate(segs):
+        if g > 0:
+            ckpt.append(rho.copy())
+        for j in range(a, b):
+            rho = step(j, rho)
+    lam = vec_observable(prog.row_terms, nb)  # vec(H) / phi_scale
+    value = float(np.vdot(lam, rho).real) * prog.phi_scale
+    grads = np.zeros(prog.n_slots)
+    for g in range(len(segs) - 1, -1, -1):
+        a, b = segs[g]
+        rho = rho0.copy() if g == 0 else ckpt[g - 1].copy()
+        stretch = []
+        for j in range(a, b):
+            rho = step(j, rho)
+            stretch.append(rho)
+        for j in range(b - 1, a - 1, -1):
+            ra, rb = prog.rev_range[j]
+            if rb > ra:
+                _, lam, gj = run_adjoint_model(sub(prog.rev, ra, rb), stretch[j - a], lam, mr, prog.n_slots)
+                grads += gj
+    return value, prog.chain(grads, symvals), rho

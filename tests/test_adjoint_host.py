@@ -163,3 +163,26 @@ def test_adjoint_rejects_measurements_and_foreign_observables():
         AdjointProgram(cirq.Circuit(cirq.rx(t).on(q[0]), cirq.measure(q[0])), cirq.Z(q[0]))
     with pytest.raises(KeyError):
         AdjointProgram(cirq.Circuit(cirq.rx(t).on(q[0])), cirq.Z(q[1]))
+
+
+@pytest.mark.parametrize("tiles", [(3, 2), (3, 3), (4, 2)])
+def test_controlled_gradient_op_keeps_its_inverses_in_its_pass(tiles):
+    """A parameterised controlled gate whose control is outside the adjoint tile: its GRAD op needs
+    the control local, so its two one-state inverses must wait for the same pass; otherwise the
+    gradient of an earlier gate on other bits is taken while only psi has been un-computed."""
+    b, z, c = sympy.symbols("b z c")
+    q = cirq.LineQubit.range(4)
+    circuit = cirq.Circuit(
+        [cirq.H(x) for x in q],
+        cirq.ControlledGate(cirq.ry(b)).on(q[0], q[2]),
+        cirq.ry(b).on(q[1]),
+        cirq.ZZPowGate(exponent=z).on(q[1], q[2]),
+        cirq.ControlledGate(cirq.rx(c)).on(q[0], q[3]),  # control on the highest bit, target on bit 0
+    )
+    psum = cirq.X(q[0]) * cirq.Y(q[2]) + 0.5 * cirq.Z(q[1]) * cirq.X(q[3]) + cirq.Y(q[0])
+    prog = AdjointProgram(circuit, psum, tile_bits=tiles[0], adjoint_tile_bits=tiles[1])
+    vals = np.array([0.9, -0.4, 1.3])
+    value, grad, _ = _model_value_and_grad(prog, psum, vals)
+    ref = _oracle_grad(circuit, list(q), psum, prog.symbols, vals)
+    assert np.max(np.abs(grad - ref)) < GRAD_ATOL, (grad, ref)
+    assert np.max(np.abs(ref)) > 0.05

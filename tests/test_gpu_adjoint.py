@@ -102,3 +102,26 @@ def test_adjoint_rejects_non_unitary_scale(gpu):
     circuit = cirq.Circuit(cirq.rx(t).on(q[0]), cirq.PauliString(qubit_pauli_map={q[1]: cirq.X}, coefficient=2.0))
     with pytest.raises(ValueError):
         AdjointGradient(circuit, cirq.Z(q[0]))
+
+
+def test_adjoint_large_coefficients_through_mfma_blocks(gpu):
+    """phi = H psi with sum |c| = 2000 on 10 qubits, reversed through MFMA blocks: the sweep starts
+    from (H / s) psi (s a power of two >= sum |c|) so phi stays in the split-FP16 range."""
+    _log("adjoint with large observable coefficients")
+    import cirq_amd as cirq_local
+
+    g1 = generators.grid_circuit(2, 5, 12, seed=1)
+    qs = sorted(g1.all_qubits())
+    syms = sympy.symbols(f"a0:{len(qs)}")
+    circuit = cirq_local.Circuit(list(g1.all_operations()) + [cirq_local.rx(syms[i]).on(q) for i, q in enumerate(qs)]
+                                 + list(generators.grid_circuit(2, 5, 12, seed=2).all_operations()))
+    psum = 200.0 * generators.sum_z(qs)
+    ag = AdjointGradient(circuit, psum)
+    assert ag.prog.phi_scale == 2048.0
+    assert int(((ag.prog.rev.ops_arr["kind"] & 0xFF) == 5).sum()) > 0  # MFMA blocks in the sweep
+    vals = np.linspace(-1.0, 1.2, len(qs))
+    value, grad = ag.value_and_grad(vals)
+    v2, g2 = ag.parameter_shift(vals)
+    assert np.all(np.isfinite(grad))
+    assert abs(value - v2) < 1e-5 * 2000
+    np.testing.assert_allclose(grad, g2, atol=1e-5 * 2000)
