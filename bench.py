@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--tile-bits", type=int, default=None)
     ap.add_argument("--coalesce-bits", type=int, default=None)
     ap.add_argument("--no-blocks", action="store_true", help="disable MFMA block fusion")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="compile the main circuit with the default tile / fusion settings instead of timing candidates")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-qubits", type=int, default=26)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
@@ -347,10 +349,32 @@ def main():
     qmap = {q: i for i, q in enumerate(qubits)}
     terms = [pauli_masks(t, qmap, n) for z in zz for t in cirq.PauliSum.wrap(z)]
 
+    tuner = None
+    if not args.no_autotune and args.tile_bits is None and args.coalesce_bits is None and not args.no_blocks:
+        from cirq_amd.autotune import Autotuner
+
+        tuner = Autotuner(os.environ.get("CIRQ_AMD_TUNE_CACHE"), reps=1)
     sim = cirq.Simulator(
-        seed=1234, tile_bits=args.tile_bits, coalesce_bits=args.coalesce_bits, mfma_blocks=not args.no_blocks
+        seed=1234, tile_bits=args.tile_bits, coalesce_bits=args.coalesce_bits, mfma_blocks=not args.no_blocks,
+        autotune=tuner or False,
     )
+    t0 = time.perf_counter()
     comp = sim._compile(circuit, qubits)
+    t_compile = time.perf_counter() - t0
+    tune_info = None
+    if tuner is not None:
+        import dataclasses
+
+        from cirq_amd.autotune import default_config
+
+        tune_info = {"config": dataclasses.asdict(comp.tuning), "compile_s": t_compile}
+        if tuner.log:  # timed here (not read from the cache file)
+            (log,) = tuner.log.values()
+            times = dict(log)
+            tune_info.update(candidates=len(log), default_ms=times[default_config(n)], tuned_ms=times[comp.tuning])
+        else:
+            tune_info["from_cache"] = os.environ.get("CIRQ_AMD_TUNE_CACHE")
+        torch.cuda.empty_cache()  # the tuner's scratch state
     factors = product_state_factors(circuit, qubits)
     assert len(factors) == 1 and len(factors[0]) == n
     axes = [qmap[q] for q in factors[0]]
@@ -436,6 +460,7 @@ def main():
             "gate_applications_per_s": n_ops * args.steps / wall,
             "hbm_gbs_whole_step": n_pass * bytes_per_launch / (ms_per_step / 1e3) / 1e9,
             "sample_check": int(bits.shape[0]),
+            "autotune": tune_info,
         },
     }
     if not args.no_adjoint:

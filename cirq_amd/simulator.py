@@ -133,7 +133,8 @@ def product_state_factors(circuit, qubits) -> list[list]:
 class _Compiled:
     """A circuit lowered, scheduled and uploaded for one qubit order."""
 
-    def __init__(self, circuit, qubits, dm: bool, device, L: int, coalesce: int, fuse: bool, blocks: bool = True):
+    def __init__(self, circuit, qubits, dm: bool, device, L: int, coalesce: int, fuse: bool, blocks: bool = True,
+                 tuner=None):
         self.qubits = tuple(qubits)
         self.n = len(self.qubits)
         self.dm = dm
@@ -142,7 +143,13 @@ class _Compiled:
         low.lower_circuit(circuit)
         self.lowering = low
         self.measurements: list[Measurement] = low.measurements
-        self.packed = compile_ops(low.ops, self.n_bits, L=L, coalesce=coalesce, fuse=fuse, blocks=blocks)
+        if tuner is not None and low.ops:
+            self.tuning = tuner.config_for(low.ops, self.n_bits)
+            kw = self.tuning.compile_kwargs(self.n_bits)
+        else:
+            self.tuning = None
+            kw = dict(L=L, coalesce=coalesce, blocks=blocks)
+        self.packed = compile_ops(low.ops, self.n_bits, fuse=fuse, **kw)
         self.evaluator = ParamEvaluator(low.param_refs)
         self.has_params = bool(self.packed.param_entries)
         self.dev = engine.DeviceProgram(self.packed, device)
@@ -175,7 +182,12 @@ class _SimulatorBase:
         coalesce_bits: int | None = None,
         fuse: bool = True,
         mfma_blocks: bool = True,
+        autotune=False,
     ):
+        """Cirq's arguments (dtype, noise, seed, split_untangled_states) plus engine settings:
+        tile_bits / coalesce_bits / mfma_blocks fix the pass compiler's tile and fusion choices;
+        autotune=True (or an autotune.Autotuner) instead times candidate settings on the GPU the
+        first time each circuit structure is compiled and keeps the fastest."""
         if np.dtype(dtype).kind != "c":
             raise ValueError(f"dtype must be a complex type but was {dtype}")
         if np.dtype(dtype) != np.complex64:
@@ -188,6 +200,11 @@ class _SimulatorBase:
         self._coalesce = coalesce_bits
         self._fuse = fuse
         self._blocks = mfma_blocks
+        if autotune is True:
+            from .autotune import default_autotuner
+
+            autotune = default_autotuner()
+        self._tuner = autotune or None
         self._cache: "collections.OrderedDict" = collections.OrderedDict()
         self._device = None
         self._sampler = None
@@ -223,7 +240,7 @@ class _SimulatorBase:
             return hit[1]
         nb = 2 * len(qubits) if self._dm else len(qubits)
         L, c = self._tile_params(nb)
-        comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse, self._blocks)
+        comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse, self._blocks, self._tuner)
         try:
             ref = weakref.ref(circuit)
         except TypeError:

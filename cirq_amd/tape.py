@@ -409,10 +409,11 @@ COSTS = {"dense1": 1.0, "dense2": 1.8, "dense3": 1.75, "diag": 0.05, "pair": 0.0
          "mfma5": 2.5, "reg1": 0.45, "reg2": 0.9, "reg3": 1.8, "permute": 0.8}
 
 
-def _valu_cost(op: LOp) -> float:
+def _valu_cost(op: LOp, costs: dict | None = None) -> float:
+    costs = COSTS if costs is None else costs
     if op.kind == "dense":
-        return COSTS[f"dense{len(op.targets)}"] if len(op.targets) <= 3 else 1e9
-    return COSTS.get(op.kind, 0.3)
+        return costs[f"dense{len(op.targets)}"] if len(op.targets) <= 3 else 1e9
+    return costs.get(op.kind, 0.3)
 
 
 MAX_VARIANT_BITS = 3  # outer bits a fused op may depend on (2^3 matrices per op)
@@ -468,7 +469,8 @@ def _rebase(blk: _Block, bits: list, obits: list) -> np.ndarray:
     return out
 
 
-def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = True) -> list[LOp]:
+def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = True, *, costs: dict | None = None,
+                variant_bits: int | None = None) -> list[LOp]:
     """Fuses a pass's fixed ops on local bits into blocks of at most kmax local bits.
 
     Several blocks stay open at once (one per disjoint bit group), so a block keeps absorbing
@@ -477,7 +479,10 @@ def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = 
     absorbed too: the block then holds one matrix per value of up to MAX_VARIANT_BITS outer bits,
     and each tile applies the one its outer bits select.  A closed block runs as: its original
     ops, one fused dense op (k <= 3, VALU) or an MFMA block (k = 4, 5; smaller groups padded),
-    whichever the cost table rates cheapest."""
+    whichever the cost table rates cheapest (`costs`, default COSTS; `variant_bits` caps the outer
+    bits, default MAX_VARIANT_BITS)."""
+    costs = COSTS if costs is None else costs
+    vmax = MAX_VARIANT_BITS if variant_bits is None else min(int(variant_bits), MAX_VARIANT_BITS)
     out: list[LOp] = []
     open_blocks: list[_Block] = []
     owner: dict = {}
@@ -492,9 +497,9 @@ def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = 
             out.append(members[0])
             return
         nb = len(blk.bits)
-        separate = sum(_valu_cost(m) for m in members)
-        fused = COSTS[f"dense{nb}"] if nb <= 3 else 1e9
-        mf = (COSTS["mfma4"] if nb <= 4 else COSTS["mfma5"]) if use_mfma else 1e9
+        separate = sum(_valu_cost(m, costs) for m in members)
+        fused = costs[f"dense{nb}"] if nb <= 3 else 1e9
+        mf = (costs["mfma4"] if nb <= 4 else costs["mfma5"]) if use_mfma else 1e9
         best = min(separate, fused, mf)
         vb = tuple(blk.obits)
         single = blk.mats[0] if not vb else blk.mats
@@ -516,7 +521,7 @@ def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = 
         bits = list(dict.fromkeys(list(op.targets) + [b for b, _ in op.controls]))
         lb = [b for b in bits if b in local]
         ob = [b for b in bits if b not in local]
-        absorbable = fixed and lb and len(lb) <= kmax and len(ob) <= MAX_VARIANT_BITS
+        absorbable = fixed and lb and len(lb) <= kmax and len(ob) <= vmax
         if absorbable and op.kind == "dense" and any(b not in local for b in op.targets):
             absorbable = False
         if not absorbable:
@@ -536,7 +541,7 @@ def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = 
             uobits += [b for b in blk.obits if b not in uobits]
         ubits += [b for b in lb if b not in ubits]
         uobits += [b for b in ob if b not in uobits]
-        if len(ubits) > kmax or len(uobits) > MAX_VARIANT_BITS:
+        if len(ubits) > kmax or len(uobits) > vmax:
             for blk in touched:
                 close(blk)
             touched = []
@@ -954,7 +959,7 @@ def _op_dependencies(ops: list[LOp]) -> list[list[int]]:
     return succ
 
 
-def plan_registers(p: Pass, coalesce: int) -> Pass:
+def plan_registers(p: Pass, coalesce: int, costs: dict | None = None) -> Pass:
     """Orders a pass's ops and inserts PERMUTE ops so that dense ops on 1-3 bits run in registers.
 
     The pass kernel keeps each thread's 2^NR tile elements in registers (NR = the top tile bits);
@@ -965,6 +970,7 @@ def plan_registers(p: Pass, coalesce: int) -> Pass:
     (PERMUTE) so that a new set of register bits lets more ops run in registers, or runs the
     ready op through LDS, whichever the cost table says is cheaper.  The coalescing bits stay the
     lowest tile bits throughout, so loads and stores remain contiguous."""
+    costs = COSTS if costs is None else costs
     L = p.L
     NT = threads_for_tile(L)
     TB = NT.bit_length() - 1
@@ -988,8 +994,8 @@ def plan_registers(p: Pass, coalesce: int) -> Pass:
 
     def lds_cost(op) -> float:
         if op.kind == "dense":
-            return COSTS[f"dense{len(op.targets)}"] if len(op.targets) <= 3 else COSTS["mfma5"]
-        return COSTS["mfma5"]
+            return costs[f"dense{len(op.targets)}"] if len(op.targets) <= 3 else costs["mfma5"]
+        return costs["mfma5"]
 
     def drain(regs, npred, ready, emit=None) -> float:
         """Runs every elementwise / register op reachable; returns the LDS cost saved."""
@@ -1002,7 +1008,7 @@ def plan_registers(p: Pass, coalesce: int) -> Pass:
                 if op.kind in _ELEMENTWISE or reg_ok(op, regs):
                     ready.discard(j)
                     if op.kind == "dense":
-                        gain += lds_cost(op) - COSTS[f"reg{len(op.targets)}"]
+                        gain += lds_cost(op) - costs[f"reg{len(op.targets)}"]
                     if emit is not None:
                         emit.append(op)
                     for s2 in succ[j]:
@@ -1079,7 +1085,7 @@ def plan_registers(p: Pass, coalesce: int) -> Pass:
             j = forced[0]
         else:
             S, gain = best_regs(npred, ready, done)
-            if gain > COSTS["permute"]:
+            if gain > costs["permute"]:
                 layout = layout_for(layout, S)
                 out.append(LOp("permute", tuple(layout), None))
                 continue
@@ -1097,20 +1103,23 @@ def plan_registers(p: Pass, coalesce: int) -> Pass:
 
 
 def plan_passes(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
-                blocks: bool = True, kmax: int = 5) -> list:
+                blocks: bool = True, kmax: int = 5, costs: dict | None = None,
+                variant_bits: int | None = None) -> list:
     """The tile passes of a forward program (before packing)."""
     if fuse:
         ops = fuse_single_bit(ops)
     passes = schedule(ops, n_bits, L, coalesce)
     if blocks:
-        passes = [Pass(p.L, p.lpos, fuse_blocks(p.ops, set(p.lpos), p.L, kmax)) for p in passes]
-    return [plan_registers(order_tile_bits(p, coalesce), coalesce) for p in passes]
+        passes = [Pass(p.L, p.lpos, fuse_blocks(p.ops, set(p.lpos), p.L, kmax, costs=costs, variant_bits=variant_bits))
+                  for p in passes]
+    return [plan_registers(order_tile_bits(p, coalesce), coalesce, costs) for p in passes]
 
 
 def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
-                blocks: bool = True, kmax: int = 5) -> PackedProgram:
-    return PackedProgram(plan_passes(ops, n_bits, L=L, coalesce=coalesce, fuse=fuse, blocks=blocks, kmax=kmax),
-                         n_bits)
+                blocks: bool = True, kmax: int = 5, costs: dict | None = None,
+                variant_bits: int | None = None) -> PackedProgram:
+    return PackedProgram(plan_passes(ops, n_bits, L=L, coalesce=coalesce, fuse=fuse, blocks=blocks, kmax=kmax,
+                                     costs=costs, variant_bits=variant_bits), n_bits)
 
 
 # --------------------------------------------------------------------------------------------
