@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-dm", action="store_true", help="skip the noisy density-matrix leg")
     ap.add_argument("--dm-qubits", type=int, default=16)
     ap.add_argument("--dm-layers", type=int, default=20)
+    ap.add_argument("--no-q32", action="store_true", help="skip the 32-qubit depth-20 grid leg")
+    ap.add_argument("--q32-steps", type=int, default=2)
     ap.add_argument("--no-dm-adjoint", action="store_true", help="skip the noisy density-matrix gradient leg")
     ap.add_argument("--dm-adjoint-qubits", type=int, default=12)
     ap.add_argument("--dm-adjoint-layers", type=int, default=10)
@@ -314,35 +316,19 @@ def cpu_baseline(args, circuit_ops_fn, n_gpu: int, layers: int, ops_per_layer: f
     }
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        if int(os.environ.get("RANK", "0")) == 0:
-            print(
-                "bench.py runs on one GPU only: this simulator has no multi-GPU path "
-                "(sharding is described in DESIGN.md)", file=sys.stderr
-            )
-        sys.exit(2)
-
+def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int, warmup: int, autotune: bool):
+    """One step of the random-grid workload: |0..0>, the circuit's tile passes, `reps` samples of
+    every qubit in Cirq's factor order and <Z_i Z_j> over nearest-neighbour pairs.  Times `steps`
+    steps after `warmup`, the passes with HIP events on the stream they run on."""
     import torch
-
-    if args.copy_calibration:
-        n = args.copy_calibration * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n, "copies": 5}))
-        return
 
     import cirq_amd as cirq
     from cirq_amd import engine, generators
     from cirq_amd.paulis import pauli_masks
     from cirq_amd.simulator import product_state_factors
 
-    n = args.rows * args.cols
-    circuit = generators.grid_circuit(args.rows, args.cols, args.depth, seed=0)
+    n = rows * cols
+    circuit = generators.grid_circuit(rows, cols, depth, seed=0)
     qubits = sorted(circuit.all_qubits())
     n_ops = sum(1 for _ in circuit.all_operations())
     zz = generators.nearest_neighbour_zz(qubits)
@@ -350,7 +336,7 @@ def main():
     terms = [pauli_masks(t, qmap, n) for z in zz for t in cirq.PauliSum.wrap(z)]
 
     tuner = None
-    if not args.no_autotune and args.tile_bits is None and args.coalesce_bits is None and not args.no_blocks:
+    if autotune and args.tile_bits is None and args.coalesce_bits is None and not args.no_blocks:
         from cirq_amd.autotune import Autotuner
 
         tuner = Autotuner(os.environ.get("CIRQ_AMD_TUNE_CACHE"), reps=1)
@@ -392,17 +378,17 @@ def main():
         e1.record(stream)
         if record:
             pass_events.append((e0, e1))
-        u = sim._prng.random_sample(args.reps)
+        u = sim._prng.random_sample(reps)
         idx = sim._sampler.draw(st, n, False, u, axes=axes)
         bits = sim._sampler.bits(idx, n)
         vals = engine.expect_terms(st, n, False, terms, sim._ws).cpu().numpy()
         return bits, vals
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         bits, vals = step(True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -412,8 +398,65 @@ def main():
     avg_launch_s = pass_ms / 1e3 / n_pass
     bytes_per_launch = 2 * 8 * (2**n)
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    ms_per_step = wall / args.steps * 1e3
-    layers_per_s = args.depth * args.steps / wall
+    ms_per_step = wall / steps * 1e3
+    layers_per_s = depth * steps / wall
+    norm = engine.norm2(st, n, sim._ws)  # FP64 on the GPU: a size-independent check of the passes
+    return dict(n=n, n_ops=n_ops, zz=zz, comp=comp, bits=bits, vals=vals, wall=wall, pass_ms=pass_ms, n_pass=n_pass,
+                avg_launch_s=avg_launch_s, bytes_per_launch=bytes_per_launch, achieved=achieved,
+                ms_per_step=ms_per_step, layers_per_s=layers_per_s, tune_info=tune_info, norm=norm)
+
+
+
+def q32_leg(args):
+    """The larger configuration's 32-qubit variant: the random grid circuit on a 4x8 grid, depth 20
+    (32 GiB complex64 state), with the same step as the main leg (10^6 samples, 52 <ZZ>)."""
+    g = grid_workload(args, 4, 8, 20, args.reps, args.q32_steps, 1, autotune=not args.no_autotune)
+    return {
+        "workload": f"random grid circuit 4x8 (32 qubits), depth 20, {g['n_ops']} gates, {args.reps} samples, "
+        f"{len(g['zz'])} <ZZ>",
+        "layers_per_s": g["layers_per_s"],
+        "ms_per_step": g["ms_per_step"],
+        "steps": args.q32_steps,
+        "passes": g["n_pass"],
+        "passes_ms_per_step": g["pass_ms"],
+        "pass_kernel_avg_ms": g["avg_launch_s"] * 1e3,
+        "pass_hbm_gbs": g["achieved"],
+        "pass_roofline_frac": g["achieved"] / HBM_PEAK_GBS,
+        "norm_error": abs(g["norm"] - 1.0),
+        "autotune": g["tune_info"],
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        if int(os.environ.get("RANK", "0")) == 0:
+            print(
+                "bench.py runs on one GPU only: this simulator has no multi-GPU path "
+                "(sharding is described in DESIGN.md)", file=sys.stderr
+            )
+        sys.exit(2)
+
+    import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n, "copies": 5}))
+        return
+
+    from cirq_amd import generators
+
+    g = grid_workload(args, args.rows, args.cols, args.depth, args.reps, args.steps, args.warmup,
+                      autotune=not args.no_autotune)
+    n, n_ops, zz, comp, bits = g["n"], g["n_ops"], g["zz"], g["comp"], g["bits"]
+    wall, pass_ms, n_pass = g["wall"], g["pass_ms"], g["n_pass"]
+    avg_launch_s, bytes_per_launch, achieved = g["avg_launch_s"], g["bytes_per_launch"], g["achieved"]
+    ms_per_step, layers_per_s, tune_info = g["ms_per_step"], g["layers_per_s"], g["tune_info"]
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_k_pass_{n}q.json")
@@ -460,9 +503,12 @@ def main():
             "gate_applications_per_s": n_ops * args.steps / wall,
             "hbm_gbs_whole_step": n_pass * bytes_per_launch / (ms_per_step / 1e3) / 1e9,
             "sample_check": int(bits.shape[0]),
+            "norm_error": abs(g["norm"] - 1.0),
             "autotune": tune_info,
         },
     }
+    if not args.no_q32:
+        out["extra"]["q32"] = q32_leg(args)
     if not args.no_adjoint:
         out["extra"]["adjoint"] = adjoint_leg(args)
     if not args.no_dm_adjoint:
