@@ -233,6 +233,17 @@ class Circuit:
                         return False
         return True
 
+    def are_any_measurements_terminal(self) -> bool:
+        """True iff some measurement is followed by no operation on its qubits
+        (cirq-core/cirq/circuits/circuit.py:889-894)."""
+        for i, m in enumerate(self._moments):
+            for op in m.operations:
+                if _is_measurement(op):
+                    later = self._moments[i + 1 :]
+                    if not any(mm.operates_on(op.qubits) for mm in later):
+                        return True
+        return False
+
     def __eq__(self, other):
         if not isinstance(other, Circuit):
             return NotImplemented

@@ -206,6 +206,7 @@ class _SimulatorBase:
             autotune = default_autotuner()
         self._tuner = autotune or None
         self._cache: "collections.OrderedDict" = collections.OrderedDict()
+        self._split_memo: dict = {}
         self._device = None
         self._sampler = None
         self._ws = None
@@ -344,6 +345,31 @@ class _SimulatorBase:
     def run_sweep(self, program, params, repetitions: int = 1) -> list[ResultDict]:
         return list(self.run_sweep_iter(program, params, repetitions))
 
+    def _split_cached(self, circuit):
+        """_split_prefix, reusing the result for a circuit object seen before (repeated run_sweep
+        calls on one program then also reuse its compiled prefix)."""
+        hit = self._split_memo.get(id(circuit))
+        if hit is not None and hit[0]() is circuit:
+            return hit[1], hit[2]
+        prefix, suffix = _split_prefix(circuit, self._can_be_in_run_prefix)
+        try:
+            ref = weakref.ref(circuit)
+        except TypeError:
+            return prefix, suffix
+        self._split_memo[id(circuit)] = (ref, prefix, suffix)
+        while len(self._split_memo) > 16:
+            self._split_memo.pop(next(iter(self._split_memo)))
+        return prefix, suffix
+
+    def sample_expectation_values(self, program, observables, *, num_samples: int, params=None,
+                                  permit_terminal_measurements: bool = False) -> list[list[float]]:
+        """Expectation values estimated from `num_samples` samples per observable group
+        (cirq-core/cirq/work/sampler.py:311-400); see cirq_amd.observables."""
+        from .observables import sample_expectation_values
+
+        return sample_expectation_values(self, program, observables, num_samples=num_samples, params=params,
+                                         permit_terminal_measurements=permit_terminal_measurements)
+
     def sample(self, program, *, repetitions: int = 1, params=None):
         import pandas as pd
 
@@ -377,7 +403,7 @@ class _SimulatorBase:
         resolvers = list(to_resolvers(params))
         circuit = self._noisy(program)
         qubits = tuple(sorted(circuit.all_qubits()))
-        prefix, suffix = _split_prefix(circuit, self._can_be_in_run_prefix)
+        prefix, suffix = self._split_cached(circuit)
         meas_ops = list(suffix.all_operations())
         if not all(_is_measurement_op(op) for op in meas_ops):
             yield from self._run_trajectories(prefix, suffix, qubits, resolvers, repetitions)
