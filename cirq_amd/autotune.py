@@ -52,7 +52,7 @@ def _axes(n_bits: int, base: TuneConfig) -> list[tuple[str, list]]:
     Ls = [L for L in (base.tile_bits, base.tile_bits - 1, base.tile_bits + 1) if 8 <= L <= min(13, n_bits)]
     return [
         ("tile_bits", Ls or [base.tile_bits]),
-        ("mfma_weight", [base.mfma_weight, 0.7, 1.4]),
+        ("mfma_weight", [base.mfma_weight, 0.8, 1.3, 1.7]),
         ("variant_bits", [base.variant_bits, 0, 2]),
         ("kmax", [base.kmax, 4]),
         ("coalesce_bits", [base.coalesce_bits, 3, 5]),

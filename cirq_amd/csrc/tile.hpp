@@ -143,11 +143,31 @@ __device__ __forceinline__ void tile_dense(float2* tile, int L, sop_t& op, MPtr 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
 
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// Split x * s into FP16 hi + lo (x * s = hi + lo + O(2^-22 |x s|)) for a complex value, packed as
+// (re, im) pairs.  Each half comes from one mixed-precision FMA that rounds to nearest straight
+// into the low or high half of a register: hi = f16(x * s), lo = f16(x * s - hi), where the FMA
+// computes x * s - hi exactly before its one rounding.
+__device__ __forceinline__ void split_f16(float xr, float xi, float s, half2_t& hi, half2_t& lo) {
+  // the low-half writes are plain outputs: the high-half write that follows fills the rest
+  asm("v_fma_mixlo_f16 %0, %1, %2, 0" : "=v"(hi) : "v"(xr), "v"(s));
+  asm("v_fma_mixhi_f16 %0, %1, %2, 0" : "+v"(hi) : "v"(xi), "v"(s));
+  asm("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel_hi:[0,0,1]" : "=v"(lo) : "v"(xr), "v"(s), "v"(hi));
+  asm("v_fma_mixhi_f16 %0, %1, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(lo) : "v"(xi), "v"(s), "v"(hi));
+}
+
 // Y = U X on tile bits op.t[0..K) with v_mfma_f32_32x32x16_f16.  The tile is viewed as a
 // (2^K rows) x (2^(L-K) columns) complex matrix; in real-interleaved form (row 2r = Re, 2r+1 = Im)
 // it is R (2^(K+1) square) times X.  Each wave owns whole 32-column blocks, so reads and writes of
 // different waves never overlap and the block needs no barrier inside.  X is split as
-// hi + lo FP16 after scaling by `scale`; U's split comes from the host in A-operand order.
+// hi + lo FP16 after scaling by `scale` (split_f16); U's split comes from the host in A-operand
+// order.
+//
+// Addressing: the tile index of (row r, column c) is c with zeros inserted at the targets, OR the
+// bits of r deposited at the targets.  pad() is additive over disjoint bit sets, so the padded
+// index is a per-lane column part + a per-lane part from the lane's half (h) + a part that depends
+// only on compile-time row bits and the op's targets (scalar).
 template <int K, int NT>
 __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, const half8_t* A,
                                                 float scale, float inv_scale) {
@@ -155,53 +175,78 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, 
   constexpr int R = 2 * D;
   constexpr int KS = R / 16;
   constexpr int RB = R / 32;
+  constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, c32 = lane & 31;
-  int tmask = 0;
+  int pbit[K];  // pad(1 << t[m])
+  int ts[K];    // targets in ascending order
 #pragma unroll
-  for (int m = 0; m < K; ++m) tmask |= 1 << op.t[m];
-  auto rowoff = [&](int r) {
-    int j = 0;
+  for (int m = 0; m < K; ++m) {
+    const int b = 1 << op.t[m];
+    pbit[m] = b + (b >> kPadShift);
+    ts[m] = op.t[m];
+  }
 #pragma unroll
-    for (int m = 0; m < K; ++m) j |= ((r >> m) & 1) << op.t[m];
-    return j;
+  for (int a = 1; a < K; ++a)
+#pragma unroll
+    for (int b = K - 1; b >= a; --b)
+      if (ts[b - 1] > ts[b]) {
+        const int x = ts[b - 1];
+        ts[b - 1] = ts[b];
+        ts[b] = x;
+      }
+  auto prow = [&](int r) {  // padded offset of row r (r is a compile-time constant at every use)
+    int o = 0;
+#pragma unroll
+    for (int m = 0; m < K; ++m)
+      if ((r >> m) & 1) o += pbit[m];
+    return o;
   };
   const int ncol = 1 << (L - K);
   const int ncb = (ncol + 31) >> 5;
-  for (int cb = wave; cb < ncb; cb += NT / 64) {
+  const int h_in = h ? pbit[2] : 0;   // B operand row r = 8 ks + 4 h + e: bit 2 is h
+  const int h_out = h ? pbit[1] : 0;  // output row 16 rb + 4 (a >> 2) + 2 h + ((a >> 1) & 1)
+  // A fragments stay in registers across column blocks when they are few (K = 4)
+  constexpr bool kHoistA = KS * RB * 2 <= 4;
+  half8_t ahold[kHoistA ? KS * RB * 2 : 1];
+  if constexpr (kHoistA) {
+#pragma unroll
+    for (int f = 0; f < KS * RB * 2; ++f) ahold[f] = A[f * 64 + lane];
+  }
+  for (int cb = wave; cb < ncb; cb += NW) {
     const int col = cb * 32 + c32;
     const bool valid = col < ncol;
-    int jc = 0;
-    for (int b = 0, i = 0; b < L; ++b) {
-      if (!((tmask >> b) & 1)) {
-        jc |= ((col >> i) & 1) << b;
-        ++i;
-      }
-    }
+    int jc = col;
+#pragma unroll
+    for (int m = 0; m < K; ++m) jc = insert0(jc, ts[m]);
+    const int pc = jc + (jc >> kPadShift);
+    const int b_in = pc + h_in, b_out = pc + h_out;
     floatx16_t acc[RB];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-      for (int a = 0; a < 16; ++a) acc[rb][a] = 0.f;
-#pragma unroll 1
     for (int ks = 0; ks < KS; ++ks) {
-      half8_t bhi, blo;
+      half2_t hh[4], ll[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = ks * 8 + 4 * h + e;
-        const float2 v = valid ? tile[pad(jc | rowoff(r))] : make_float2(0.f, 0.f);
-        const float xr = v.x * scale, xi = v.y * scale;
-        const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
-        bhi[2 * e] = hr;
-        bhi[2 * e + 1] = hi;
-        blo[2 * e] = (_Float16)(xr - (float)hr);
-        blo[2 * e + 1] = (_Float16)(xi - (float)hi);
+        const float2 v = valid ? tile[b_in + prow(ks * 8 + e)] : make_float2(0.f, 0.f);
+        split_f16(v.x, v.y, scale, hh[e], ll[e]);
       }
+      const half8_t bhi = __builtin_shufflevector(__builtin_shufflevector(hh[0], hh[1], 0, 1, 2, 3),
+                                                  __builtin_shufflevector(hh[2], hh[3], 0, 1, 2, 3), 0, 1, 2, 3, 4,
+                                                  5, 6, 7);
+      const half8_t blo = __builtin_shufflevector(__builtin_shufflevector(ll[0], ll[1], 0, 1, 2, 3),
+                                                  __builtin_shufflevector(ll[2], ll[3], 0, 1, 2, 3), 0, 1, 2, 3, 4,
+                                                  5, 6, 7);
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) {
-        // A operands stream from the L1/L2 caches (every workgroup reads the same matrix)
-        const half8_t ahi = A[(ks * RB + rb) * 64 + lane];
-        const half8_t alo = A[(KS * RB + ks * RB + rb) * 64 + lane];
-        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bhi, acc[rb], 0, 0, 0);
+        half8_t ahi, alo;
+        if constexpr (kHoistA) {
+          ahi = ahold[ks * RB + rb];
+          alo = ahold[KS * RB + ks * RB + rb];
+        } else {
+          // A operands stream from the L1/L2 caches (every workgroup reads the same matrix)
+          ahi = A[(ks * RB + rb) * 64 + lane];
+          alo = A[(KS * RB + ks * RB + rb) * 64 + lane];
+        }
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bhi, ks == 0 ? floatx16_t{} : acc[rb], 0, 0, 0);
         acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bhi, acc[rb], 0, 0, 0);
         acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, blo, acc[rb], 0, 0, 0);
       }
@@ -210,10 +255,9 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, 
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int a = 0; a < 16; a += 2) {
-          const int i = rb * 32 + (a & 3) + 8 * (a >> 2) + 4 * h;  // even: real part of row i/2
-          tile[pad(jc | rowoff(i >> 1))] = make_float2(acc[rb][a] * inv_scale, acc[rb][a + 1] * inv_scale);
-        }
+        for (int a = 0; a < 16; a += 2)  // a even: real part, a + 1: imaginary part of one row
+          tile[b_out + prow(rb * 16 + 4 * (a >> 2) + ((a >> 1) & 1))] =
+              make_float2(acc[rb][a] * inv_scale, acc[rb][a + 1] * inv_scale);
     }
   }
 }

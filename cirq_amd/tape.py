@@ -402,11 +402,15 @@ def _op_matrix(op: LOp) -> np.ndarray:
     raise ValueError(op.kind)
 
 
-# Relative costs (per amplitude, arbitrary units) used to choose how a fused group runs.
-# Measured on MI355X at 28 qubits, tile of 12 bits (ms per op over the state): LDS dense1 0.115,
-# dense2 0.21, dense3 0.20, MFMA block k=4 0.25, k=5 0.29, diagonal / pair phase ~0.003.
-COSTS = {"dense1": 1.0, "dense2": 1.8, "dense3": 1.75, "diag": 0.05, "pair": 0.03, "scale": 0.03, "mfma4": 2.2,
-         "mfma5": 2.5, "reg1": 0.45, "reg2": 0.9, "reg3": 1.8, "permute": 0.8}
+# Relative costs (arbitrary units) the planners use to choose how a fused group runs.  Marginal
+# cost per op measured on MI355X at 28 qubits, tile of 12 bits (ms): register dense1 0.100,
+# dense2 0.139, dense3 0.245; through LDS dense1 0.143, dense2 0.176, dense3 0.383; MFMA block
+# k=4 0.177, k=5 0.275; PERMUTE 0.058; diagonal 0.107; CZ pair phase 0.033.  The table is not
+# proportional to these: its MFMA weights were fit by timing whole-circuit plans (the 30-qubit
+# grid circuit was fastest with MFMA at 0.5-0.7 of the earlier 2.2 / 2.5), and the autotuner
+# scales them per circuit.
+COSTS = {"dense1": 1.0, "dense2": 1.8, "dense3": 1.75, "diag": 0.05, "pair": 0.03, "scale": 0.03, "mfma4": 1.3,
+         "mfma5": 1.5, "reg1": 0.45, "reg2": 0.9, "reg3": 1.8, "permute": 0.8}
 
 
 def _valu_cost(op: LOp, costs: dict | None = None) -> float:
