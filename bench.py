@@ -357,7 +357,7 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
         if tuner.log:  # timed here (not read from the cache file)
             (log,) = tuner.log.values()
             times = dict(log)
-            tune_info.update(candidates=len(log), default_ms=times[default_config(n)], tuned_ms=times[comp.tuning])
+            tune_info.update(candidates=len(log), default_ms=times[default_config(n, False)], tuned_ms=times[comp.tuning])
         else:
             tune_info["from_cache"] = os.environ.get("CIRQ_AMD_TUNE_CACHE")
         torch.cuda.empty_cache()  # the tuner's scratch state

@@ -3,7 +3,8 @@
 The pass compiler has a handful of knobs whose best values depend on the circuit and the device:
 the tile width L (more gates per pass vs. fewer workgroups in flight), the coalescing bits (how
 many low flat bits every tile keeps), the widest fused block and the number of outer bits a fused
-op may depend on, and how the planner weighs MFMA blocks against VALU gates.  `Autotuner`
+op may depend on, how the planner weighs MFMA blocks against VALU gates, and whether small blocks
+left at the end of a pass share MFMA blocks.  `Autotuner`
 compiles candidates for a lowered circuit, runs each on a scratch state with HIP events around a
 few repetitions, walks the knobs one at a time from the default (coordinate descent), and caches
 the fastest per circuit structure (optionally in a JSON file, so a later process reuses it).
@@ -21,7 +22,7 @@ import os
 import torch
 
 from . import engine
-from .tape import COSTS, MAX_VARIANT_BITS, LOp, compile_ops
+from .tape import COSTS, DEFAULT_KMAX, MAX_VARIANT_BITS, LOp, compile_ops
 
 
 @dataclasses.dataclass(frozen=True)
@@ -32,6 +33,7 @@ class TuneConfig:
     variant_bits: int = MAX_VARIANT_BITS  # outer bits a fused op may depend on
     mfma_weight: float = 1.0  # multiplies the MFMA entries of the planner's cost table
     blocks: bool = True
+    kron_merge: bool = True  # small blocks left at the end of a pass share MFMA blocks
 
     def compile_kwargs(self, n_bits: int) -> dict:
         L = min(self.tile_bits, n_bits)
@@ -39,12 +41,13 @@ class TuneConfig:
         costs["mfma4"] *= self.mfma_weight
         costs["mfma5"] *= self.mfma_weight
         return dict(L=L, coalesce=min(self.coalesce_bits, L), blocks=self.blocks, kmax=self.kmax, costs=costs,
-                    variant_bits=self.variant_bits)
+                    variant_bits=self.variant_bits, kron_merge=self.kron_merge)
 
 
-def default_config(n_bits: int) -> TuneConfig:
+def default_config(n_bits: int, dm: bool = False) -> TuneConfig:
+    """The settings the simulators use without tuning."""
     L = 12 if n_bits >= 20 else (10 if n_bits >= 14 else n_bits)
-    return TuneConfig(tile_bits=min(L, n_bits), coalesce_bits=min(4, L))
+    return TuneConfig(tile_bits=min(L, n_bits), coalesce_bits=min(4, L), kmax=DEFAULT_KMAX[bool(dm)])
 
 
 def _axes(n_bits: int, base: TuneConfig) -> list[tuple[str, list]]:
@@ -52,10 +55,11 @@ def _axes(n_bits: int, base: TuneConfig) -> list[tuple[str, list]]:
     Ls = [L for L in (base.tile_bits, base.tile_bits - 1, base.tile_bits + 1) if 8 <= L <= min(13, n_bits)]
     return [
         ("tile_bits", Ls or [base.tile_bits]),
-        ("mfma_weight", [base.mfma_weight, 0.8, 1.3, 1.7]),
-        ("variant_bits", [base.variant_bits, 0, 2]),
-        ("kmax", [base.kmax, 4]),
+        ("mfma_weight", [base.mfma_weight, 0.7, 0.85, 1.2]),
+        ("variant_bits", [base.variant_bits, 2]),
+        ("kmax", [base.kmax, 9 - base.kmax]),
         ("coalesce_bits", [base.coalesce_bits, 3, 5]),
+        ("kron_merge", [base.kron_merge, not base.kron_merge]),
         ("blocks", [base.blocks, not base.blocks]),
     ]
 
@@ -108,11 +112,11 @@ class Autotuner:
         e1.synchronize()
         return e0.elapsed_time(e1) / self.reps
 
-    def tune(self, ops: list[LOp], n_bits: int, base: TuneConfig | None = None) -> TuneConfig:
+    def tune(self, ops: list[LOp], n_bits: int, base: TuneConfig | None = None, dm: bool = False) -> TuneConfig:
         """Coordinate descent from `base` (default_config): each knob in turn takes the value
         with the lowest time, keeping a change only if it saves more than min_gain."""
         device = engine.require_gpu()
-        base = base or default_config(n_bits)
+        base = base or default_config(n_bits, dm)
         state = engine.new_state(n_bits, device)
         timings: dict[TuneConfig, float] = {}
 
@@ -136,10 +140,10 @@ class Autotuner:
         self._save()
         return best
 
-    def config_for(self, ops: list[LOp], n_bits: int) -> TuneConfig:
+    def config_for(self, ops: list[LOp], n_bits: int, dm: bool = False) -> TuneConfig:
         key = structure_key(ops, n_bits)
         hit = self.cache.get(key)
-        return hit if hit is not None else self.tune(ops, n_bits)
+        return hit if hit is not None else self.tune(ops, n_bits, dm=dm)
 
 
 _default: Autotuner | None = None
@@ -153,8 +157,8 @@ def default_autotuner() -> Autotuner:
     return _default
 
 
-def tuned_compile_kwargs(ops: list[LOp], n_bits: int, tuner: Autotuner | None = None) -> dict:
-    return (tuner or default_autotuner()).config_for(ops, n_bits).compile_kwargs(n_bits)
+def tuned_compile_kwargs(ops: list[LOp], n_bits: int, tuner: Autotuner | None = None, dm: bool = False) -> dict:
+    return (tuner or default_autotuner()).config_for(ops, n_bits, dm).compile_kwargs(n_bits)
 
 
 __all__ = ["Autotuner", "TuneConfig", "default_config", "default_autotuner", "structure_key", "tuned_compile_kwargs"]

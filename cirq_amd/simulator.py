@@ -30,6 +30,7 @@ from .paulis import PauliSum, pauli_masks
 from .qubits import QubitOrder
 from .study import ParamResolver, ResultDict, to_resolvers
 from .tape import (
+    DEFAULT_KMAX,
     Lowering,
     Measurement,
     ParamEvaluator,
@@ -144,11 +145,11 @@ class _Compiled:
         self.lowering = low
         self.measurements: list[Measurement] = low.measurements
         if tuner is not None and low.ops:
-            self.tuning = tuner.config_for(low.ops, self.n_bits)
+            self.tuning = tuner.config_for(low.ops, self.n_bits, dm)
             kw = self.tuning.compile_kwargs(self.n_bits)
         else:
             self.tuning = None
-            kw = dict(L=L, coalesce=coalesce, blocks=blocks)
+            kw = dict(L=L, coalesce=coalesce, blocks=blocks, kmax=DEFAULT_KMAX[bool(dm)])
         self.packed = compile_ops(low.ops, self.n_bits, fuse=fuse, **kw)
         self.evaluator = ParamEvaluator(low.param_refs)
         self.has_params = bool(self.packed.param_entries)

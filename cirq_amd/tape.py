@@ -406,11 +406,12 @@ def _op_matrix(op: LOp) -> np.ndarray:
 # cost per op measured on MI355X at 28 qubits, tile of 12 bits (ms): register dense1 0.100,
 # dense2 0.139, dense3 0.245; through LDS dense1 0.143, dense2 0.176, dense3 0.383; MFMA block
 # k=4 0.177, k=5 0.275; PERMUTE 0.058; diagonal 0.107; CZ pair phase 0.033.  The table is not
-# proportional to these: its MFMA weights were fit by timing whole-circuit plans (the 30-qubit
-# grid circuit was fastest with MFMA at 0.5-0.7 of the earlier 2.2 / 2.5), and the autotuner
-# scales them per circuit.
-COSTS = {"dense1": 1.0, "dense2": 1.8, "dense3": 1.75, "diag": 0.05, "pair": 0.03, "scale": 0.03, "mfma4": 1.3,
-         "mfma5": 1.5, "reg1": 0.45, "reg2": 0.9, "reg3": 1.8, "permute": 0.8}
+# proportional to these: its MFMA weights were fit by timing whole-circuit plans (with blocks left
+# at the end of a pass merged, the 30-qubit grid circuit and the 16-qubit noisy density matrix
+# were fastest around these values), and the autotuner scales them per circuit.
+DEFAULT_KMAX = {False: 4, True: 5}  # widest fused block: state vectors, density matrices (timed)
+COSTS = {"dense1": 1.0, "dense2": 1.8, "dense3": 1.75, "diag": 0.05, "pair": 0.03, "scale": 0.03, "mfma4": 2.0,
+         "mfma5": 2.55, "reg1": 0.45, "reg2": 0.9, "reg3": 1.8, "permute": 0.8}
 
 
 def _valu_cost(op: LOp, costs: dict | None = None) -> float:
@@ -474,7 +475,7 @@ def _rebase(blk: _Block, bits: list, obits: list) -> np.ndarray:
 
 
 def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = True, *, costs: dict | None = None,
-                variant_bits: int | None = None) -> list[LOp]:
+                variant_bits: int | None = None, kron_merge: bool = True) -> list[LOp]:
     """Fuses a pass's fixed ops on local bits into blocks of at most kmax local bits.
 
     Several blocks stay open at once (one per disjoint bit group), so a block keeps absorbing
@@ -484,7 +485,8 @@ def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = 
     and each tile applies the one its outer bits select.  A closed block runs as: its original
     ops, one fused dense op (k <= 3, VALU) or an MFMA block (k = 4, 5; smaller groups padded),
     whichever the cost table rates cheapest (`costs`, default COSTS; `variant_bits` caps the outer
-    bits, default MAX_VARIANT_BITS)."""
+    bits, default MAX_VARIANT_BITS).  With kron_merge, small blocks still open at the end of the
+    pass share MFMA blocks (Kronecker products of disjoint blocks) when that rates cheaper."""
     costs = COSTS if costs is None else costs
     vmax = MAX_VARIANT_BITS if variant_bits is None else min(int(variant_bits), MAX_VARIANT_BITS)
     out: list[LOp] = []
@@ -563,8 +565,50 @@ def fuse_blocks(ops: list[LOp], local: set, L: int, kmax: int = 5, mfma: bool = 
         open_blocks.append(nb)
         for b in ubits:
             owner[b] = nb
+    # End of the pass: every open block closes and none can grow any more.  Blocks that would run
+    # as VALU ops (a few one- or two-qubit gates each) are disjoint and nothing follows them in the
+    # pass, so several can share one MFMA block: the Kronecker product of their matrices.
+    small = []
     for blk in list(open_blocks):
-        close(blk)
+        nb = len(blk.bits)
+        own = min(sum(_valu_cost(m, costs) for m in blk.members), costs[f"dense{nb}"] if nb <= 3 else 1e9)
+        mf = (costs["mfma4"] if nb <= 4 else costs["mfma5"]) if use_mfma else 1e9
+        if use_mfma and kron_merge and nb <= 3 and own < mf:
+            small.append((own, blk))
+        else:
+            close(blk)
+    small.sort(key=lambda x: (-len(x[1].bits), -x[0]))
+    groups: list[list] = []
+    for own, blk in small:
+        for g in groups:
+            bits = sum(len(b.bits) for _, b in g) + len(blk.bits)
+            obits = set(ob for _, b in g for ob in b.obits) | set(blk.obits)
+            if bits <= kmax and len(obits) <= vmax:
+                g.append((own, blk))
+                break
+        else:
+            groups.append([(own, blk)])
+    for g in groups:
+        nb = sum(len(b.bits) for _, b in g)
+        mf = costs["mfma4"] if nb <= 4 else costs["mfma5"]
+        if len(g) < 2 or mf >= sum(own for own, _ in g):
+            for _, blk in g:
+                close(blk)
+            continue
+        bits = [x for _, b in g for x in b.bits]
+        obits = list(dict.fromkeys(ob for _, b in g for ob in b.obits))
+        mats = np.broadcast_to(np.eye(1 << nb, dtype=np.complex128), (1 << len(obits), 1 << nb, 1 << nb)).copy()
+        for _, blk in g:
+            mats = np.einsum("vij,vjk->vik", _rebase(blk, bits, obits), mats)
+            open_blocks.remove(blk)
+            for b in blk.bits:
+                owner.pop(b, None)
+        if nb < 4:
+            extra = [b for b in sorted(local) if b not in bits][: 4 - nb]
+            mats = np.stack([embed(m, bits, (), bits + extra) for m in mats])
+            bits = bits + extra
+        vb = tuple(obits)
+        out.append(LOp("block", tuple(bits), mats[0] if not vb else mats, vbits=vb))
     return out
 
 
@@ -1108,22 +1152,23 @@ def plan_registers(p: Pass, coalesce: int, costs: dict | None = None) -> Pass:
 
 def plan_passes(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
                 blocks: bool = True, kmax: int = 5, costs: dict | None = None,
-                variant_bits: int | None = None) -> list:
+                variant_bits: int | None = None, kron_merge: bool = True) -> list:
     """The tile passes of a forward program (before packing)."""
     if fuse:
         ops = fuse_single_bit(ops)
     passes = schedule(ops, n_bits, L, coalesce)
     if blocks:
-        passes = [Pass(p.L, p.lpos, fuse_blocks(p.ops, set(p.lpos), p.L, kmax, costs=costs, variant_bits=variant_bits))
+        passes = [Pass(p.L, p.lpos, fuse_blocks(p.ops, set(p.lpos), p.L, kmax, costs=costs, variant_bits=variant_bits,
+                                                kron_merge=kron_merge))
                   for p in passes]
     return [plan_registers(order_tile_bits(p, coalesce), coalesce, costs) for p in passes]
 
 
 def compile_ops(ops: list[LOp], n_bits: int, *, L: int = 12, coalesce: int = 4, fuse: bool = True,
                 blocks: bool = True, kmax: int = 5, costs: dict | None = None,
-                variant_bits: int | None = None) -> PackedProgram:
+                variant_bits: int | None = None, kron_merge: bool = True) -> PackedProgram:
     return PackedProgram(plan_passes(ops, n_bits, L=L, coalesce=coalesce, fuse=fuse, blocks=blocks, kmax=kmax,
-                                     costs=costs, variant_bits=variant_bits), n_bits)
+                                     costs=costs, variant_bits=variant_bits, kron_merge=kron_merge), n_bits)
 
 
 # --------------------------------------------------------------------------------------------

@@ -59,7 +59,8 @@ def test_structure_key():
 
 
 def test_default_config_and_cache_file(tmp_path):
-    assert default_config(30) == TuneConfig(tile_bits=12, coalesce_bits=4)
+    assert default_config(30) == TuneConfig(tile_bits=12, coalesce_bits=4, kmax=4)
+    assert default_config(32, dm=True).kmax == 5
     assert default_config(16).tile_bits == 10 and default_config(6).tile_bits == 6
     path = str(tmp_path / "tune.json")
     t = Autotuner(path)
