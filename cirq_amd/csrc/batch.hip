@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_batch_init(float2* __restrict__ arena, 
 }
 
 template <int L>
-__global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 3 : 1) void k_batch_pass(
+__global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_batch_pass(
     float2* __restrict__ arena, const cqs_batch_item* __restrict__ items, int n_items,
     const cqs_batch_pass* __restrict__ passes, const cqs_op* __restrict__ ops, const float2* __restrict__ mats,
     const int32_t* __restrict__ aux) {

@@ -41,7 +41,7 @@ __global__ void k_scale(float2* __restrict__ a, uint64_t n, float2 s) {
 // One pass over a single state: workgroup b owns tile b (2^L amplitudes).  The tile goes from HBM
 // straight into registers and back; LDS only carries the ops that mix elements across threads.
 template <int L>
-__global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 3 : 1) void k_pass(float2* __restrict__ psi, TileProg P) {
+__global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_pass(float2* __restrict__ psi, TileProg P) {
   extern __shared__ float2 lds[];
   const uint64_t base = deposit64((uint64_t)blockIdx.x, P.opos, P.nouter);
   pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds);
