@@ -40,8 +40,11 @@ __global__ void k_scale(float2* __restrict__ a, uint64_t n, float2 s) {
 
 // One pass over a single state: workgroup b owns tile b (2^L amplitudes).  The tile goes from HBM
 // straight into registers and back; LDS only carries the ops that mix elements across threads.
+// Every tile width runs 16 waves per CU (4 per SIMD, 128 VGPRs): 4 workgroups of 256 threads up to
+// L = 12, 2 of 512 at L = 13, 1 of 1024 at L = 14 (the LDS tile allows no more).
 template <int L>
-__global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_pass(float2* __restrict__ psi, TileProg P) {
+__global__ __launch_bounds__(TileShape<L>::NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pass(
+    float2* __restrict__ psi, TileProg P) {
   extern __shared__ float2 lds[];
   const uint64_t base = deposit64((uint64_t)blockIdx.x, P.opos, P.nouter);
   pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds);
