@@ -10,7 +10,7 @@ __global__ __launch_bounds__(TileShape<L>::NT) void k_adjoint_pass(float2* __res
                                                                  int64_t ntiles) {
   extern __shared__ float2 lds[];
   __shared__ double red[TileShape<L>::NT / 64];
-  const uint64_t base = deposit64((uint64_t)blockIdx.x, P.opos, P.nouter);
+  const uint64_t base = deposit_bounded<31>((uint64_t)blockIdx.x, P.opos, P.nouter);
   adjoint_tile<L>(psi, phi, P, P.lpos, base, partial, ntiles, blockIdx.x, lds, red);
 }
 

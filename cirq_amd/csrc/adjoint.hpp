@@ -92,7 +92,9 @@ __device__ __forceinline__ double grad_diag(const float2* psi, const float2* phi
   constexpr int TB = __builtin_ctz(NT);
   const int tid = threadIdx.x;
   int fixed = 0, emask = 0;
-  for (int m = 0; m < op.k; ++m) {
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {  // k <= 6
+    if (m >= op.k) break;
     const int t = op.t[m];
     if (t >= kOuter)
       fixed |= (int)((base >> (t - kOuter)) & 1ull) << m;
@@ -106,8 +108,9 @@ __device__ __forceinline__ double grad_diag(const float2* psi, const float2* phi
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     int idx = fixed;
-    for (int m = 0; m < op.k; ++m)
-      if ((emask >> m) & 1) idx |= ((e >> (op.t[m] - TB)) & 1) << m;
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+      if (m < op.k && ((emask >> m) & 1)) idx |= ((e >> (op.t[m] - TB)) & 1) << m;
     const float2 d = Dg[idx], x = psi[pad_nat<NT>(e)], f = phi[pad_nat<NT>(e)];
     const double yr = (double)d.x * x.x - (double)d.y * x.y;
     const double yi = (double)d.x * x.y + (double)d.y * x.x;

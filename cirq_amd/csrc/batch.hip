@@ -68,7 +68,7 @@ __global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_batch_pas
   const cqs_batch_item& I = items[find_item(items, n_items, blockIdx.x)];
   const cqs_batch_pass& B = passes[I.pass];
   const ProgView P = view_of(B, I, ops, mats, aux);
-  const uint64_t base = deposit64((uint64_t)blockIdx.x - (uint64_t)I.tile_begin, B.opos, B.nouter);
+  const uint64_t base = deposit_bounded<31>((uint64_t)blockIdx.x - (uint64_t)I.tile_begin, B.opos, B.nouter);
   pass_tile<L>(arena + I.state, P, B.lpos, B.lpos_out, base, lds);
 }
 
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(TileShape<L>::NT) void k_batch_adjoint(
   const cqs_batch_pass& B = passes[I.pass];
   const ProgView P = view_of(B, I, ops, mats, aux);
   const int64_t tile = (int64_t)blockIdx.x - I.tile_begin;
-  const uint64_t base = deposit64((uint64_t)tile, B.opos, B.nouter);
+  const uint64_t base = deposit_bounded<31>((uint64_t)tile, B.opos, B.nouter);
   adjoint_tile<L>(arena + I.state, arena + I.state2, P, B.lpos, base, partial + I.partial, int64_t(1) << B.nouter,
                   tile, lds, red);
 }

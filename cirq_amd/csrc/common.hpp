@@ -56,6 +56,16 @@ __device__ __forceinline__ uint64_t deposit64(uint64_t x, const int32_t* pos, in
   for (int b = 0; b < nb; ++b) r |= ((x >> b) & 1ull) << pos[b];
   return r;
 }
+// deposit64 for x < 2^MAXB (a workgroup's tile index): unrolled to a compile-time bound so that
+// the position loads are issued together instead of one dependent scalar load per bit.
+template <int MAXB>
+__device__ __forceinline__ uint64_t deposit_bounded(uint64_t x, const int32_t* pos, int nb) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b)
+    if (b < nb) r |= ((x >> b) & 1ull) << pos[b];
+  return r;
+}
 
 // Insert a zero bit at position b of x.
 __device__ __forceinline__ int insert0(int x, int b) {

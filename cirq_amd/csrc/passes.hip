@@ -46,7 +46,7 @@ template <int L>
 __global__ __launch_bounds__(TileShape<L>::NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pass(
     float2* __restrict__ psi, TileProg P) {
   extern __shared__ float2 lds[];
-  const uint64_t base = deposit64((uint64_t)blockIdx.x, P.opos, P.nouter);
+  const uint64_t base = deposit_bounded<31>((uint64_t)blockIdx.x, P.opos, P.nouter);
   pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds);
 }
 

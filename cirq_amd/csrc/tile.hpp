@@ -312,7 +312,9 @@ __device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, sop_t& 
 #pragma unroll
   for (int r = 0; r < (NR > 0 ? NR : 1); ++r) w[r] = 0;
   int any_reg = 0;
-  for (int m = 0; m < op.k; ++m) {
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {  // k <= 6; unrolled so the target loads issue together
+    if (m >= op.k) break;
     const int t = op.t[m];
     if (t >= kOuter) {
       fixed |= (int)((base >> (t - kOuter)) & 1ull) << m;
@@ -351,7 +353,11 @@ __device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const Prog& P, sop_t&
   smat_t* ph = scalar_ptr(P.mats) + op.mat;
   const int tid = threadIdx.x;
   float2 cph = make_float2(1.f, 0.f);
-  for (int p = 0; p < op.k; ++p) {
+  for (int p0 = 0; p0 < op.k; p0 += 8)  // chunks of 8 pairs, unrolled: their aux loads issue together
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int p = p0 + q;
+    if (p >= op.k) break;
     const int ab[2] = {pr[2 * p], pr[2 * p + 1]};
     bool en = true;
     int tneed = 0, rneed = 0;
@@ -432,8 +438,9 @@ __device__ __forceinline__ bool is_register_dense(sop_t& op) {
   constexpr int NR = __builtin_ctz(PER);
   if (op.kind != CQS_OP_DENSE || op.k < 1 || op.k > 3 || op.k > NR) return false;
   if (op.t[0] < TB) return false;
-  for (int m = 1; m < op.k; ++m)
-    if (op.t[m] <= op.t[m - 1]) return false;
+#pragma unroll
+  for (int m = 1; m < 3; ++m)
+    if (m < op.k && op.t[m] <= op.t[m - 1]) return false;
   return true;
 }
 
@@ -493,7 +500,9 @@ __device__ __forceinline__ void regs_to_tile_permuted(const float2 (&v)[PER], fl
   constexpr int NR = __builtin_ctz(PER);
   const int tb = L < TB ? L : TB;
   int jt = 0;
-  for (int b = 0; b < tb; ++b) jt |= ((threadIdx.x >> b) & 1) << dst[b];
+#pragma unroll
+  for (int b = 0; b < TB; ++b)
+    if (b < tb) jt |= ((threadIdx.x >> b) & 1) << dst[b];
   int eb[NR > 0 ? NR : 1];
 #pragma unroll
   for (int r = 0; r < NR; ++r) eb[r] = 1 << dst[TB + r];

@@ -920,11 +920,12 @@ class PackedProgram:
             run_pairs: list[LOp] = []
 
             def flush():
-                if run_pairs:
-                    bits = tuple(b for p in run_pairs for b in p.targets)
-                    phases = np.array([p.mat for p in run_pairs], dtype=np.complex128)
+                for i in range(0, len(run_pairs), 8):  # the kernel walks pairs in unrolled chunks of 8
+                    chunk = run_pairs[i : i + 8]
+                    bits = tuple(b for p in chunk for b in p.targets)
+                    phases = np.array([p.mat for p in chunk], dtype=np.complex128)
                     ops.append(LOp("pairs", bits, phases))
-                    run_pairs.clear()
+                run_pairs.clear()
 
             for op in ps.ops:
                 if op.kind == "pair":
