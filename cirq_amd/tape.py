@@ -778,8 +778,8 @@ class PackedProgram:
                     local, layout = new_local, list(op.targets)
                     recs.append(r)
                     continue
-                if op.kind == "dense" and len(op.targets) <= 3:
-                    op = sorted_targets(op)
+                if (op.kind == "dense" and len(op.targets) <= 3) or op.kind == "block":
+                    op = sorted_targets(op, key=local.get)
                 if op.kind == "dense":
                     local_ctl = [(b, v) for b, v in op.controls if b in local]
                     outer_ctl = [(b, v) for b, v in op.controls if b not in local]
@@ -963,12 +963,14 @@ def order_tile_bits(p: Pass, coalesce: int) -> Pass:
     return Pass(L, tuple(sorted(low) + rest), p.ops)
 
 
-def sorted_targets(op: LOp) -> LOp:
-    """The same dense op with its targets in ascending order (matrix indices permuted to match),
-    the form the pass kernel runs in registers."""
+def sorted_targets(op: LOp, key=None) -> LOp:
+    """The same dense op or block with its targets in ascending order of key(bit) (default: the
+    bit; the packer passes the bit's tile position), matrix indices permuted to match: the form
+    the pass kernel runs in registers."""
     t = list(op.targets)
     k = len(t)
-    order = sorted(range(k), key=lambda m: t[m])  # new target m' is old target order[m']
+    key = key or (lambda b: b)
+    order = sorted(range(k), key=lambda m: key(t[m]))  # new target m' is old target order[m']
     if order == list(range(k)):
         return op
     P = np.array([sum(((i >> m) & 1) << order[m] for m in range(k)) for i in range(1 << k)])
