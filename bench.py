@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--adjoint-reps", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (reference for HBM byte counters) and exit")
+    ap.add_argument("--layout-probe", action="store_true",
+                    help="time load/store-only passes over tile layouts and the grid program's load/store floor")
     ap.add_argument("--no-dm", action="store_true", help="skip the noisy density-matrix leg")
     ap.add_argument("--dm-qubits", type=int, default=16)
     ap.add_argument("--dm-layers", type=int, default=20)
@@ -407,6 +409,73 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
 
 
 
+def layout_probe(args):
+    """How fast HBM streams for the access patterns the pass kernel makes, on the rows x cols state.
+
+    Times (HIP events) a device-to-device copy of the state and passes that only load and store it
+    (one SCALE by 1) with a 12-bit tile: 4 contiguous low bits plus 8 bits placed in different
+    ranges of the flat index.  Bits >= 18 are 2 MiB apart or more in memory.  Then, for the grid
+    circuit compiled with 12- and 13-bit tiles, the sum of such load/store-only passes on the
+    program's own pass layouts (its streaming floor) beside the program's real pass time."""
+    import torch
+
+    from cirq_amd import engine, generators, tape
+
+    n = args.rows * args.cols
+    dev = engine.require_gpu()
+    st = engine.new_state(n, dev)
+    gb = 2 * 8 * 2**n / 1e9
+
+    def timed(fn, reps=4):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def load_store(L, lpos):
+        one = tape.LOp("scale", (), np.complex128(1.0))
+        dp = engine.DeviceProgram(tape.PackedProgram([tape.Pass(L, tuple(lpos), [one])], n), dev)
+        return timed(lambda: dp.run(st))
+
+    out = {"n_qubits": n, "state_gb": gb / 2}
+    y = torch.empty_like(st)
+    ms = timed(lambda: y.copy_(st))
+    out["device_copy"] = {"ms": ms, "tb_s": gb / ms}
+    del y
+    torch.cuda.empty_cache()
+    low = list(range(4))
+    pick = lambda lo, hi, seed: sorted(np.random.RandomState(seed).choice(np.arange(lo, hi), 8, replace=False).tolist())
+    layouts = {
+        "contiguous 0-11": [list(range(12))],
+        "4 low + 8 in [4,18)": [low + pick(4, 18, s) for s in range(3)],
+        "4 low + 8 in [18,n)": [low + pick(18, n, s) for s in range(3)],
+        "4 low + top 8": [low + list(range(n - 8, n))],
+    }
+    out["load_store_L12"] = {}
+    for name, lps in layouts.items():
+        ts = [load_store(12, lp) for lp in lps]
+        out["load_store_L12"][name] = {"ms": float(np.mean(ts)), "tb_s": gb / float(np.mean(ts)), "layouts": lps}
+    circuit = generators.grid_circuit(args.rows, args.cols, args.depth, seed=0)
+    lw = tape.Lowering(sorted(circuit.all_qubits()), dm=False)
+    lw.lower_circuit(circuit)
+    out["grid_program"] = {}
+    for L in (12, 13):
+        p = tape.compile_ops(lw.ops, n, L=L, coalesce=4, kmax=4)
+        dp = engine.DeviceProgram(p, dev)
+        real = timed(lambda: dp.run(st), reps=2)
+        floor = sum(load_store(int(r["L"]), [int(x) for x in r["lpos"][: int(r["L"])]]) for r in p.pass_arr)
+        high = [sum(1 for x in r["lpos"][: int(r["L"])] if int(x) >= 18) for r in p.pass_arr]
+        out["grid_program"][f"L{L}"] = {"passes": int(p.n_passes), "pass_ms": real, "load_store_floor_ms": floor,
+                                       "mean_local_bits_ge_18": float(np.mean(high))}
+    print(json.dumps(out))
+
+
 def q32_leg(args):
     """The larger configuration's 32-qubit variant: the random grid circuit on a 4x8 grid, depth 20
     (32 GiB complex64 state), with the same step as the main leg (10^6 samples, 52 <ZZ>)."""
@@ -449,6 +518,10 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n, "copies": 5}))
         return
 
+    if args.layout_probe:
+        layout_probe(args)
+        return
+
     from cirq_amd import generators
 
     g = grid_workload(args, args.rows, args.cols, args.depth, args.reps, args.steps, args.warmup,
@@ -458,11 +531,14 @@ def main():
     avg_launch_s, bytes_per_launch, achieved = g["avg_launch_s"], g["bytes_per_launch"], g["achieved"]
     ms_per_step, layers_per_s, tune_info = g["ms_per_step"], g["layers_per_s"], g["tune_info"]
 
-    traffic = None
+    traffic = None  # measured HBM bytes per launch (rocprofv3 PMC), if measured for this tile width
     tpath = os.path.join(ROOT, "profiles", f"traffic_k_pass_{n}q.json")
+    tile_l = int(comp.packed.pass_arr["L"].max())
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            rec = json.load(f)
+        if rec.get("kernel") == f"k_pass<{tile_l}>":
+            traffic = rec.get("bytes_per_launch")
 
     out = {
         "metric": "30q layers/s & HBM GB/s; adjoint vs param-shift grads/s at grad err<=1e-5",

@@ -52,6 +52,24 @@ def test_grid_circuit_matches_oracle(gpu, n, depth, seed):
     assert fidelity(got, want) >= 1 - 1e-6
 
 
+@pytest.mark.parametrize("tile_bits", [12, 13])
+@pytest.mark.parametrize("kind", ["grid", "random"])
+def test_wide_tiles_match_oracle(gpu, tile_bits, kind):
+    """Tiles of 12 and 13 bits (16 elements per thread, 4 register bits) on a 15-qubit state: the
+    widths the large-state simulators run, with PERMUTE re-layouts and MFMA blocks in each pass."""
+    _log(f"wide tiles tile_bits={tile_bits} kind={kind}")
+    if kind == "grid":
+        c = generators.grid_circuit(3, 5, 16, seed=4)
+        qs = sorted(c.all_qubits())
+    else:
+        c, qs = _random_circuit(15, 200, np.random.RandomState(21))
+    sim = cirq.Simulator(tile_bits=tile_bits, coalesce_bits=4)
+    got = sim.simulate(c, qubit_order=qs).final_state_vector
+    want = O.simulate_state_vector(to_oracle_ops(c, qs), len(qs))
+    assert np.max(np.abs(got - want)) <= ATOL
+    assert fidelity(got, want) >= 1 - 1e-6
+
+
 def test_sampling_bit_exact_with_numpy_choice(gpu):
     """Same p-vector (from the GPU state) + same RandomState -> identical indices."""
     _log("sampling bit-exact")
