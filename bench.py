@@ -59,6 +59,11 @@ def parse():
     ap.add_argument("--no-dm-adjoint", action="store_true", help="skip the noisy density-matrix gradient leg")
     ap.add_argument("--dm-adjoint-qubits", type=int, default=12)
     ap.add_argument("--dm-adjoint-layers", type=int, default=10)
+    ap.add_argument("--no-trajectories", action="store_true", help="skip the noisy-trajectory leg")
+    ap.add_argument("--traj-qubits", type=int, default=16)
+    ap.add_argument("--traj-layers", type=int, default=10)
+    ap.add_argument("--traj-reps", type=int, default=1024)
+    ap.add_argument("--traj-looped-reps", type=int, default=3)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
     ap.add_argument("--sweep-circuits", type=int, default=256)
     ap.add_argument("--sweep-points", type=int, default=16)
@@ -115,6 +120,46 @@ def dm_leg(args):
         "pass_avg_ms": ms / max(n_pass, 1),
         "pass_hbm_gbs": bytes_per_pass / (ms / 1e3 / max(n_pass, 1)) / 1e9,
         "tr_rho_sum_z": ev,
+    }
+
+
+def trajectory_leg(args):
+    """run() of a noisy circuit on a state vector: every channel is sampled per repetition, as
+    Cirq's simulator does.  Batched (all repetitions as copies of one state) against one
+    repetition at a time (a sample of repetitions, scaled), each after a warm-up call that
+    compiles its programs."""
+    import cirq_amd as cirq
+    from cirq_amd import generators
+
+    circuit = generators.noisy_line_circuit(args.traj_qubits, args.traj_layers, seed=0)
+    qs = sorted(circuit.all_qubits())
+    from cirq_amd.tape import _has_channel
+
+    n_channels = sum(1 for op in circuit.all_operations() if _has_channel(op))
+    circuit = circuit + cirq.Circuit(cirq.measure(*qs, key="m"))
+    sim = cirq.Simulator(seed=1)
+    t0 = time.perf_counter()
+    sim.run(circuit, repetitions=args.traj_reps)
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = sim.run(circuit, repetitions=args.traj_reps)
+    batched = time.perf_counter() - t0
+    one = cirq.Simulator(seed=1, trajectory_batch=1)
+    one.run(circuit, repetitions=1)
+    t0 = time.perf_counter()
+    one.run(circuit, repetitions=args.traj_looped_reps)
+    looped = (time.perf_counter() - t0) / args.traj_looped_reps
+    ones = res.records["m"][:, 0, :].mean()
+    return {
+        "workload": f"noisy line {args.traj_qubits} qubits x {args.traj_layers} layers on a state vector, "
+        f"{n_channels} sampled channels per repetition, {args.traj_reps} repetitions",
+        "batched_s": batched,
+        "batched_reps_per_s": args.traj_reps / batched,
+        "first_call_s": first,
+        "looped_reps_per_s": 1.0 / looped,
+        "looped_sample": f"{args.traj_looped_reps} repetitions one at a time ({looped * args.traj_looped_reps:.2f} s)",
+        "speedup": looped * args.traj_reps / batched,
+        "mean_measured_bit": float(ones),
     }
 
 
@@ -591,6 +636,8 @@ def main():
         out["extra"]["dm_adjoint"] = dm_adjoint_leg(args)
     if not args.no_sweep:
         out["extra"]["sweep"] = sweep_leg(args)
+    if not args.no_trajectories:
+        out["extra"]["trajectories"] = trajectory_leg(args)
     if not args.no_dm:
         out["extra"]["density_matrix"] = dm_leg(args)
     if not args.no_cpu_baseline:

@@ -517,9 +517,12 @@ __device__ __forceinline__ float pairwise_short(uint64_t len, F at) {
   return ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
 }
 
-// Short rows (<= 128 unmeasured combinations): one thread per output row.
+// Short rows (<= 128 unmeasured combinations): one thread per output row.  blockIdx.y picks one
+// of several states stored `stride` entries apart (and its own 2^m outputs).
 __global__ __launch_bounds__(256) void k_marginal_short(Loader ld, int n, AxisBits ab, int m, uint64_t rows,
-                                                        float* __restrict__ out) {
+                                                        float* __restrict__ out, uint64_t stride) {
+  ld.amps += (uint64_t)blockIdx.y * stride;
+  out += (uint64_t)blockIdx.y * rows;
   const uint64_t row = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (row >= rows) return;
   const int rlen_bits = n - m;
@@ -533,7 +536,10 @@ __global__ __launch_bounds__(256) void k_marginal_short(Loader ld, int n, AxisBi
   out[row] = pairwise_short(1ull << rlen_bits, at);
 }
 
-__global__ __launch_bounds__(256) void k_marginal(Loader ld, int n, AxisBits ab, int m, float* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_marginal(Loader ld, int n, AxisBits ab, int m, float* __restrict__ out,
+                                                  uint64_t stride) {
+  ld.amps += (uint64_t)blockIdx.y * stride;
+  out += (uint64_t)blockIdx.y * gridDim.x;
   const int* axes_bits = ab.b;
   // axes_bits[0..m) : flat bit of measured qubit k (k-th most significant output bit)
   // axes_bits[m..n) : flat bits of the unmeasured qubits in ascending axis order
@@ -838,7 +844,13 @@ int cqs_indices_to_bits(const int64_t* index, int64_t reps, int32_t m, uint8_t* 
 
 int cqs_marginal_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const int32_t* qubit_axes,
                        int32_t m, float* out, void* stream) {
+  return cqs_marginal_probs_batch(amps, n_qubits, is_dm, 1, qubit_axes, m, out, stream);
+}
+
+int cqs_marginal_probs_batch(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_states,
+                             const int32_t* qubit_axes, int32_t m, float* out, void* stream) {
   CQS_REQUIRE(amps != nullptr && out != nullptr && qubit_axes != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_states >= 1 && n_states <= 65535, "n_states=%d out of range (1..65535)", n_states);
   CQS_REQUIRE(n_qubits >= 1 && n_qubits <= 40 && m >= 1 && m <= n_qubits && m <= 31, "bad sizes");
   // The device needs the flat bit of every axis: measured first, then the unmeasured ones.
   // qubit_axes is a HOST array of m Cirq axes.
@@ -855,13 +867,15 @@ int cqs_marginal_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
     if (!used[a]) bits.b[w++] = n_qubits - 1 - a;
   const Loader ld = make_loader(amps, nullptr, n_qubits, is_dm);
   const uint64_t rows = 1ull << m;
+  const uint64_t stride = is_dm ? (1ull << (2 * n_qubits)) : (1ull << n_qubits);  // entries per state
   if (n_qubits - m <= 7) {
-    hipLaunchKernelGGL(k_marginal_short, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, as_stream(stream), ld,
-                       (int)n_qubits, bits, (int)m, rows, out);
+    CQS_REQUIRE((rows + 255) / 256 <= 0x7fffffffull, "too many outcomes");
+    hipLaunchKernelGGL(k_marginal_short, dim3((unsigned)((rows + 255) / 256), n_states), dim3(256), 0,
+                       as_stream(stream), ld, (int)n_qubits, bits, (int)m, rows, out, stride);
   } else {
     CQS_REQUIRE(m <= 24, "marginals over more than 2^24 outcomes with long rows are not supported");
-    hipLaunchKernelGGL(k_marginal, dim3((unsigned)rows), dim3(256), 0, as_stream(stream), ld, (int)n_qubits, bits,
-                       (int)m, out);
+    hipLaunchKernelGGL(k_marginal, dim3((unsigned)rows, n_states), dim3(256), 0, as_stream(stream), ld, (int)n_qubits,
+                       bits, (int)m, out, stride);
   }
   CQS_HIP(hipGetLastError());
   return CQS_OK;

@@ -1,0 +1,253 @@
+// Batched trajectories: R copies of a state, one per repetition, stored one after the other
+// (copy r at amps + r * 2^n_bits).  Deterministic stretches of a circuit run on all copies at
+// once as one pass program over n_bits + log2(R) bits (the copy index is a block of outer bits no
+// op touches); what differs per copy is here: the projection after a measurement, the matrix a
+// mixture or Kraus channel picked, and the Kraus weights ||K_i psi_r||^2 that the pick depends on.
+// Grids put the copy in blockIdx.y, so per-copy values are uniform within a workgroup.
+#include "common.hpp"
+
+namespace cqs {
+
+__device__ __forceinline__ uint64_t insert0_64(uint64_t x, int b) {
+  return ((x >> b) << (b + 1)) | (x & ((1ull << b) - 1ull));
+}
+
+struct SelTargets {
+  int t[3];   // flat bit of matrix index bit m
+  int ts[3];  // the same bits in ascending order
+};
+
+// Flat index of group g's first element (zeros at the targets) and its 2^K members.
+template <int K>
+__device__ __forceinline__ void group_indices(uint64_t g, const SelTargets& T, uint64_t (&idx)[1 << K]) {
+  uint64_t j0 = g;
+#pragma unroll
+  for (int s = 0; s < K; ++s) j0 = insert0_64(j0, T.ts[s]);
+#pragma unroll
+  for (int m = 0; m < (1 << K); ++m) {
+    uint64_t j = j0;
+#pragma unroll
+    for (int b = 0; b < K; ++b) j |= (uint64_t)((m >> b) & 1) << T.t[b];
+    idx[m] = j;
+  }
+}
+
+// amps_r[i] = ((i & mask) == values[r]) ? amps_r[i] * scales[r] : 0
+__global__ __launch_bounds__(256) void k_collapse_batch(float2* __restrict__ amps, uint64_t per, uint64_t mask,
+                                                        const uint64_t* __restrict__ values,
+                                                        const float* __restrict__ scales) {
+  float2* s = amps + (uint64_t)blockIdx.y * per;
+  const uint64_t v = values[blockIdx.y];
+  const float sc = scales[blockIdx.y];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per; i += stride) {
+    const float2 x = s[i];
+    s[i] = ((i & mask) == v) ? make_float2(x.x * sc, x.y * sc) : make_float2(0.f, 0.f);
+  }
+}
+
+// Copy r: every group of 2^K amplitudes on the targets <- scales[r] * (M_{select[r]} x), the
+// product accumulated from zero in column order.  A copy whose select is out of range is left
+// as it is (the host checks the selection before it uploads it).
+template <int K>
+__global__ __launch_bounds__(256) void k_select_batch(float2* __restrict__ amps, uint64_t per, SelTargets T,
+                                                      const float2* __restrict__ mats, int n_mats,
+                                                      const int32_t* __restrict__ select,
+                                                      const float* __restrict__ scales) {
+  constexpr int D = 1 << K;
+  const int sel = select[blockIdx.y];
+  if (sel < 0 || sel >= n_mats) return;
+  const float2* M = mats + (uint64_t)sel * D * D;
+  const float sc = scales[blockIdx.y];
+  float2* s = amps + (uint64_t)blockIdx.y * per;
+  const uint64_t groups = per >> K;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
+    uint64_t idx[D];
+    group_indices<K>(g, T, idx);
+    float2 x[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) x[c] = s[idx[c]];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc = cfma(M[r * D + c], x[c], acc);
+      s[idx[r]] = make_float2(acc.x * sc, acc.y * sc);
+    }
+  }
+}
+
+constexpr int kMaxKraus = 16;
+
+// partial[(r * gridDim.x + blockIdx.x) * n_mats + i] = sum over this workgroup's groups of copy r
+// of ||K_i x||^2, in FP64 (fixed order: per-thread sums, then a wave tree, then the 4 waves).
+template <int K>
+__global__ __launch_bounds__(256) void k_kraus_weights(const float2* __restrict__ amps, uint64_t per, SelTargets T,
+                                                       const float2* __restrict__ mats, int n_mats,
+                                                       double* __restrict__ partial) {
+  constexpr int D = 1 << K;
+  __shared__ float2 sm[kMaxKraus * D * D];
+  __shared__ double red[4][kMaxKraus];
+  for (int i = threadIdx.x; i < n_mats * D * D; i += blockDim.x) sm[i] = mats[i];
+  __syncthreads();
+  const float2* s = amps + (uint64_t)blockIdx.y * per;
+  const uint64_t groups = per >> K;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  double acc[kMaxKraus];
+#pragma unroll
+  for (int i = 0; i < kMaxKraus; ++i) acc[i] = 0.0;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
+    uint64_t idx[D];
+    group_indices<K>(g, T, idx);
+    float2 x[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) x[c] = s[idx[c]];
+#pragma unroll
+    for (int i = 0; i < kMaxKraus; ++i) {
+      if (i >= n_mats) break;
+      const float2* M = sm + i * D * D;
+      double w = 0.0;
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        float2 y = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < D; ++c) y = cfma(M[r * D + c], x[c], y);
+        w += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
+      }
+      acc[i] += w;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kMaxKraus; ++i) {
+    if (i >= n_mats) break;
+    const double v = wave_sum(acc[i]);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < n_mats) {
+    const int i = threadIdx.x;
+    partial[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * n_mats + i] =
+        ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+  }
+}
+
+// out[r * n_mats + i] = sum over b of partial[(r * nblk + b) * n_mats + i], b ascending.
+__global__ void k_kraus_finish(const double* __restrict__ partial, int nblk, int n_mats, int n_states,
+                               double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_states * n_mats) return;
+  const int r = e / n_mats, i = e % n_mats;
+  double v = 0.0;
+  for (int b = 0; b < nblk; ++b) v += partial[((uint64_t)r * nblk + b) * n_mats + i];
+  out[e] = v;
+}
+
+// Workgroups per copy: enough for about 4096 workgroups in all, at most one per 256 groups.
+static int blocks_per_copy(uint64_t groups, int n_states) {
+  uint64_t want = (groups + 255) / 256;
+  uint64_t cap = (uint64_t)(4096 / (n_states > 0 ? n_states : 1));
+  if (cap < 1) cap = 1;
+  if (want > cap) want = cap;
+  if (want < 1) want = 1;
+  return (int)want;
+}
+
+static int make_targets(SelTargets& T, int n_bits, int k, const int32_t* targets) {
+  CQS_REQUIRE(k >= 1 && k <= 3, "batched channel ops act on 1-3 qubits (got %d)", k);
+  CQS_REQUIRE(targets != nullptr, "targets is NULL");
+  uint64_t seen = 0;
+  for (int m = 0; m < k; ++m) {
+    const int b = targets[m];
+    CQS_REQUIRE(b >= 0 && b < n_bits && !((seen >> b) & 1ull), "bad or repeated target bit %d", b);
+    seen |= 1ull << b;
+    T.t[m] = b;
+    T.ts[m] = b;
+  }
+  for (int a = 1; a < k; ++a)
+    for (int b = a; b > 0 && T.ts[b - 1] > T.ts[b]; --b) {
+      const int x = T.ts[b - 1];
+      T.ts[b - 1] = T.ts[b];
+      T.ts[b] = x;
+    }
+  for (int m = k; m < 3; ++m) T.t[m] = T.ts[m] = 0;
+  return CQS_OK;
+}
+
+}  // namespace cqs
+
+using namespace cqs;
+
+extern "C" {
+
+int cqs_collapse_batch(void* amps, int32_t n_bits, int32_t n_states, uint64_t mask, const uint64_t* values,
+                       const float* scales, void* stream) {
+  CQS_REQUIRE(amps != nullptr && values != nullptr && scales != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_bits >= 0 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
+  CQS_REQUIRE(n_states >= 1 && n_states <= 65535, "n_states=%d out of range (1..65535)", n_states);
+  const uint64_t per = 1ull << n_bits;
+  const int gx = blocks_per_copy(per, n_states);
+  hipLaunchKernelGGL(k_collapse_batch, dim3(gx, n_states), dim3(256), 0, as_stream(stream), (float2*)amps, per, mask,
+                     values, scales);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+int cqs_apply_select_batch(void* amps, int32_t n_bits, int32_t n_states, int32_t k, const int32_t* targets,
+                           const void* mats, int32_t n_mats, const int32_t* select, const float* scales,
+                           void* stream) {
+  CQS_REQUIRE(amps != nullptr && mats != nullptr && select != nullptr && scales != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_bits >= 1 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
+  CQS_REQUIRE(n_states >= 1 && n_states <= 65535, "n_states=%d out of range (1..65535)", n_states);
+  CQS_REQUIRE(n_mats >= 1, "n_mats=%d", n_mats);
+  SelTargets T;
+  if (int rc = make_targets(T, n_bits, k, targets)) return rc;
+  const uint64_t per = 1ull << n_bits;
+  const int gx = blocks_per_copy(per >> k, n_states);
+  const dim3 grid(gx, n_states);
+  hipStream_t s = as_stream(stream);
+  const float2* M = (const float2*)mats;
+  switch (k) {
+    case 1: hipLaunchKernelGGL(k_select_batch<1>, grid, dim3(256), 0, s, (float2*)amps, per, T, M, n_mats, select, scales); break;
+    case 2: hipLaunchKernelGGL(k_select_batch<2>, grid, dim3(256), 0, s, (float2*)amps, per, T, M, n_mats, select, scales); break;
+    default: hipLaunchKernelGGL(k_select_batch<3>, grid, dim3(256), 0, s, (float2*)amps, per, T, M, n_mats, select, scales); break;
+  }
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+size_t cqs_kraus_workspace_bytes(int32_t n_bits, int32_t n_states, int32_t k, int32_t n_mats) {
+  if (n_bits < 1 || n_states < 1 || k < 1 || k > 3 || n_mats < 1) return 0;
+  const int gx = blocks_per_copy((1ull << n_bits) >> k, n_states);
+  return (size_t)n_states * gx * n_mats * sizeof(double);
+}
+
+int cqs_kraus_weights_batch(const void* amps, int32_t n_bits, int32_t n_states, int32_t k, const int32_t* targets,
+                            const void* mats, int32_t n_mats, double* out, void* workspace, void* stream) {
+  CQS_REQUIRE(amps != nullptr && mats != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_bits >= 1 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
+  CQS_REQUIRE(n_states >= 1 && n_states <= 65535, "n_states=%d out of range (1..65535)", n_states);
+  CQS_REQUIRE(n_mats >= 1 && n_mats <= kMaxKraus, "n_mats=%d out of range (1..%d)", n_mats, kMaxKraus);
+  SelTargets T;
+  if (int rc = make_targets(T, n_bits, k, targets)) return rc;
+  const uint64_t per = 1ull << n_bits;
+  const int gx = blocks_per_copy(per >> k, n_states);
+  const dim3 grid(gx, n_states);
+  hipStream_t s = as_stream(stream);
+  const float2* M = (const float2*)mats;
+  double* partial = (double*)workspace;
+  switch (k) {
+    case 1: hipLaunchKernelGGL(k_kraus_weights<1>, grid, dim3(256), 0, s, (const float2*)amps, per, T, M, n_mats, partial); break;
+    case 2: hipLaunchKernelGGL(k_kraus_weights<2>, grid, dim3(256), 0, s, (const float2*)amps, per, T, M, n_mats, partial); break;
+    default: hipLaunchKernelGGL(k_kraus_weights<3>, grid, dim3(256), 0, s, (const float2*)amps, per, T, M, n_mats, partial); break;
+  }
+  CQS_HIP(hipGetLastError());
+  const int total = n_states * n_mats;
+  hipLaunchKernelGGL(k_kraus_finish, dim3((total + 255) / 256), dim3(256), 0, s, (const double*)partial, gx, n_mats,
+                     (int)n_states, out);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+}  // extern "C"

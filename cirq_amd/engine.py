@@ -134,6 +134,90 @@ def collapse(state: torch.Tensor, n_bits: int, mask: int, value: int, s: float):
     check(lib().cqs_collapse(ptr(state), n_bits, int(mask), int(value), float(s), 0.0, stream_handle()), "cqs_collapse")
 
 
+# ---------------------------------------------------------------- batched trajectories
+# `states` holds n_states copies of one state one after the other (copy r at r * 2^n_bits).
+
+
+def _dev_array(x: np.ndarray, device) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device)
+
+
+def _check_copies(states: torch.Tensor, n_bits: int, n_states: int):
+    if states.dtype != torch.complex64 or not states.is_contiguous() or states.numel() < (n_states << n_bits):
+        raise ValueError(f"need {n_states} contiguous complex64 copies of 2^{n_bits} amplitudes")
+
+
+def marginal_batch(states: torch.Tensor, n_qubits: int, is_dm: bool, n_states: int, axes) -> torch.Tensor:
+    """Float32 marginal probabilities over `axes` of every copy (device [n_states, 2^m])."""
+    _check_copies(states, 2 * n_qubits if is_dm else n_qubits, n_states)
+    m = len(axes)
+    out = torch.empty((n_states, 1 << m), dtype=torch.float32, device=states.device)
+    ax = np.array(axes, dtype=np.int32)
+    check(
+        lib().cqs_marginal_probs_batch(ptr(states), n_qubits, int(is_dm), n_states, ax.ctypes.data, m, ptr(out),
+                                       stream_handle()),
+        "cqs_marginal_probs_batch",
+    )
+    return out
+
+
+def collapse_batch(states: torch.Tensor, n_bits: int, n_states: int, mask: int, values: np.ndarray,
+                   scales: np.ndarray):
+    _check_copies(states, n_bits, n_states)
+    if np.shape(values) != (n_states,) or np.shape(scales) != (n_states,):
+        raise ValueError("one value and one scale per copy")
+    v = _dev_array(np.asarray(values, dtype=np.uint64), states.device)
+    s = _dev_array(np.asarray(scales, dtype=np.float32), states.device)
+    check(lib().cqs_collapse_batch(ptr(states), n_bits, n_states, int(mask), ptr(v), ptr(s), stream_handle()),
+          "cqs_collapse_batch")
+
+
+def apply_select_batch(states: torch.Tensor, n_bits: int, n_states: int, targets, mats: np.ndarray,
+                       select: np.ndarray, scales: np.ndarray):
+    """Copy r <- scales[r] * mats[select[r]] on `targets` (matrix index bit m <-> flat bit targets[m])."""
+    _check_copies(states, n_bits, n_states)
+    mats = np.asarray(mats, dtype=np.complex64)
+    n_mats = int(mats.shape[0])
+    d = 1 << len(targets)
+    if mats.shape != (n_mats, d, d):
+        raise ValueError(f"matrices of shape {mats.shape} for {len(targets)} target bits")
+    sel = np.asarray(select, dtype=np.int32)
+    if sel.shape != (n_states,) or np.shape(scales) != (n_states,):
+        raise ValueError("one selection and one scale per copy")
+    if sel.min() < 0 or sel.max() >= n_mats:
+        raise ValueError("selection outside the channel's matrices")
+    t = np.array(targets, dtype=np.int32)
+    m = _dev_array(mats.view(np.float32), states.device)
+    sd = _dev_array(sel, states.device)
+    sc = _dev_array(np.asarray(scales, dtype=np.float32), states.device)
+    check(
+        lib().cqs_apply_select_batch(ptr(states), n_bits, n_states, len(t), t.ctypes.data, ptr(m), n_mats, ptr(sd),
+                                     ptr(sc), stream_handle()),
+        "cqs_apply_select_batch",
+    )
+
+
+def kraus_weights_batch(states: torch.Tensor, n_bits: int, n_states: int, targets, mats: np.ndarray,
+                        ws: Workspace) -> np.ndarray:
+    """||K_i psi_r||^2 in FP64 for every copy r and Kraus operator i (host [n_states, n_mats])."""
+    _check_copies(states, n_bits, n_states)
+    mats = np.asarray(mats, dtype=np.complex64)
+    n_mats = int(mats.shape[0])
+    d = 1 << len(targets)
+    if mats.shape != (n_mats, d, d):
+        raise ValueError(f"matrices of shape {mats.shape} for {len(targets)} target bits")
+    t = np.array(targets, dtype=np.int32)
+    m = _dev_array(mats.view(np.float32), states.device)
+    out = torch.empty((n_states, n_mats), dtype=torch.float64, device=states.device)
+    w = ws.get(lib().cqs_kraus_workspace_bytes(n_bits, n_states, len(t), n_mats))
+    check(
+        lib().cqs_kraus_weights_batch(ptr(states), n_bits, n_states, len(t), t.ctypes.data, ptr(m), n_mats, ptr(out),
+                                      ptr(w), stream_handle()),
+        "cqs_kraus_weights_batch",
+    )
+    return out.cpu().numpy()
+
+
 class Sampler:
     """Bit-exact NumPy-choice sampling of (marginal) probabilities of a state."""
 

@@ -40,6 +40,7 @@ from .tape import (
     fill_param_mats,
     kraus_of,
 )
+from .trajectories import TrajectoryBatch
 
 
 def parse_random_state(seed) -> np.random.RandomState:
@@ -184,11 +185,14 @@ class _SimulatorBase:
         fuse: bool = True,
         mfma_blocks: bool = True,
         autotune=False,
+        trajectory_batch: int | None = None,
     ):
         """Cirq's arguments (dtype, noise, seed, split_untangled_states) plus engine settings:
         tile_bits / coalesce_bits / mfma_blocks fix the pass compiler's tile and fusion choices;
         autotune=True (or an autotune.Autotuner) instead times candidate settings on the GPU the
-        first time each circuit structure is compiled and keeps the fastest."""
+        first time each circuit structure is compiled and keeps the fastest; trajectory_batch caps
+        how many repetitions of a circuit with mid-circuit measurements or state-vector noise run
+        together (None: as many as a quarter of free device memory holds; 1: one at a time)."""
         if np.dtype(dtype).kind != "c":
             raise ValueError(f"dtype must be a complex type but was {dtype}")
         if np.dtype(dtype) != np.complex64:
@@ -206,6 +210,7 @@ class _SimulatorBase:
 
             autotune = default_autotuner()
         self._tuner = autotune or None
+        self._traj_batch = trajectory_batch
         self._cache: "collections.OrderedDict" = collections.OrderedDict()
         self._split_memo: dict = {}
         self._device = None
@@ -431,7 +436,9 @@ class _SimulatorBase:
         """Per-repetition simulation of a general suffix after a deterministic prefix, as
         SimulatorBase._run does (cirq-core/cirq/sim/simulator_base.py:248-273): the prefix state is
         computed once, every repetition runs the suffix on a copy, and records of a key measured
-        several times are padded to the same count."""
+        several times are padded to the same count.  Repetitions run in batches of copies
+        (trajectories.TrajectoryBatch) unless trajectory_batch is 1 or a channel is too wide for
+        the batched kernels."""
         comp = self._compile(prefix, qubits)
         self._check_resolved(resolvers, comp)
         actions = self._actions(suffix, qubits)
@@ -439,9 +446,16 @@ class _SimulatorBase:
         for op in suffix.all_operations():
             if _is_measurement_op(op) and op.gate.key not in keys:
                 keys.append(op.gate.key)
+        batch = None
+        if self._traj_batch != 1 and repetitions > 0 and TrajectoryBatch.supported(actions, self._dm):
+            batch = self._trajectory_batch(qubits, actions)
         for r, st0 in zip(resolvers, self._run_program(comp, resolvers, None)):
             per_key: dict = {k: [] for k in keys}
-            for i in range(repetitions):
+            if batch is not None:
+                for rec in batch.run(st0, r, repetitions, self._prng):
+                    for k in keys:
+                        per_key[k].append([bits for kk, bits in rec if kk == k])
+            for i in range(repetitions if batch is None else 0):
                 st = st0.clone() if i < repetitions - 1 else st0
                 records: dict = {}
                 for _ in self._run_actions(st, qubits, actions, r, records):
@@ -458,6 +472,21 @@ class _SimulatorBase:
                         xs[i, j, : len(bits)] = bits
                 out[k] = xs
             yield ResultDict(params=r, records=out)
+
+    def _trajectory_batch(self, qubits, actions) -> TrajectoryBatch:
+        """The batched runner of an action list, kept (with the programs it compiled for each
+        batch size) as long as the action list is cached."""
+        key = ("trajectories", id(actions))
+        hit = self._cache.get(key)
+        if hit is not None and hit[0] is actions:
+            self._cache.move_to_end(key)
+            return hit[1]
+        batch = TrajectoryBatch(qubits, actions, self._dm, self._dev(), self._tile_params, self._fuse, self._blocks,
+                                self._ws, self._traj_batch)
+        self._cache[key] = (actions, batch)
+        while len(self._cache) > 16:
+            self._cache.popitem(last=False)
+        return batch
 
     # ---------------------------------------------------------------- moment steps
     def simulate_moment_steps(self, circuit, param_resolver=None, qubit_order=QubitOrder.DEFAULT,

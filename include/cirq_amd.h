@@ -318,6 +318,43 @@ int cqs_marginal_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
                        int32_t m, float* out, void* stream);
 
 /*
+ * Batched trajectories.  n_states copies of a state (one per repetition of a circuit whose
+ * measurements or state-vector channels are not all terminal) lie one after the other in amps,
+ * copy r at amps + r * 2^n_bits entries (2^(2 n_qubits) for density matrices).  They replace the
+ * per-repetition loop of cirq-core/cirq/sim/simulator_base.py:248-273: the deterministic stretches
+ * run on all copies as one pass program (cqs_run_passes over n_bits + log2(n_states) bits), and
+ * the calls below do what differs per copy.  Per-copy arrays are DEVICE arrays of n_states
+ * entries; 1 <= n_states <= 65535.
+ *
+ * cqs_marginal_probs_batch: cqs_marginal_probs of every copy, out[r * 2^m ..] (the same float32
+ * arithmetic per copy), for measure_state_vector / measure_density_matrix
+ * (cirq-core/cirq/sim/state_vector.py:235-322, cirq-core/cirq/sim/density_matrix_utils.py:97-182).
+ */
+int cqs_marginal_probs_batch(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_states,
+                             const int32_t* qubit_axes, int32_t m, float* out, void* stream);
+
+/* Copy r: amps_r[i] = ((i & mask) == values[r]) ? amps_r[i] * scales[r] : 0 (each copy's
+ * measurement outcome projected and renormalised, as cqs_collapse does for one state). */
+int cqs_collapse_batch(void* amps, int32_t n_bits, int32_t n_states, uint64_t mask, const uint64_t* values,
+                       const float* scales, void* stream);
+
+/* Copy r: the k-qubit matrix mats[select[r]] (complex64, row-major 2^k x 2^k, matrix index bit m
+ * <-> flat bit targets[m]; targets: HOST int32[k], 1 <= k <= 3) times scales[r], on every group of
+ * 2^k amplitudes.  The unitary a mixture drew (cirq-core/cirq/sim/state_vector_simulation_state.py:183-203)
+ * or the Kraus operator a channel drew, renormalised (:205-257).  Copies whose select is outside
+ * [0, n_mats) are left unchanged. */
+int cqs_apply_select_batch(void* amps, int32_t n_bits, int32_t n_states, int32_t k, const int32_t* targets,
+                           const void* mats, int32_t n_mats, const int32_t* select, const float* scales,
+                           void* stream);
+
+/* out[r * n_mats + i] = ||K_i psi_r||^2 in FP64 (DEVICE double[n_states * n_mats], fixed-order
+ * sums; 1 <= n_mats <= 16): the weights cirq-core/cirq/sim/state_vector_simulation_state.py:224-250
+ * walks to pick a Kraus operator.  workspace: DEVICE, cqs_kraus_workspace_bytes bytes. */
+size_t cqs_kraus_workspace_bytes(int32_t n_bits, int32_t n_states, int32_t k, int32_t n_mats);
+int cqs_kraus_weights_batch(const void* amps, int32_t n_bits, int32_t n_states, int32_t k, const int32_t* targets,
+                            const void* mats, int32_t n_mats, double* out, void* workspace, void* stream);
+
+/*
  * out[j] (DEVICE float32[2^n_qubits]) = the probability of the basis state whose big-endian digits,
  * qubit qubit_axes[0] first, spell j: p = float32(|amp|^2) rounded as NumPy rounds it (state
  * vector) or Re rho[k, k] (density matrix).  The transpose Cirq's product state applies when it

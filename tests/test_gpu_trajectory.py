@@ -120,3 +120,97 @@ def test_density_matrix_mid_circuit_measurement(gpu):
     # q0 is now classical: populations only where q0 == m
     diag = np.real(np.diag(rho)).reshape(2, 2)
     assert np.allclose(diag[1 - m], 0, atol=1e-6)
+
+
+# ---------------------------------------------------------------- batched trajectories
+def _agreement(ra, rb, key):
+    a, b = ra.records[key], rb.records[key]
+    assert a.shape == b.shape
+    return float(np.mean(np.all(a.reshape(len(a), -1) == b.reshape(len(b), -1), axis=1)))
+
+
+def _mid_circuit(n=8, seed=0):
+    rng = np.random.RandomState(seed)
+    qs = cirq.LineQubit.range(n)
+    c = cirq.Circuit()
+    for layer in range(4):
+        for q in qs:
+            c.append(cirq.rx(rng.uniform(-3, 3))(q))
+            c.append(cirq.rz(rng.uniform(-3, 3))(q))
+        for i in range(layer % 2, n - 1, 2):
+            c.append(cirq.CNOT(qs[i], qs[i + 1]))
+        if layer == 1:
+            c.append(cirq.measure(qs[1], qs[4], key="mid"))
+        if layer == 2:
+            c.append(cirq.measure(qs[6], key="again"))
+            c.append(cirq.measure(qs[6], key="again"))
+    c.append(cirq.measure(*qs, key="end"))
+    return c
+
+
+def test_batched_trajectories_match_one_at_a_time(gpu):
+    """Every repetition consumes the same uniforms in both paths; outcomes agree except where a
+    probability rounds differently at a draw's boundary."""
+    _log("batched vs one-at-a-time trajectories (mid-circuit measurements)")
+    c = _mid_circuit()
+    sb = cirq.Simulator(seed=11)
+    ss = cirq.Simulator(seed=11, trajectory_batch=1)
+    rb = sb.run(c, repetitions=300)
+    rs = ss.run(c, repetitions=300)
+    for key in ("mid", "again", "end"):
+        assert _agreement(rb, rs, key) >= 0.99, key
+    assert rb.records["again"].shape == (300, 2, 1)
+    assert np.array_equal(rb.records["again"][:, 0], rb.records["again"][:, 1])
+    assert sb._prng.random_sample() == ss._prng.random_sample()  # the same draws were consumed
+
+
+def test_batched_trajectory_chunks(gpu):
+    """Batches of 64 copies (the last one padded) give the records of one batch of 1024."""
+    _log("batched trajectories in chunks")
+    c = _mid_circuit(seed=2)
+    r1 = cirq.Simulator(seed=5).run(c, repetitions=1000)
+    r2 = cirq.Simulator(seed=5, trajectory_batch=64).run(c, repetitions=1000)
+    for key in ("mid", "again", "end"):
+        assert _agreement(r1, r2, key) >= 0.995, key
+
+
+def _noisy_circuit():
+    qs = cirq.LineQubit.range(6)
+    c = cirq.Circuit(cirq.H.on_each(*qs))
+    c.append(cirq.CNOT(qs[i], qs[i + 1]) for i in range(5))
+    c.append([cirq.depolarize(0.08)(qs[0]), cirq.depolarize(0.05, n_qubits=2)(qs[1], qs[2]),
+              cirq.amplitude_damp(0.3)(qs[3]), cirq.bit_flip(0.1)(qs[4]), cirq.phase_damp(0.2)(qs[5])])
+    c.append(cirq.rx(0.7)(q) for q in qs)
+    c.append([cirq.amplitude_damp(0.25)(qs[0]), cirq.generalized_amplitude_damp(0.3, 0.2)(qs[2])])
+    c.append(cirq.ry(0.4)(q) for q in qs)
+    return c, qs
+
+
+def test_batched_noisy_trajectories(gpu):
+    """Mixtures and Kraus channels on a state vector: the batched path agrees with the
+    one-at-a-time path and its outcome frequencies match the density-matrix probabilities."""
+    _log("batched noisy trajectories")
+    c, qs = _noisy_circuit()
+    meas = c + cirq.Circuit(cirq.measure(qs[0], qs[2], qs[3], key="m"))
+    rb = cirq.Simulator(seed=3).run(meas, repetitions=300)
+    rs = cirq.Simulator(seed=3, trajectory_batch=1).run(meas, repetitions=300)
+    assert _agreement(rb, rs, "m") >= 0.98
+    rho = cirq.DensityMatrixSimulator().simulate(c, qubit_order=qs).final_density_matrix
+    p = np.real(np.diag(rho)).reshape([2] * 6).transpose(0, 2, 3, 1, 4, 5).reshape(8, 8).sum(axis=1)
+    big = cirq.Simulator(seed=9).run(meas, repetitions=6000).records["m"][:, 0, :]
+    idx = big[:, 0] * 4 + big[:, 1] * 2 + big[:, 2]
+    freq = np.bincount(idx, minlength=8) / len(idx)
+    assert 0.5 * np.abs(freq - p).sum() < 0.035, (freq, p)
+
+
+def test_batched_density_matrix_mid_circuit(gpu):
+    _log("batched density-matrix trajectories")
+    q0, q1, q2 = cirq.LineQubit.range(3)
+    c = cirq.Circuit(cirq.H(q0), cirq.CNOT(q0, q1), cirq.amplitude_damp(0.2)(q1), cirq.measure(q1, key="a"),
+                     cirq.rx(0.9)(q2), cirq.CNOT(q1, q2), cirq.depolarize(0.1)(q2), cirq.measure(q0, q2, key="b"))
+    rb = cirq.DensityMatrixSimulator(seed=4).run(c, repetitions=400)
+    rs = cirq.DensityMatrixSimulator(seed=4, trajectory_batch=1).run(c, repetitions=400)
+    for key in ("a", "b"):
+        assert _agreement(rb, rs, key) >= 0.99, key
+    a = rb.records["a"][:, 0, 0]
+    assert 0.3 < a.mean() < 0.5  # P(q1 = 1) = 0.5 * (1 - 0.2)
