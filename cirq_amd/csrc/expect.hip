@@ -95,7 +95,8 @@ __global__ __launch_bounds__(64) void k_expect_final(const double* __restrict__ 
 // Terms without X or Y (x mask 0): sum_i (-1)^popcount(i & z) |psi_i|^2.  Each thread takes
 // 16 consecutive amplitudes i = 16g + j, so the sign splits into (-1)^popcount(g & z>>4), the same
 // for the 16, times (-1)^popcount(j & z&15), the same for every thread: one FP64 FMA per
-// amplitude and term.  |psi_i|^2 is exact in FP64 up to its final rounding.
+// amplitude and distinct low mask, then one add per term.  |psi_i|^2 is exact in FP64 up to its
+// final rounding.
 constexpr int kZTerms = 32;
 constexpr int kZBlocks = 2048;  // fixed grid => deterministic reduction order
 struct ZBatch {
@@ -121,14 +122,21 @@ __global__ __launch_bounds__(256) void k_expect_z(const float2* __restrict__ amp
       p[2 * q] = fma((double)v.x, (double)v.x, (double)v.y * (double)v.y);
       p[2 * q + 1] = fma((double)v.z, (double)v.z, (double)v.w * (double)v.w);
     }
+    // The host orders a batch's terms by their low 4 z bits, so the signed 16-term sum is formed
+    // once per run of equal low masks (a uniform branch) and shared by the run's terms.
+    int zl_done = -1;
+    double w = 0.0;
 #pragma unroll
     for (int t = 0; t < kZTerms; ++t) {
       if (t < zb.count) {
         const uint64_t z = zb.z[t];
         const int zl = (int)(z & 15);
-        double w = 0.0;
+        if (zl != zl_done) {
+          w = 0.0;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) w = fma((__popc(j & zl) & 1) ? -1.0 : 1.0, p[j], w);
+          for (int j = 0; j < 16; ++j) w = fma((__popc(j & zl) & 1) ? -1.0 : 1.0, p[j], w);
+          zl_done = zl;
+        }
         acc[t] += (__popcll(g & (z >> 4)) & 1) ? -w : w;
       }
     }
@@ -226,7 +234,11 @@ int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t
   for (int t = 0; t < n_terms; ++t)
     CQS_REQUIRE(!(xmask[t] & ~valid) && !(zmask[t] & ~valid), "term %d has masks beyond %d qubits", t,
                 n_qubits);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return xmask[a] < xmask[b]; });
+  // by X mask (terms sharing one launch); Z-only terms also by their low 4 z bits (k_expect_z)
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (xmask[a] != xmask[b]) return xmask[a] < xmask[b];
+    return xmask[a] == 0 && (zmask[a] & 15) < (zmask[b] & 15);
+  });
   hipStream_t s = as_stream(stream);
   double* partials = (double*)workspace;
   const uint64_t N = 1ull << n_qubits;

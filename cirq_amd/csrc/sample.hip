@@ -154,15 +154,27 @@ __global__ __launch_bounds__(256) void k_sum_buffers(Loader ld, uint64_t len, fl
 // 8192 of them at a time in LDS so that the one adding thread never waits on global memory.
 __global__ __launch_bounds__(256) void k_total32(const float* __restrict__ bsum, uint64_t nb, SampleHdr* hdr) {
   constexpr int kStage = 8192;
-  __shared__ float st[kStage];
+  __shared__ __align__(16) float st[kStage];
   float S = 0.f;
   for (uint64_t c0 = 0; c0 < nb; c0 += kStage) {
     const int cnt = (int)((nb - c0) < (uint64_t)kStage ? (nb - c0) : (uint64_t)kStage);
     __syncthreads();
     for (int i = threadIdx.x; i < cnt; i += 256) st[i] = bsum[c0 + i];
     __syncthreads();
-    if (threadIdx.x == 0)
-      for (int i = 0; i < cnt; ++i) S = S + st[i];
+    if (threadIdx.x == 0) {
+      // the same sequential adds; 16-byte LDS reads issued ahead of the dependent chain
+      int i = 0;
+      const float4* st4 = reinterpret_cast<const float4*>(st);
+#pragma unroll 8
+      for (; i + 4 <= cnt; i += 4) {
+        const float4 v = st4[i >> 2];
+        S = S + v.x;
+        S = S + v.y;
+        S = S + v.z;
+        S = S + v.w;
+      }
+      for (; i < cnt; ++i) S = S + st[i];
+    }
   }
   if (threadIdx.x == 0) hdr->T32 = S;
 }
@@ -363,9 +375,10 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
                                                     const ChunkFn* __restrict__ scfn,
                                                     double* __restrict__ scstart, int* __restrict__ scmode,
                                                     double* __restrict__ cstart) {
-  constexpr int kStage = 1024;  // superchunk maps staged in LDS, so the serial walk reads LDS only
+  constexpr int kStage = 512;  // superchunk maps staged in LDS, so the serial walk reads LDS only
   __shared__ double xs[kChunk];
   __shared__ ChunkFn sf[kStage];
+  __shared__ ChunkFn cf[kSuper];  // the chunk maps of a superchunk walked chunk by chunk
   const float T32 = hdr->T32;
   const int t = threadIdx.x;
   double S = 0.0;
@@ -383,10 +396,14 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
       continue;
     }
     if (t == 0) scmode[sc] = 0;
+    // one batch of loads for the superchunk's maps instead of a dependent load per chunk
+    __syncthreads();
+    for (uint64_t i = t; i < SCl; i += 64) cf[i] = cfn[sc * SCl + i];
+    __syncthreads();
     for (uint64_t k = 0; k < SCl; ++k) {
       const uint64_t c = sc * SCl + k;
       if (t == 0) cstart[c] = S;
-      const ChunkFn f = cfn[c];
+      const ChunkFn f = cf[k];
       if (f.ok && try_apply(S, f.E, f.d0, f.d1)) continue;
       __syncthreads();
       for (uint64_t i = t; i < Cl; i += 64) xs[i] = px(ld, T32, c * Cl + i);
@@ -742,6 +759,7 @@ int cqs_sample_prepare(const void* amps, const float* probs_in, int32_t n_qubits
   hipLaunchKernelGGL(k_chunk_fn, dim3(gch), dim3(256), 0, s, ld, (const SampleHdr*)hdr, l.NC, l.Cl, l.SCl,
                      (const double*)capx, (const double*)cin, (const double*)scpre, cfn);
   hipLaunchKernelGGL(k_super_fn, dim3((unsigned)l.NSC), dim3(256), 0, s, (const ChunkFn*)cfn, l.SCl, scfn);
+  CQS_REQUIRE(l.SCl <= (uint64_t)kSuper, "superchunk of %llu chunks", (unsigned long long)l.SCl);
   hipLaunchKernelGGL(k_scan_serial, dim3(1), dim3(64), 0, s, ld, hdr, l.NSC, l.SCl, l.Cl,
                      (const ChunkFn*)cfn, (const ChunkFn*)scfn, scstart, scmode, cstart);
   hipLaunchKernelGGL(k_chunk_starts, dim3((unsigned)l.NSC), dim3(256), 0, s, (const ChunkFn*)cfn, l.SCl,

@@ -40,7 +40,7 @@ from .tape import (
     fill_param_mats,
     kraus_of,
 )
-from .trajectories import TrajectoryBatch
+from .trajectories import MAX_BATCH_CHANNEL_QUBITS, MAX_KRAUS, TrajectoryBatch, kraus_draws
 
 
 def parse_random_state(seed) -> np.random.RandomState:
@@ -645,14 +645,27 @@ class _SimulatorBase:
         obj = getattr(op, "gate", None) or op
         mix = getattr(obj, "_mixture_", None)
         mixture = mix() if mix is not None else None
+        n = len(qubits)
+        narrow = len(op.qubits) <= MAX_BATCH_CHANNEL_QUBITS  # the one-copy case of the batched kernels
+        idx = {q: i for i, q in enumerate(qubits)}
+        targets = [n - 1 - idx[q] for q in reversed(op.qubits)]
         if mixture is not None and mixture is not NotImplemented:
             probabilities, unitaries = zip(*mixture)
             index = self._prng.choice(range(len(unitaries)), p=probabilities)
-            self._apply_matrix(st, qubits, op.qubits, unitaries[index])
+            if narrow:
+                engine.apply_select_batch(st, n, 1, targets, np.stack([np.asarray(u) for u in unitaries]),
+                                          np.array([index]), np.ones(1, dtype=np.float32))
+            else:
+                self._apply_matrix(st, qubits, op.qubits, unitaries[index])
             return
         kraus = kraus_of(obj)
-        n = len(qubits)
         p = self._prng.random()
+        if narrow and len(kraus) <= MAX_KRAUS:
+            mats = np.stack([np.asarray(K) for K in kraus])
+            W = engine.kraus_weights_batch(st, n, 1, targets, mats, self._ws)
+            sel, w = kraus_draws(W, np.array([p]))
+            engine.apply_select_batch(st, n, 1, targets, mats, sel, np.array([1.0 / math.sqrt(w[0])], dtype=np.float32))
+            return
         fallback_weight, fallback_index = 0.0, 0
         weight, buf = None, None
         for index, K in enumerate(kraus):
