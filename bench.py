@@ -7,7 +7,7 @@ Workload (one step): the 30-qubit random grid circuit on a 5x6 grid, 40 cycles o
 sampling in Cirq's factor order) and <Z_i Z_j> for the 49 nearest-neighbour pairs.
 
 Metric: layers (cycles) per second of the whole step; HBM GB/s of the gate-pass kernel.
-Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N with N > 1 exits: the simulator runs on one GPU.
 """
 
 from __future__ import annotations
@@ -547,7 +547,7 @@ def main():
         if int(os.environ.get("RANK", "0")) == 0:
             print(
                 "bench.py runs on one GPU only: this simulator has no multi-GPU path "
-                "(sharding is described in DESIGN.md)", file=sys.stderr
+                "(a state would shard by its top qubit bits, one process per GPU)", file=sys.stderr
             )
         sys.exit(2)
 
