@@ -648,14 +648,23 @@ __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ a
       it |= 1ull << pt.perm[pt.S[pt.A[k]]];
     }
   const bool act = (int)threadIdx.x < cnt;
-  for (int e = 0; e < (cnt >> tb); ++e) {
+  // element loops unrolled to their compile-time bound (2^(kPermT - 8)) so that the per-element
+  // offsets (uniform) and the loads issue together
+  constexpr int kE = 1 << (kPermT - 8);
+  const int ne = cnt >> tb;
+#pragma unroll
+  for (int e = 0; e < kE; ++e) {
+    if (e >= ne) break;
     int le = 0;
     uint64_t ie = 0;
-    for (int k = tb; k < T; ++k)
-      if ((e >> (k - tb)) & 1) {
+#pragma unroll
+    for (int kk = 0; kk < kPermT - 8; ++kk) {
+      const int k = tb + kk;
+      if (k < T && ((e >> kk) & 1)) {
         le |= 1 << pt.A[k];
         ie |= 1ull << pt.perm[pt.S[pt.A[k]]];
       }
+    }
     if (act) {
       const uint64_t i = base_in | it | ie;
       float v;
@@ -674,10 +683,15 @@ __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ a
   uint64_t ot = base_out;
   for (int b = 0; b < tb; ++b)
     if ((threadIdx.x >> b) & 1) ot |= 1ull << pt.S[b];
-  for (int e = 0; e < (cnt >> tb); ++e) {
+#pragma unroll
+  for (int e = 0; e < kE; ++e) {
+    if (e >= ne) break;
     uint64_t oe = 0;
-    for (int b = tb; b < T; ++b)
-      if ((e >> (b - tb)) & 1) oe |= 1ull << pt.S[b];
+#pragma unroll
+    for (int kk = 0; kk < kPermT - 8; ++kk) {
+      const int b = tb + kk;
+      if (b < T && ((e >> kk) & 1)) oe |= 1ull << pt.S[b];
+    }
     if (act) {
       const int j = (int)threadIdx.x + (e << tb);
       out[ot | oe] = tile[j + (j >> 5)];
