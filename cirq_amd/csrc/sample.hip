@@ -101,6 +101,22 @@ struct Loader {
     }
     return v < 0.f ? 0.f : v;
   }
+  // q(i), ..., q(i + 3) for i a multiple of 4: one 16-byte load of probabilities or two of
+  // amplitudes when the layout allows it, else four single reads.
+  __device__ __forceinline__ float4 q4(uint64_t i) const {
+    if (!permute && mode == 2 && (reinterpret_cast<uintptr_t>(probs + i) & 15) == 0) {
+      const float4 v = *reinterpret_cast<const float4*>(probs + i);
+      return make_float4(v.x < 0.f ? 0.f : v.x, v.y < 0.f ? 0.f : v.y, v.z < 0.f ? 0.f : v.z, v.w < 0.f ? 0.f : v.w);
+    }
+    if (!permute && mode == 0 && (reinterpret_cast<uintptr_t>(amps + i) & 15) == 0) {
+      const float4 a = *reinterpret_cast<const float4*>(amps + i);
+      const float4 b = *reinterpret_cast<const float4*>(amps + i + 2);
+      const float p0 = fmaf(a.x, a.x, a.y * a.y), p1 = fmaf(a.z, a.z, a.w * a.w);
+      const float p2 = fmaf(b.x, b.x, b.y * b.y), p3 = fmaf(b.z, b.z, b.w * b.w);
+      return make_float4(p0 < 0.f ? 0.f : p0, p1 < 0.f ? 0.f : p1, p2 < 0.f ? 0.f : p2, p3 < 0.f ? 0.f : p3);
+    }
+    return make_float4(q(i), q(i + 1), q(i + 2), q(i + 3));
+  }
 };
 
 __device__ __forceinline__ double ldexp_d(double x, int e) { return ldexp(x, e); }
@@ -111,7 +127,18 @@ __device__ __forceinline__ double ldexp_d(double x, int e) { return ldexp(x, e);
 __global__ __launch_bounds__(256) void k_sum_buffers(Loader ld, uint64_t len, float* __restrict__ out) {
   __shared__ float sq[kNpBuf + kNpBuf / 128];
   const uint64_t off = (uint64_t)blockIdx.x * len;
-  for (uint64_t i = threadIdx.x; i < len; i += 256) sq[i + (i >> 7)] = ld.q(off + i);
+  if (len % 4 == 0) {
+    for (uint64_t i = 4 * threadIdx.x; i < len; i += 1024) {
+      const float4 v = ld.q4(off + i);  // i .. i + 3 lie in one 128-element block
+      float* d = sq + i + (i >> 7);
+      d[0] = v.x;
+      d[1] = v.y;
+      d[2] = v.z;
+      d[3] = v.w;
+    }
+  } else {
+    for (uint64_t i = threadIdx.x; i < len; i += 256) sq[i + (i >> 7)] = ld.q(off + i);
+  }
   __syncthreads();
   if (threadIdx.x >= 64) return;
   const int t = threadIdx.x;
@@ -193,8 +220,17 @@ __global__ __launch_bounds__(256) void k_chunk_approx(Loader ld, const SampleHdr
   const int lc = threadIdx.x >> 3, sub = threadIdx.x & 7;
   const uint64_t c = c0 + lc;
   double s = 0.0;
-  if (c < NC)
-    for (uint64_t i = sub; i < Cl; i += 8) s += px(ld, T32, c * Cl + i);
+  if (c < NC) {
+    if (Cl % 4 == 0) {
+      for (uint64_t i = 4 * sub; i < Cl; i += 32) {
+        const float4 v = ld.q4(c * Cl + i);
+        s += (double)__fdiv_rn(v.x, T32) + (double)__fdiv_rn(v.y, T32) + (double)__fdiv_rn(v.z, T32) +
+             (double)__fdiv_rn(v.w, T32);
+      }
+    } else {
+      for (uint64_t i = sub; i < Cl; i += 8) s += px(ld, T32, c * Cl + i);
+    }
+  }
   part[threadIdx.x] = s;
   __syncthreads();
   if (sub == 0 && c < NC) {
@@ -263,9 +299,21 @@ __global__ __launch_bounds__(256) void k_chunk_fn(Loader ld, const SampleHdr* hd
   const float T32 = hdr->T32;
   const uint64_t c0 = (uint64_t)blockIdx.x * kChunksPerWG;
   const uint64_t nloc = (NC - c0) < (uint64_t)kChunksPerWG ? (NC - c0) : (uint64_t)kChunksPerWG;
-  for (uint64_t i = threadIdx.x; i < nloc * Cl; i += 256) {
-    const uint64_t lc = i / Cl, k = i % Cl;
-    sp[lc * (kChunk + 1) + k] = __fdiv_rn(ld.q((c0 + lc) * Cl + k), T32);
+  if (Cl % 4 == 0) {
+    for (uint64_t i = 4 * threadIdx.x; i < nloc * Cl; i += 1024) {
+      const uint64_t lc = i / Cl, k = i % Cl;  // k .. k + 3 lie in one chunk
+      const float4 v = ld.q4(c0 * Cl + i);
+      float* d = sp + lc * (kChunk + 1) + k;
+      d[0] = __fdiv_rn(v.x, T32);
+      d[1] = __fdiv_rn(v.y, T32);
+      d[2] = __fdiv_rn(v.z, T32);
+      d[3] = __fdiv_rn(v.w, T32);
+    }
+  } else {
+    for (uint64_t i = threadIdx.x; i < nloc * Cl; i += 256) {
+      const uint64_t lc = i / Cl, k = i % Cl;
+      sp[lc * (kChunk + 1) + k] = __fdiv_rn(ld.q((c0 + lc) * Cl + k), T32);
+    }
   }
   __syncthreads();
   const int t = threadIdx.x;
@@ -355,17 +403,29 @@ __global__ __launch_bounds__(256) void k_super_fn(const ChunkFn* __restrict__ cf
   }
 }
 
-// Applies map f to exact value S if S lies in binade E and the result stays inside it.
+// Applies map f to exact value S if S lies in binade E and the result stays inside it.  Inside
+// one binade a double is its bit pattern: S in [2^E, 2^(E+1)) exactly when its exponent field is
+// E + 1023 (S is a positive normal double here), the ulp parity is the pattern's lowest bit, and
+// adding d >= 0 ulps (the maps only add probabilities) is adding d to the pattern, exact while the
+// exponent field stays the same.  Integer steps only: the walk that calls this is one serial chain.
 __device__ __forceinline__ bool try_apply(double& S, int E, long long d0, long long d1) {
   if (!(S > 0.0)) return false;
-  const double lo = ldexp_d(1.0, E), hi = ldexp_d(1.0, E + 1);
-  if (!(S >= lo && S < hi)) return false;
-  const long long k = (long long)ldexp_d(S, 52 - E);
-  const long long d = (k & 1) ? d1 : d0;
-  const double r = S + ldexp_d((double)d, E - 52);
-  if (!(r < hi)) return false;
-  S = r;
+  constexpr long long kExp = 0x7ff0000000000000ll;
+  const long long ebits = (long long)(E + 1023) << 52;
+  const long long b = __double_as_longlong(S);
+  if ((b & kExp) != ebits) return false;
+  const long long r = b + ((b & 1) ? d1 : d0);
+  if ((r & kExp) != ebits) return false;
+  S = __longlong_as_double(r);
   return true;
+}
+
+// Reads a map with all its fields in one round trip: the empty asm uses every field, so the loads
+// are issued together instead of one after each test of the walk.
+__device__ __forceinline__ ChunkFn load_map(const ChunkFn* p) {
+  ChunkFn f = *p;
+  asm volatile("" ::"v"(f.d0), "v"(f.d1), "v"(f.E), "v"(f.ok));
+  return f;
 }
 
 // One workgroup of 64 lanes walks the superchunks in order.  All lanes keep identical copies
@@ -390,7 +450,7 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
       __syncthreads();
     }
     if (t == 0) scstart[sc] = S;
-    const ChunkFn F = sf[sc % kStage];
+    const ChunkFn F = load_map(sf + sc % kStage);
     if (F.ok && try_apply(S, F.E, F.d0, F.d1)) {
       if (t == 0) scmode[sc] = 1;
       continue;
@@ -403,7 +463,7 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
     for (uint64_t k = 0; k < SCl; ++k) {
       const uint64_t c = sc * SCl + k;
       if (t == 0) cstart[c] = S;
-      const ChunkFn f = cf[k];
+      const ChunkFn f = load_map(cf + k);
       if (f.ok && try_apply(S, f.E, f.d0, f.d1)) continue;
       __syncthreads();
       for (uint64_t i = t; i < Cl; i += 64) xs[i] = px(ld, T32, c * Cl + i);
