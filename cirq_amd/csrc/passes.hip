@@ -47,7 +47,7 @@ __global__ __launch_bounds__(TileShape<L>::NT) __attribute__((amdgpu_waves_per_e
     float2* __restrict__ psi, TileProg P) {
   extern __shared__ float2 lds[];
   const uint64_t base = deposit_bounded<31>((uint64_t)blockIdx.x, P.opos, P.nouter);
-  pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds);
+  pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds, P.basis);
 }
 
 // amps[i] = ((i & mask) == value) ? amps[i] * s : 0
@@ -108,6 +108,7 @@ bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const 
   }
   P.max_k = ps.max_k;
   P.mscale = 32768.f;
+  P.basis = -1;
   P.ops = ops + ps.op_begin;
   P.mats = mats;
   P.aux = aux;
@@ -178,18 +179,23 @@ static int ensure_pass_lds(int L, size_t shmem) {
   return CQS_OK;
 }
 
-int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
-                   const cqs_op* ops, const void* mats, const int32_t* aux, void* stream) {
+static int run_passes(void* amps, int32_t n_bits, int64_t basis, const cqs_pass* passes, int32_t n_passes,
+                      const cqs_op* ops, const void* mats, const int32_t* aux, void* stream) {
   CQS_REQUIRE(amps != nullptr, "amps is NULL");
   CQS_REQUIRE(n_bits >= 0 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
   CQS_REQUIRE(n_passes >= 0 && (n_passes == 0 || passes != nullptr), "bad pass list");
+  CQS_REQUIRE(basis < 0 || (uint64_t)basis < (1ull << n_bits), "basis state %lld out of range",
+              (long long)basis);
   hipStream_t s = as_stream(stream);
+  int64_t start = basis;  // consumed by the first pass that runs
   for (int i = 0; i < n_passes; ++i) {
     const cqs_pass& ps = passes[i];
     if (ps.op_count <= 0) continue;
     CQS_REQUIRE(ops != nullptr && mats != nullptr, "ops/mats are NULL");
     TileProg P;
     if (!make_prog(P, n_bits, ps, ops, (const float2*)mats, aux)) return CQS_ERR_ARG;
+    P.basis = start;
+    start = -1;
     const uint64_t tiles = 1ull << P.nouter;
     CQS_REQUIRE(tiles <= 0x7fffffffull, "too many tiles (n=%d, L=%d)", n_bits, ps.L);
     const int mk = ps.max_k;
@@ -200,7 +206,20 @@ int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n
                        (float2*)amps, P);
     CQS_HIP(hipGetLastError());
   }
+  if (start >= 0) return cqs_init_basis(amps, n_bits, (uint64_t)start, stream);  // no pass ran
   return CQS_OK;
+}
+
+int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
+                   const cqs_op* ops, const void* mats, const int32_t* aux, void* stream) {
+  return run_passes(amps, n_bits, -1, passes, n_passes, ops, mats, aux, stream);
+}
+
+int cqs_run_passes_from_basis(void* amps, int32_t n_bits, uint64_t basis, const cqs_pass* passes,
+                              int32_t n_passes, const cqs_op* ops, const void* mats, const int32_t* aux,
+                              void* stream) {
+  CQS_REQUIRE(basis < (1ull << 62), "basis state out of range");
+  return run_passes(amps, n_bits, (int64_t)basis, passes, n_passes, ops, mats, aux, stream);
 }
 
 }  // extern "C"

@@ -28,6 +28,7 @@ struct TileProg {
   const cqs_op* ops;
   const float2* mats;
   const int32_t* aux;
+  int64_t basis;  // >= 0: the pass starts from basis state |basis> instead of reading psi
 };
 
 // The part of a pass program the tile interpreter reads (TileProg carries the same fields).
@@ -587,10 +588,12 @@ __device__ __forceinline__ void tile_offsets(const int32_t* pos, uint64_t& off_l
 }
 
 // One workgroup's whole pass: load the tile from HBM (layout lpos) into registers, run the
-// program, store it (layout lpos_out).  `base` is the flat offset of the tile's outer bits.
+// program, store it (layout lpos_out).  `base` is the flat offset of the tile's outer bits.  With
+// basis >= 0 nothing is read: the tile starts as its part of the basis state |basis>.
 template <int L, class Prog>
 __device__ __forceinline__ void pass_tile(float2* __restrict__ psi, const Prog& P, const int32_t* lpos,
-                                          const int32_t* lpos_out, uint64_t base, float2* lds) {
+                                          const int32_t* lpos_out, uint64_t base, float2* lds,
+                                          int64_t basis = -1) {
   constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
   float2* tile = lds;
   float2* mstage = lds + tile_words(L);
@@ -600,8 +603,14 @@ __device__ __forceinline__ void pass_tile(float2* __restrict__ psi, const Prog& 
     uint64_t off_lo, off_hi[PER];
     tile_offsets<L>(lpos, off_lo, off_hi);
     if (act) {
+      if (basis >= 0) {
 #pragma unroll
-      for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
+        for (int e = 0; e < PER; ++e)
+          v[e] = make_float2(base + off_lo + off_hi[e] == (uint64_t)basis ? 1.f : 0.f, 0.f);
+      } else {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
+      }
     }
   }
   run_program<PER, NT>(v, tile, mstage, P, base, act);

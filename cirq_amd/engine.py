@@ -44,7 +44,18 @@ class DeviceProgram:
     def set_mats_device(self, mats_f32: torch.Tensor):
         self.mats = mats_f32
 
-    def run(self, state: torch.Tensor):
+    def run(self, state: torch.Tensor, basis: int | None = None):
+        """Runs the passes on state in place; with basis given, on |basis> instead (the first
+        pass builds it, so state's contents on entry do not matter)."""
+        if basis is not None:
+            check(
+                lib().cqs_run_passes_from_basis(
+                    ptr(state), self.n_bits, int(basis), self.pass_arr.ctypes.data, int(self.packed.n_passes),
+                    ptr(self.ops), ptr(self.mats), ptr(self.aux), stream_handle(),
+                ),
+                "cqs_run_passes_from_basis",
+            )
+            return
         if self.packed.n_passes == 0:
             return
         check(
@@ -62,11 +73,16 @@ class DeviceProgram:
         )
 
 
-def new_state(n_bits: int, device, basis: int = 0) -> torch.Tensor:
+def empty_state(n_bits: int, device) -> torch.Tensor:
+    """Uninitialised storage for 2^n_bits complex64 amplitudes."""
     try:
-        st = torch.empty(1 << n_bits, dtype=torch.complex64, device=device)
+        return torch.empty(1 << n_bits, dtype=torch.complex64, device=device)
     except torch.OutOfMemoryError as e:  # the reference raises MemoryError
         raise MemoryError(f"cannot allocate a state of 2^{n_bits} complex64 amplitudes") from e
+
+
+def new_state(n_bits: int, device, basis: int = 0) -> torch.Tensor:
+    st = empty_state(n_bits, device)
     check(lib().cqs_init_basis(ptr(st), n_bits, int(basis), stream_handle()), "cqs_init_basis")
     return st
 

@@ -289,11 +289,20 @@ class _SimulatorBase:
 
     def _run_program(self, comp: _Compiled, resolvers: list, initial_state) -> Iterator[torch.Tensor]:
         mats = comp.mats_for(resolvers) if comp.has_params else None
+        basis = None
+        if initial_state is None or isinstance(initial_state, (int, np.integer)):
+            basis = self._initial_index(initial_state, comp.n)
+            if self._dm:
+                basis = basis * (2**comp.n) + basis
         for i, _ in enumerate(resolvers):
             if mats is not None:
                 comp.dev.set_mats(mats[i])
-            st = self._prepare(comp, initial_state)
-            comp.dev.run(st)
+            if basis is None:
+                st = self._prepare(comp, initial_state)
+                comp.dev.run(st)
+            else:  # the first pass builds the basis state itself
+                st = engine.empty_state(comp.n_bits, self._dev())
+                comp.dev.run(st, basis=basis)
             yield st
 
     @staticmethod

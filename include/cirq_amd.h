@@ -144,6 +144,17 @@ int cqs_init_basis(void* amps, int32_t n_bits, uint64_t index, void* stream);
 int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
                    const cqs_op* ops, const void* mats, const int32_t* aux, void* stream);
 
+/*
+ * cqs_init_basis(amps, n_bits, basis) followed by cqs_run_passes, with the initial state never
+ * written or read: the first pass that runs builds its tiles of |basis> in registers (amps may
+ * hold anything on entry).  With no pass to run it is cqs_init_basis.  The same initial states as
+ * cirq-core/cirq/sim/state_vector_simulation_state.py:49-76 and
+ * cirq-core/cirq/sim/density_matrix_simulation_state.py:41-76, one HBM write and read fewer.
+ */
+int cqs_run_passes_from_basis(void* amps, int32_t n_bits, uint64_t basis, const cqs_pass* passes,
+                              int32_t n_passes, const cqs_op* ops, const void* mats, const int32_t* aux,
+                              void* stream);
+
 /* ---------------------------------------------------------- adjoint gradient */
 
 /*

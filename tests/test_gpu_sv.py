@@ -305,3 +305,34 @@ def test_unsupported_ops_raise_type_error(gpu):
     # a channel on a state vector is sampled as one trajectory (a normalised state)
     psi = cirq.Simulator(seed=1).simulate(cirq.Circuit(cirq.depolarize(0.1).on(q[0]))).final_state_vector
     assert abs(np.linalg.norm(psi) - 1) < 1e-6
+
+
+@pytest.mark.parametrize("dm", [False, True])
+def test_passes_from_basis_state_match_init_then_passes(gpu, dm):
+    """cqs_run_passes_from_basis builds |b> in the first pass: bit-identical to cqs_init_basis
+    followed by cqs_run_passes, for several tiles and basis states, and with no pass to run."""
+    _log(f"passes from a basis state dm={dm}")
+    from cirq_amd import engine, tape
+
+    rng = np.random.RandomState(8)
+    n = 5 if dm else 11
+    c, qs = _random_circuit(n, 60, rng, with_channels=dm)
+    low = tape.Lowering(qs, dm)
+    low.lower_circuit(c)
+    nb = 2 * n if dm else n
+    dev = torch.device("cuda")
+    for L, co in [(4, 2), (7, 3), (min(nb, 10), 4)]:
+        prog = engine.DeviceProgram(tape.compile_ops(low.ops, nb, L=L, coalesce=co), dev)
+        for b in (0, 1, 37 % 2**n, 2**n - 1):
+            idx = b * 2**n + b if dm else b
+            a = engine.new_state(nb, dev, idx)
+            prog.run(a)
+            x = torch.full((1 << nb,), complex(np.nan, np.nan), dtype=torch.complex64, device=dev)  # never read
+            prog.run(x, basis=idx)
+            assert torch.equal(a, x), (L, b)
+    empty = engine.DeviceProgram(tape.compile_ops([], nb), dev)
+    x = torch.full((1 << nb,), 7.0, dtype=torch.complex64, device=dev)
+    empty.run(x, basis=3)
+    want = torch.zeros(1 << nb, dtype=torch.complex64, device=dev)
+    want[3] = 1
+    assert torch.equal(x, want)
