@@ -214,3 +214,28 @@ def test_batched_density_matrix_mid_circuit(gpu):
         assert _agreement(rb, rs, key) >= 0.99, key
     a = rb.records["a"][:, 0, 0]
     assert 0.3 < a.mean() < 0.5  # P(q1 = 1) = 0.5 * (1 - 0.2)
+
+
+def test_batched_trajectories_parameterised_sweep_and_noise_model(gpu):
+    """Symbols resolved per sweep point in the batched programs; a noise model's channels sampled
+    per repetition.  Both agree with the one-at-a-time path."""
+    _log("batched trajectories: parameter sweep and noise model")
+    import sympy
+
+    t = sympy.Symbol("t")
+    qs = cirq.LineQubit.range(5)
+    c = cirq.Circuit(cirq.rx(t)(q) for q in qs)
+    c.append(cirq.CNOT(qs[i], qs[i + 1]) for i in range(4))
+    c.append(cirq.measure(qs[0], qs[2], key="mid"))
+    c.append(cirq.ry(2 * t)(q) for q in qs)
+    c.append(cirq.measure(*qs, key="end"))
+    sweep = cirq.Points("t", [0.3, 1.7])
+    rb = cirq.Simulator(seed=21).run_sweep(c, sweep, repetitions=200)
+    rs = cirq.Simulator(seed=21, trajectory_batch=1).run_sweep(c, sweep, repetitions=200)
+    for a, b in zip(rb, rs):
+        for key in ("mid", "end"):
+            assert _agreement(a, b, key) >= 0.99, key
+    noisy = cirq.Circuit(cirq.H.on_each(*qs), cirq.CNOT(qs[0], qs[1]), cirq.measure(*qs, key="m"))
+    nb = cirq.Simulator(seed=2, noise=cirq.depolarize(0.05)).run(noisy, repetitions=300)
+    ns = cirq.Simulator(seed=2, noise=cirq.depolarize(0.05), trajectory_batch=1).run(noisy, repetitions=300)
+    assert _agreement(nb, ns, "m") >= 0.98
