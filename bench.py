@@ -93,10 +93,9 @@ def dm_leg(args):
     stream = torch.cuda.current_stream()
 
     def once():
-        engine.check(engine.lib().cqs_init_basis(engine.ptr(rho), 2 * n, 0, engine.stream_handle()), "init")
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        comp.dev.run(rho)
+        comp.dev.run(rho, basis=0)  # |0..0><0..0|, built by the first pass
         e1.record(stream)
         ev = engine.expect_terms(rho, n, True, terms, sim._ws).sum().real.item()
         return e0.elapsed_time(e1), ev
@@ -108,7 +107,7 @@ def dm_leg(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     n_pass = int(comp.packed.n_passes)
-    bytes_per_pass = 2 * 8 * 4**n
+    bytes_per_pass = (2 * n_pass - 1) * 8 * 4**n / max(n_pass, 1)  # the first pass only writes
     del rho
     torch.cuda.empty_cache()
     return {
