@@ -188,50 +188,73 @@ def collapse_batch(states: torch.Tensor, n_bits: int, n_states: int, mask: int, 
           "cqs_collapse_batch")
 
 
-def apply_select_batch(states: torch.Tensor, n_bits: int, n_states: int, targets, mats: np.ndarray,
-                       select: np.ndarray, scales: np.ndarray):
-    """Copy r <- scales[r] * mats[select[r]] on `targets` (matrix index bit m <-> flat bit targets[m])."""
-    _check_copies(states, n_bits, n_states)
+def device_mats(mats: np.ndarray, targets, device) -> tuple[torch.Tensor, int]:
+    """A channel's 2^k x 2^k matrices (k = len(targets)) as a device complex64 buffer, and their count."""
     mats = np.asarray(mats, dtype=np.complex64)
     n_mats = int(mats.shape[0])
     d = 1 << len(targets)
     if mats.shape != (n_mats, d, d):
         raise ValueError(f"matrices of shape {mats.shape} for {len(targets)} target bits")
+    return _dev_array(mats.view(np.float32), device), n_mats
+
+
+def apply_select_batch_dev(states: torch.Tensor, n_bits: int, n_states: int, targets, mats_dev: torch.Tensor,
+                           n_mats: int, select_dev: torch.Tensor, scales_dev: torch.Tensor):
+    """Copy r <- scales[r] * mats[select[r]] on `targets` (matrix index bit m <-> flat bit
+    targets[m]); matrices, selections (int32) and scales (float32) already on the device.  A copy
+    whose selection is out of range is left unchanged."""
+    _check_copies(states, n_bits, n_states)
+    d = 1 << len(targets)
+    if mats_dev.dtype != torch.float32 or mats_dev.numel() != n_mats * d * d * 2:
+        raise ValueError("device matrices do not match n_mats and the targets")
+    if select_dev.dtype != torch.int32 or select_dev.numel() < n_states:
+        raise ValueError("one int32 selection per copy")
+    if scales_dev.dtype != torch.float32 or scales_dev.numel() < n_states:
+        raise ValueError("one float32 scale per copy")
+    t = np.array(targets, dtype=np.int32)
+    check(
+        lib().cqs_apply_select_batch(ptr(states), n_bits, n_states, len(t), t.ctypes.data, ptr(mats_dev), n_mats,
+                                     ptr(select_dev), ptr(scales_dev), stream_handle()),
+        "cqs_apply_select_batch",
+    )
+
+
+def apply_select_batch(states: torch.Tensor, n_bits: int, n_states: int, targets, mats: np.ndarray,
+                       select: np.ndarray, scales: np.ndarray):
+    """apply_select_batch_dev with host arrays (uploaded here)."""
+    m, n_mats = device_mats(mats, targets, states.device)
     sel = np.asarray(select, dtype=np.int32)
     if sel.shape != (n_states,) or np.shape(scales) != (n_states,):
         raise ValueError("one selection and one scale per copy")
     if sel.min() < 0 or sel.max() >= n_mats:
         raise ValueError("selection outside the channel's matrices")
+    apply_select_batch_dev(states, n_bits, n_states, targets, m, n_mats, _dev_array(sel, states.device),
+                           _dev_array(np.asarray(scales, dtype=np.float32), states.device))
+
+
+def kraus_weights_batch_dev(states: torch.Tensor, n_bits: int, n_states: int, targets, mats_dev: torch.Tensor,
+                            n_mats: int, ws: Workspace) -> np.ndarray:
+    """||K_i psi_r||^2 in FP64 for every copy r and Kraus operator i (host [n_states, n_mats])."""
+    _check_copies(states, n_bits, n_states)
+    d = 1 << len(targets)
+    if mats_dev.dtype != torch.float32 or mats_dev.numel() != n_mats * d * d * 2:
+        raise ValueError("device matrices do not match n_mats and the targets")
     t = np.array(targets, dtype=np.int32)
-    m = _dev_array(mats.view(np.float32), states.device)
-    sd = _dev_array(sel, states.device)
-    sc = _dev_array(np.asarray(scales, dtype=np.float32), states.device)
+    out = torch.empty((n_states, n_mats), dtype=torch.float64, device=states.device)
+    w = ws.get(lib().cqs_kraus_workspace_bytes(n_bits, n_states, len(t), n_mats))
     check(
-        lib().cqs_apply_select_batch(ptr(states), n_bits, n_states, len(t), t.ctypes.data, ptr(m), n_mats, ptr(sd),
-                                     ptr(sc), stream_handle()),
-        "cqs_apply_select_batch",
+        lib().cqs_kraus_weights_batch(ptr(states), n_bits, n_states, len(t), t.ctypes.data, ptr(mats_dev), n_mats,
+                                      ptr(out), ptr(w), stream_handle()),
+        "cqs_kraus_weights_batch",
     )
+    return out.cpu().numpy()
 
 
 def kraus_weights_batch(states: torch.Tensor, n_bits: int, n_states: int, targets, mats: np.ndarray,
                         ws: Workspace) -> np.ndarray:
-    """||K_i psi_r||^2 in FP64 for every copy r and Kraus operator i (host [n_states, n_mats])."""
-    _check_copies(states, n_bits, n_states)
-    mats = np.asarray(mats, dtype=np.complex64)
-    n_mats = int(mats.shape[0])
-    d = 1 << len(targets)
-    if mats.shape != (n_mats, d, d):
-        raise ValueError(f"matrices of shape {mats.shape} for {len(targets)} target bits")
-    t = np.array(targets, dtype=np.int32)
-    m = _dev_array(mats.view(np.float32), states.device)
-    out = torch.empty((n_states, n_mats), dtype=torch.float64, device=states.device)
-    w = ws.get(lib().cqs_kraus_workspace_bytes(n_bits, n_states, len(t), n_mats))
-    check(
-        lib().cqs_kraus_weights_batch(ptr(states), n_bits, n_states, len(t), t.ctypes.data, ptr(m), n_mats, ptr(out),
-                                      ptr(w), stream_handle()),
-        "cqs_kraus_weights_batch",
-    )
-    return out.cpu().numpy()
+    """kraus_weights_batch_dev with host matrices (uploaded here)."""
+    m, n_mats = device_mats(mats, targets, states.device)
+    return kraus_weights_batch_dev(states, n_bits, n_states, targets, m, n_mats, ws)
 
 
 class Sampler:

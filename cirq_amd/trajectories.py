@@ -107,6 +107,7 @@ class TrajectoryBatch:
         self.index = {q: i for i, q in enumerate(self.qubits)}
         self._programs: dict = {}
         self.n_draws = sum(1 for kind, _ in actions if kind == "op")
+        self._channels: dict = {}  # stochastic op index -> its matrices on the device
 
     @staticmethod
     def supported(actions, dm: bool) -> bool:
@@ -165,10 +166,45 @@ class TrajectoryBatch:
                     out[start + r].append((key, bits[r]))
         return out
 
+    def _channel_of(self, j: int, op):
+        """(targets, device matrices, count, mixture probabilities or None) of stochastic op j."""
+        hit = self._channels.get(j)
+        if hit is None:
+            targets = [self.n - 1 - self.index[q] for q in reversed(op.qubits)]
+            obj = getattr(op, "gate", None) or op
+            mix = getattr(obj, "_mixture_", None)
+            mixture = mix() if mix is not None else None
+            if mixture is not None and mixture is not NotImplemented:
+                probabilities, mats = zip(*mixture)
+                probs = np.asarray(probabilities, dtype=np.float64)
+            else:
+                mats, probs = kraus_of(obj), None
+            mats = np.stack([np.asarray(x, dtype=np.complex128) for x in mats])
+            m_dev, n_mats = engine.device_mats(mats, targets, self.device)
+            hit = (targets, m_dev, n_mats, probs)
+            self._channels[j] = hit
+        return hit
+
     def _run_batch(self, state0: torch.Tensor, resolver, R: int, u: np.ndarray) -> list:
         copy_bits = R.bit_length() - 1
-        nb = self.n_bits
         states = state0.repeat(R)
+        # a mixture's draw does not depend on the state: every copy's choice for every mixture is
+        # known now and goes to the device in one copy
+        stoch = [a for kind, a in self.actions if kind == "op"]
+        mix_rows = {}
+        for j, a in enumerate(stoch):
+            if not _is_measurement(a) and not self.dm:
+                targets, _, n_mats, probs = self._channel_of(j, a)
+                if probs is not None:
+                    mix_rows[j] = len(mix_rows)
+        sel_dev = None
+        if mix_rows:
+            table = np.empty((len(mix_rows), R), dtype=np.int32)
+            for j, row in mix_rows.items():
+                probs = self._channel_of(j, stoch[j])[3]
+                table[row] = choice_rows(np.broadcast_to(probs, (R, len(probs))), u[:, j])
+            sel_dev = engine._dev_array(table, self.device)
+        ones = torch.ones(R, dtype=torch.float32, device=self.device)
         recs: list = []
         j = 0
         for kind, a in self.actions:
@@ -180,8 +216,12 @@ class TrajectoryBatch:
             elif kind == "op":
                 if _is_measurement(a):
                     recs.append(self._measure(states, R, a, u[:, j]))
+                elif j in mix_rows:
+                    targets, m_dev, n_mats, _ = self._channel_of(j, a)
+                    engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, sel_dev[mix_rows[j]],
+                                                  ones)
                 else:
-                    self._channel(states, R, a, u[:, j])
+                    self._kraus(states, R, j, a, u[:, j])
                 j += 1
         return recs
 
@@ -214,24 +254,15 @@ class TrajectoryBatch:
             inv[i] = 1 if x else 0
         return m.key, bits ^ inv[None, :]
 
-    def _channel(self, states, R: int, op, u: np.ndarray):
-        targets = [self.n - 1 - self.index[q] for q in reversed(op.qubits)]
-        obj = getattr(op, "gate", None) or op
-        mix = getattr(obj, "_mixture_", None)
-        mixture = mix() if mix is not None else None
-        if mixture is not None and mixture is not NotImplemented:
-            probabilities, unitaries = zip(*mixture)
-            p = np.broadcast_to(np.asarray(probabilities, dtype=np.float64), (R, len(probabilities)))
-            sel = choice_rows(p, u)
-            mats = np.stack([np.asarray(x, dtype=np.complex128) for x in unitaries])
-            engine.apply_select_batch(states, self.n_bits, R, targets, mats, sel, np.ones(R, dtype=np.float32))
-            return
-        kraus = np.stack([np.asarray(K, dtype=np.complex128) for K in kraus_of(obj)])
-        W = engine.kraus_weights_batch(states, self.n_bits, R, targets, kraus, self.ws)
+    def _kraus(self, states, R: int, j: int, op, u: np.ndarray):
+        targets, m_dev, n_mats, _ = self._channel_of(j, op)
+        W = engine.kraus_weights_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, self.ws)
         sel, w = kraus_draws(W, u)
         with np.errstate(divide="ignore"):
-            scales = np.array([1.0 / math.sqrt(x) if x > 0 else np.inf for x in w])
-        engine.apply_select_batch(states, self.n_bits, R, targets, kraus, sel, scales.astype(np.float32))
+            scales = np.array([1.0 / math.sqrt(x) if x > 0 else np.inf for x in w], dtype=np.float32)
+        engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats,
+                                      engine._dev_array(sel.astype(np.int32), self.device),
+                                      engine._dev_array(scales, self.device))
 
 
 __all__ = ["TrajectoryBatch", "choice_rows", "kraus_draws", "measurement_draws"]
