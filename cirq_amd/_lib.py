@@ -119,6 +119,8 @@ _SIGNATURES = {
     "cqs_sample_draw": ([_vp, _vp, _i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp], ctypes.c_int),
     "cqs_marginal_probs": ([_vp, _i32, _i32, _vp, _i32, _vp, _vp], ctypes.c_int),
     "cqs_marginal_probs_batch": ([_vp, _i32, _i32, _i32, _vp, _i32, _vp, _vp], ctypes.c_int),
+    "cqs_draw_rows_workspace_bytes": ([_i32, _i32], _sz),
+    "cqs_draw_rows": ([_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cqs_collapse_batch": ([_vp, _i32, _i32, _u64, _vp, _vp, _vp], ctypes.c_int),
     "cqs_apply_select_batch": ([_vp, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _vp], ctypes.c_int),
     "cqs_kraus_workspace_bytes": ([_i32, _i32, _i32, _i32], _sz),

@@ -759,6 +759,40 @@ __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ a
   }
 }
 
+// ---------------------------------------------------------------- one draw per row
+// Row r of probs (n_rows rows of 2^m float32 marginals, one per copy of a batched trajectory):
+// p = clip(row, 0) / T32_r with T32_r NumPy's float32 sum of the row, then choice(2^m, p) for the
+// uniform u[r]: the float64 sequential cumsum, normalised by its last entry, searchsorted right.
+// bsum holds the row's NumPy buffer sums (k_sum_buffers over buffers of min(2^m, 8192)); the
+// row total adds them one after the other in float32, as NumPy's reduce loop does.
+__global__ __launch_bounds__(64) void k_draw_rows(const float* __restrict__ probs, int m, int n_rows,
+                                                  const float* __restrict__ bsum, int bufs_per_row,
+                                                  const double* __restrict__ uni, int64_t* __restrict__ out_index,
+                                                  float* __restrict__ out_p) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= n_rows) return;
+  float T32 = 0.f;
+  for (int b = 0; b < bufs_per_row; ++b) T32 = T32 + bsum[(size_t)r * bufs_per_row + b];
+  const uint64_t len = 1ull << m;
+  const float* row = probs + (size_t)r * len;
+  auto pr = [&](uint64_t i) {
+    const float q = row[i];
+    return __fdiv_rn(q < 0.f ? 0.f : q, T32);
+  };
+  double total = 0.0;
+  for (uint64_t i = 0; i < len; ++i) total = total + (double)pr(i);
+  const double u = uni[r];
+  double S = 0.0;
+  uint64_t idx = 0;
+  for (; idx < len; ++idx) {  // count of i with cdf_i / total <= u (cdf is non-decreasing)
+    S = S + (double)pr(idx);
+    if (!(S / total <= u)) break;
+  }
+  if (idx >= len) idx = len - 1;  // u < 1 and cdf[-1] / total == 1 make this unreachable
+  out_index[r] = (int64_t)idx;
+  out_p[r] = pr(idx);
+}
+
 }  // namespace cqs
 
 using namespace cqs;
@@ -969,6 +1003,34 @@ int cqs_marginal_probs_batch(const void* amps, int32_t n_qubits, int32_t is_dm, 
     hipLaunchKernelGGL(k_marginal, dim3((unsigned)rows, n_states), dim3(256), 0, as_stream(stream), ld, (int)n_qubits,
                        bits, (int)m, out, stride);
   }
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+
+size_t cqs_draw_rows_workspace_bytes(int32_t m, int32_t n_rows) {
+  if (m < 0 || m > 30 || n_rows < 1) return 0;
+  const uint64_t len = 1ull << m;
+  const uint64_t per = len >= (uint64_t)kNpBuf ? len / kNpBuf : 1;
+  return (size_t)(per * (uint64_t)n_rows * sizeof(float));
+}
+
+int cqs_draw_rows(const float* probs, int32_t m, int32_t n_rows, const double* uniforms, int64_t* out_index,
+                  float* out_p, void* workspace, void* stream) {
+  CQS_REQUIRE(probs != nullptr && uniforms != nullptr && out_index != nullptr && out_p != nullptr &&
+                  workspace != nullptr, "NULL pointer");
+  CQS_REQUIRE(m >= 0 && m <= 30, "m=%d out of range", m);
+  CQS_REQUIRE(n_rows >= 1, "n_rows=%d", n_rows);
+  const uint64_t len = 1ull << m;
+  const uint64_t buflen = len >= (uint64_t)kNpBuf ? (uint64_t)kNpBuf : len;
+  const uint64_t nbuf = (len / buflen) * (uint64_t)n_rows;
+  CQS_REQUIRE(nbuf <= 0x7fffffffull, "too many buffers");
+  hipStream_t s = as_stream(stream);
+  const Loader ld = make_loader(nullptr, probs, 0, 0);
+  float* bsum = (float*)workspace;
+  hipLaunchKernelGGL(k_sum_buffers, dim3((unsigned)nbuf), dim3(256), 0, s, ld, buflen, bsum);
+  hipLaunchKernelGGL(k_draw_rows, dim3((unsigned)((n_rows + 63) / 64)), dim3(64), 0, s, probs, (int)m, (int)n_rows,
+                     (const float*)bsum, (int)(len / buflen), uniforms, out_index, out_p);
   CQS_HIP(hipGetLastError());
   return CQS_OK;
 }

@@ -177,6 +177,21 @@ def marginal_batch(states: torch.Tensor, n_qubits: int, is_dm: bool, n_states: i
     return out
 
 
+def draw_rows(probs: torch.Tensor, uniforms: np.ndarray, ws: Workspace) -> tuple[np.ndarray, np.ndarray]:
+    """NumPy's choice for every row of a device float32 [rows, 2^m] marginal, one uniform per row:
+    (outcome index int64[rows], its normalised float32 probability[rows])."""
+    rows, width = probs.shape
+    m = int(width).bit_length() - 1
+    if probs.dtype != torch.float32 or not probs.is_contiguous() or width != 1 << m or len(uniforms) != rows:
+        raise ValueError("need a contiguous float32 [rows, 2^m] marginal and one uniform per row")
+    u = _dev_array(np.asarray(uniforms, dtype=np.float64), probs.device)
+    idx = torch.empty(rows, dtype=torch.int64, device=probs.device)
+    p = torch.empty(rows, dtype=torch.float32, device=probs.device)
+    w = ws.get(lib().cqs_draw_rows_workspace_bytes(m, rows))
+    check(lib().cqs_draw_rows(ptr(probs), m, rows, ptr(u), ptr(idx), ptr(p), ptr(w), stream_handle()), "cqs_draw_rows")
+    return idx.cpu().numpy(), p.cpu().numpy()
+
+
 def collapse_batch(states: torch.Tensor, n_bits: int, n_states: int, mask: int, values: np.ndarray,
                    scales: np.ndarray):
     _check_copies(states, n_bits, n_states)

@@ -6,7 +6,7 @@ Cirq repeats such a circuit's general suffix once per repetition on a copy of th
 state stored one after the other, i.e. one state of n_bits + log2(R) bits whose top bits number
 the copy.  A deterministic stretch of the suffix is one pass program over all of them (no op
 touches the copy bits); a measurement, mixture or Kraus channel is one batched kernel that gives
-every copy its own outcome (`cqs_marginal_probs_batch`, `cqs_collapse_batch`,
+every copy its own outcome (`cqs_marginal_probs_batch`, `cqs_draw_rows`, `cqs_collapse_batch`,
 `cqs_apply_select_batch`, `cqs_kraus_weights_batch`).
 
 Every stochastic op consumes exactly one uniform of the simulator's RandomState per repetition,
@@ -14,9 +14,10 @@ as in Cirq: `prng.choice(len(p), p=p)` for a measurement
 (cirq-core/cirq/sim/state_vector.py:235-322) and for a mixture
 (cirq-core/cirq/sim/state_vector_simulation_state.py:183-203), `prng.random()` for a Kraus
 channel (:205-257).  The uniforms are drawn up front in Cirq's order (repetition by repetition,
-op by op within one) and each copy uses its own row, with NumPy's choice arithmetic restated
-below; so the records equal those of the one-copy-at-a-time loop except where a probability
-computed by another kernel rounds differently at a draw's boundary.
+op by op within one) and each copy uses its own row, with NumPy's choice arithmetic restated: on
+the GPU for measurements (`cqs_draw_rows`), below for mixtures (whose probabilities are the same
+for every copy); so the records equal those of the one-copy-at-a-time loop except where a
+probability computed by another kernel rounds differently at a draw's boundary.
 """
 
 from __future__ import annotations
@@ -40,15 +41,6 @@ def choice_rows(p: np.ndarray, u: np.ndarray) -> np.ndarray:
     cdf = np.cumsum(np.asarray(p, dtype=np.float64), axis=-1)
     cdf = cdf / cdf[..., -1:]
     return np.sum(cdf <= np.asarray(u, dtype=np.float64)[:, None], axis=-1).astype(np.int64)
-
-
-def measurement_draws(probs32: np.ndarray, u: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    """Outcome index per row and its normalised float32 probability, as the one-state path
-    computes them: p = clip(marginal, 0) / its float32 sum, then choice(len(p), p=p)."""
-    p = np.clip(probs32.astype(np.float32), 0, None)
-    p = p / p.sum(axis=1, keepdims=True, dtype=np.float32)
-    idx = choice_rows(p, u)
-    return idx, p[np.arange(len(idx)), idx]
 
 
 def kraus_draws(weights: np.ndarray, u: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
@@ -230,8 +222,8 @@ class TrajectoryBatch:
         m = Measurement(g.key, tuple(op.qubits), g.full_invert_mask())
         axes = [self.index[q] for q in m.qubits]
         k = len(axes)
-        probs = engine.marginal_batch(states, self.n, self.dm, R, axes).cpu().numpy()
-        idx, p_sel = measurement_draws(probs, u)
+        probs = engine.marginal_batch(states, self.n, self.dm, R, axes)
+        idx, p_sel = engine.draw_rows(probs, u, self.ws)
         bits = ((idx[:, None] >> (k - 1 - np.arange(k))[None, :]) & 1).astype(np.uint8)
         mask = 0
         values = np.zeros(R, dtype=np.uint64)
@@ -265,4 +257,4 @@ class TrajectoryBatch:
                                       engine._dev_array(scales, self.device))
 
 
-__all__ = ["TrajectoryBatch", "choice_rows", "kraus_draws", "measurement_draws"]
+__all__ = ["TrajectoryBatch", "choice_rows", "kraus_draws"]

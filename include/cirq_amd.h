@@ -344,6 +344,17 @@ int cqs_marginal_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
 int cqs_marginal_probs_batch(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_states,
                              const int32_t* qubit_axes, int32_t m, float* out, void* stream);
 
+/* One draw per row of probs (DEVICE float32, n_rows rows of 2^m unnormalised marginals, e.g.
+ * from cqs_marginal_probs_batch), each as NumPy's RandomState.choice(2^m, p) makes it for the
+ * row's uniform (DEVICE double[n_rows]): p = clip(row, 0) / its NumPy float32 sum, the float64
+ * sequential cumsum normalised by its last entry, searchsorted right
+ * (cirq-core/cirq/sim/state_vector.py:235-322 draws one measurement this way).  out_index: DEVICE
+ * int64[n_rows]; out_p: DEVICE float32[n_rows], the drawn outcome's normalised probability.
+ * workspace: DEVICE, cqs_draw_rows_workspace_bytes bytes. */
+size_t cqs_draw_rows_workspace_bytes(int32_t m, int32_t n_rows);
+int cqs_draw_rows(const float* probs, int32_t m, int32_t n_rows, const double* uniforms, int64_t* out_index,
+                  float* out_p, void* workspace, void* stream);
+
 /* Copy r: amps_r[i] = ((i & mask) == values[r]) ? amps_r[i] * scales[r] : 0 (each copy's
  * measurement outcome projected and renormalised, as cqs_collapse does for one state). */
 int cqs_collapse_batch(void* amps, int32_t n_bits, int32_t n_states, uint64_t mask, const uint64_t* values,

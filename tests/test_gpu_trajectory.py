@@ -239,3 +239,29 @@ def test_batched_trajectories_parameterised_sweep_and_noise_model(gpu):
     nb = cirq.Simulator(seed=2, noise=cirq.depolarize(0.05)).run(noisy, repetitions=300)
     ns = cirq.Simulator(seed=2, noise=cirq.depolarize(0.05), trajectory_batch=1).run(noisy, repetitions=300)
     assert _agreement(nb, ns, "m") >= 0.98
+
+
+@pytest.mark.parametrize("m", [1, 3, 13, 14, 16])
+def test_draw_rows_bit_exact_with_numpy_choice(gpu, m):
+    """cqs_draw_rows: per row, clip, divide by NumPy's float32 sum (one pairwise sum per 8192
+    elements, added in order), then RandomState.choice with the row's uniform."""
+    _log(f"draw rows m={m}")
+    import torch
+
+    from cirq_amd import engine
+
+    rng = np.random.RandomState(m)
+    rows = 40 if m < 16 else 12
+    q = (rng.exponential(size=(rows, 2**m)) * rng.uniform(0.1, 3.0, size=(rows, 1))).astype(np.float32)
+    q[rng.uniform(size=q.shape) < 0.1] = 0
+    q[rng.uniform(size=q.shape) < 0.01] = -1e-9  # clipped to zero, as np.clip does
+    q[:, -1] = np.abs(q[:, -1]) + np.float32(1e-3)  # no all-zero row
+    u = np.random.RandomState(100 + m).random_sample(rows)
+    ws = engine.Workspace(torch.device("cuda"))
+    idx, p = engine.draw_rows(torch.from_numpy(q).cuda(), u, ws)
+    a = np.random.RandomState(100 + m)
+    for r in range(rows):
+        pr = np.clip(q[r], 0, None)
+        pr /= pr.sum()
+        want = int(a.choice(len(pr), p=pr))
+        assert idx[r] == want and p[r] == pr[want], (r, idx[r], want)

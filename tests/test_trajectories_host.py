@@ -7,7 +7,7 @@ Kraus walk of cirq-core/cirq/sim/state_vector_simulation_state.py:224-250) would
 import numpy as np
 import pytest
 
-from cirq_amd.trajectories import TrajectoryBatch, choice_rows, kraus_draws, measurement_draws
+from cirq_amd.trajectories import TrajectoryBatch, choice_rows, kraus_draws
 
 
 @pytest.mark.parametrize("seed", range(5))
@@ -24,27 +24,6 @@ def test_choice_rows_matches_randomstate_choice(seed):
     got = choice_rows(p, b.random_sample(R))
     assert np.array_equal(got, want)
     assert a.random_sample() == b.random_sample()  # the same number of draws was consumed
-
-
-@pytest.mark.parametrize("seed", range(4))
-def test_measurement_draws_match_one_state_recipe(seed):
-    """The one-state path: float32 marginal, clip, divide by its float32 sum, choice."""
-    rng = np.random.RandomState(seed)
-    R, m = 200, 1 + seed
-    probs = rng.exponential(size=(R, 2**m)).astype(np.float32) * np.float32(0.37)
-    probs[rng.uniform(size=probs.shape) < 0.15] = 0
-    probs[:, -1] += np.float32(1e-3)
-    a = np.random.RandomState(7 + seed)
-    want_idx, want_p = [], []
-    for row in probs:
-        p = np.clip(row.astype(np.float32), 0, None)
-        p /= p.sum()
-        r = int(a.choice(len(p), p=p))
-        want_idx.append(r)
-        want_p.append(p[r])
-    idx, psel = measurement_draws(probs, np.random.RandomState(7 + seed).random_sample(R))
-    assert np.array_equal(idx, want_idx)
-    assert np.array_equal(psel, np.array(want_p, dtype=np.float32))
 
 
 def _kraus_walk(weights_row, p):
