@@ -144,6 +144,38 @@ __global__ void k_kraus_finish(const double* __restrict__ partial, int nblk, int
   out[e] = v;
 }
 
+// Copy r's Kraus operator, as the walk of cirq-core/cirq/sim/state_vector_simulation_state.py:224-250
+// picks it from p = u[r]: p -= w_i in order until p < 0; the heaviest operator seen so far (the
+// first of equals) when the walk ends with p >= 0 or on a zero weight.  select[r] and
+// scale[r] = 1 / sqrt(w) feed k_select_batch.
+__global__ void k_kraus_choose(const double* __restrict__ weights, const double* __restrict__ u, int n_states,
+                               int n_mats, int32_t* __restrict__ select, float* __restrict__ scale) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_states) return;
+  const double* w = weights + (size_t)r * n_mats;
+  double p = u[r], fb_w = 0.0, last = 0.0;
+  int fb = 0, sel = -1;
+  for (int i = 0; i < n_mats; ++i) {
+    last = w[i];
+    if (last > fb_w) {
+      fb = i;
+      fb_w = last;
+    }
+    p -= last;
+    if (p < 0) {
+      sel = i;
+      break;
+    }
+  }
+  double wsel = last;
+  if (sel < 0 || last == 0.0) {
+    sel = fb;
+    wsel = fb_w;
+  }
+  select[r] = sel;
+  scale[r] = (float)(1.0 / sqrt(wsel));
+}
+
 // Workgroups per copy: enough for about 4096 workgroups in all, at most one per 256 groups.
 static int blocks_per_copy(uint64_t groups, int n_states) {
   uint64_t want = (groups + 255) / 256;
@@ -246,6 +278,17 @@ int cqs_kraus_weights_batch(const void* amps, int32_t n_bits, int32_t n_states, 
   const int total = n_states * n_mats;
   hipLaunchKernelGGL(k_kraus_finish, dim3((total + 255) / 256), dim3(256), 0, s, (const double*)partial, gx, n_mats,
                      (int)n_states, out);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+
+int cqs_kraus_choose_batch(const double* weights, const double* uniforms, int32_t n_states, int32_t n_mats,
+                           int32_t* select, float* scales, void* stream) {
+  CQS_REQUIRE(weights != nullptr && uniforms != nullptr && select != nullptr && scales != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_states >= 1 && n_mats >= 1 && n_mats <= kMaxKraus, "bad sizes");
+  hipLaunchKernelGGL(k_kraus_choose, dim3((n_states + 255) / 256), dim3(256), 0, as_stream(stream), weights, uniforms,
+                     (int)n_states, (int)n_mats, select, scales);
   CQS_HIP(hipGetLastError());
   return CQS_OK;
 }

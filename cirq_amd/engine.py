@@ -248,8 +248,9 @@ def apply_select_batch(states: torch.Tensor, n_bits: int, n_states: int, targets
 
 
 def kraus_weights_batch_dev(states: torch.Tensor, n_bits: int, n_states: int, targets, mats_dev: torch.Tensor,
-                            n_mats: int, ws: Workspace) -> np.ndarray:
-    """||K_i psi_r||^2 in FP64 for every copy r and Kraus operator i (host [n_states, n_mats])."""
+                            n_mats: int, ws: Workspace, to_host: bool = True):
+    """||K_i psi_r||^2 in FP64 for every copy r and Kraus operator i ([n_states, n_mats]; a host
+    array, or with to_host=False the device tensor, without a synchronisation)."""
     _check_copies(states, n_bits, n_states)
     d = 1 << len(targets)
     if mats_dev.dtype != torch.float32 or mats_dev.numel() != n_mats * d * d * 2:
@@ -262,7 +263,22 @@ def kraus_weights_batch_dev(states: torch.Tensor, n_bits: int, n_states: int, ta
                                       ptr(out), ptr(w), stream_handle()),
         "cqs_kraus_weights_batch",
     )
-    return out.cpu().numpy()
+    return out.cpu().numpy() if to_host else out
+
+
+def kraus_choose_batch(weights_dev: torch.Tensor, uniforms_dev: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """Per copy, the Kraus operator the channel walk picks for its uniform and 1/sqrt(its weight),
+    on the device (int32 selections, float32 scales)."""
+    n_states, n_mats = weights_dev.shape
+    if weights_dev.dtype != torch.float64 or not weights_dev.is_contiguous():
+        raise ValueError("weights must be a contiguous float64 [copies, operators] tensor")
+    if uniforms_dev.dtype != torch.float64 or not uniforms_dev.is_contiguous() or uniforms_dev.numel() < n_states:
+        raise ValueError("one contiguous float64 uniform per copy")
+    sel = torch.empty(n_states, dtype=torch.int32, device=weights_dev.device)
+    scl = torch.empty(n_states, dtype=torch.float32, device=weights_dev.device)
+    check(lib().cqs_kraus_choose_batch(ptr(weights_dev), ptr(uniforms_dev), n_states, n_mats, ptr(sel), ptr(scl),
+                                       stream_handle()), "cqs_kraus_choose_batch")
+    return sel, scl
 
 
 def kraus_weights_batch(states: torch.Tensor, n_bits: int, n_states: int, targets, mats: np.ndarray,

@@ -22,7 +22,6 @@ probability computed by another kernel rounds differently at a draw's boundary.
 
 from __future__ import annotations
 
-import math
 
 import numpy as np
 import torch
@@ -197,6 +196,7 @@ class TrajectoryBatch:
                 table[row] = choice_rows(np.broadcast_to(probs, (R, len(probs))), u[:, j])
             sel_dev = engine._dev_array(table, self.device)
         ones = torch.ones(R, dtype=torch.float32, device=self.device)
+        u_dev = engine._dev_array(np.ascontiguousarray(u.T), self.device) if u.shape[1] else None  # row j: op j
         recs: list = []
         j = 0
         for kind, a in self.actions:
@@ -213,7 +213,7 @@ class TrajectoryBatch:
                     engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, sel_dev[mix_rows[j]],
                                                   ones)
                 else:
-                    self._kraus(states, R, j, a, u[:, j])
+                    self._kraus(states, R, j, a, u_dev[j])
                 j += 1
         return recs
 
@@ -246,15 +246,12 @@ class TrajectoryBatch:
             inv[i] = 1 if x else 0
         return m.key, bits ^ inv[None, :]
 
-    def _kraus(self, states, R: int, j: int, op, u: np.ndarray):
+    def _kraus(self, states, R: int, j: int, op, u_dev: torch.Tensor):
+        """Weights, the walk and the chosen operator, all on the device: no round trip to the host."""
         targets, m_dev, n_mats, _ = self._channel_of(j, op)
-        W = engine.kraus_weights_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, self.ws)
-        sel, w = kraus_draws(W, u)
-        with np.errstate(divide="ignore"):
-            scales = np.array([1.0 / math.sqrt(x) if x > 0 else np.inf for x in w], dtype=np.float32)
-        engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats,
-                                      engine._dev_array(sel.astype(np.int32), self.device),
-                                      engine._dev_array(scales, self.device))
+        W = engine.kraus_weights_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, self.ws, to_host=False)
+        sel, scl = engine.kraus_choose_batch(W, u_dev)
+        engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, sel, scl)
 
 
 __all__ = ["TrajectoryBatch", "choice_rows", "kraus_draws"]

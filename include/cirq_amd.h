@@ -373,6 +373,13 @@ int cqs_apply_select_batch(void* amps, int32_t n_bits, int32_t n_states, int32_t
  * sums; 1 <= n_mats <= 16): the weights cirq-core/cirq/sim/state_vector_simulation_state.py:224-250
  * walks to pick a Kraus operator.  workspace: DEVICE, cqs_kraus_workspace_bytes bytes. */
 size_t cqs_kraus_workspace_bytes(int32_t n_bits, int32_t n_states, int32_t k, int32_t n_mats);
+
+/* Copy r's Kraus operator from its weights (DEVICE double[n_states * n_mats], as
+ * cqs_kraus_weights_batch writes them) and uniform (DEVICE double[n_states]), by the walk of
+ * cirq-core/cirq/sim/state_vector_simulation_state.py:224-250: select[r] (DEVICE int32) and
+ * scales[r] = 1 / sqrt(its weight) (DEVICE float32), ready for cqs_apply_select_batch. */
+int cqs_kraus_choose_batch(const double* weights, const double* uniforms, int32_t n_states, int32_t n_mats,
+                           int32_t* select, float* scales, void* stream);
 int cqs_kraus_weights_batch(const void* amps, int32_t n_bits, int32_t n_states, int32_t k, const int32_t* targets,
                             const void* mats, int32_t n_mats, double* out, void* workspace, void* stream);
 

@@ -265,3 +265,27 @@ def test_draw_rows_bit_exact_with_numpy_choice(gpu, m):
         pr /= pr.sum()
         want = int(a.choice(len(pr), p=pr))
         assert idx[r] == want and p[r] == pr[want], (r, idx[r], want)
+
+
+def test_kraus_choose_on_device_matches_the_walk(gpu):
+    """cqs_kraus_choose_batch picks what the host restatement of Cirq's Kraus walk picks, with the
+    same 1/sqrt(weight) scale."""
+    _log("kraus choice on the device")
+    import torch
+
+    from cirq_amd import engine
+    from cirq_amd.trajectories import kraus_draws
+
+    rng = np.random.RandomState(3)
+    R, M = 4000, 6
+    W = rng.exponential(size=(R, M))
+    W[rng.uniform(size=W.shape) < 0.25] = 0.0
+    W /= W.sum(axis=1, keepdims=True) + rng.uniform(-0.05, 0.05, size=(R, 1))
+    W[:3, 1:] = 0.0  # walks that end on the first operator or fall back to it
+    u = rng.uniform(size=R)
+    sel, scl = engine.kraus_choose_batch(torch.from_numpy(W).cuda(), torch.from_numpy(u).cuda())
+    want_sel, want_w = kraus_draws(W, u)
+    assert np.array_equal(sel.cpu().numpy(), want_sel)
+    with np.errstate(divide="ignore"):
+        want_scale = (1.0 / np.sqrt(want_w)).astype(np.float32)
+    assert np.array_equal(scl.cpu().numpy(), want_scale)
