@@ -78,6 +78,109 @@ __global__ __launch_bounds__(256) void k_select_batch(float2* __restrict__ amps,
   }
 }
 
+// Several per-copy selections in one sweep of the copies: a run of mixtures whose targets lie in
+// a union of at most 4 bits.  Each thread loads the 2^u amplitudes of one group of the union into
+// registers, applies the run's ops in order (op i: copy r's matrix select[row_i][r] of op i's set,
+// on union positions p0 (and p1) of the group), and stores them: one read and one write of the
+// copies for the whole run instead of one per op.
+constexpr int kRunOps = 16;
+struct SelRun {
+  int u;                 // bits in the union (1..4)
+  int bits[4];           // their flat positions, ascending
+  int nops;
+  int k[kRunOps];        // 1 or 2 targets
+  int p0[kRunOps];       // union position of matrix index bit 0
+  int p1[kRunOps];       // union position of matrix index bit 1 (k = 2)
+  int mat[kRunOps];      // offset (complex entries) of the op's matrices in mats
+  int n_mats[kRunOps];
+  int row[kRunOps];      // row of the op in the selection table (one int32 per copy)
+};
+
+// Thread t's 16 amplitudes live in LDS at xs[e * 256 + t] (consecutive threads, consecutive
+// addresses), so the ops below index them by run-time union positions.
+__device__ __forceinline__ void run_op1(float2* xs, int D, int p0, const float2* M) {
+  const int t = threadIdx.x, b0 = 1 << p0;
+  for (int e = 0; e < D; ++e) {
+    if (e & b0) continue;
+    const float2 a = xs[e * 256 + t], b = xs[(e | b0) * 256 + t];
+    xs[e * 256 + t] = cfma(M[1], b, cmul(M[0], a));
+    xs[(e | b0) * 256 + t] = cfma(M[3], b, cmul(M[2], a));
+  }
+}
+
+__device__ __forceinline__ void run_op2(float2* xs, int D, int p0, int p1, const float2* M) {
+  const int t = threadIdx.x, b0 = 1 << p0, b1 = 1 << p1;
+  for (int e = 0; e < D; ++e) {
+    if (e & (b0 | b1)) continue;
+    const int id[4] = {e, e | b0, e | b1, e | b0 | b1};  // matrix index bit 0 <-> p0, bit 1 <-> p1
+    float2 v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = xs[id[c] * 256 + t];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc = cfma(M[r * 4 + c], v[c], acc);
+      xs[id[r] * 256 + t] = acc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_select_run(float2* __restrict__ amps, uint64_t per, SelRun R,
+                                                    const float2* __restrict__ mats,
+                                                    const int32_t* __restrict__ table, int n_states) {
+  // the op fields go to LDS (compile-time indices here), so the op loop below reads them with a
+  // run-time index without copying the argument block to scratch memory
+  __shared__ int sk[kRunOps], sp0[kRunOps], sp1[kRunOps], smat[kRunOps], snm[kRunOps], srow[kRunOps];
+  __shared__ float2 xs[16 * 256];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kRunOps; ++i) {
+      sk[i] = R.k[i];
+      sp0[i] = R.p0[i];
+      sp1[i] = R.p1[i];
+      smat[i] = R.mat[i];
+      snm[i] = R.n_mats[i];
+      srow[i] = R.row[i];
+    }
+  }
+  __syncthreads();
+  const int copy = blockIdx.y;
+  float2* s = amps + (uint64_t)copy * per;
+  const int D = 1 << R.u;
+  const int t = threadIdx.x;
+  uint64_t pb[4];  // flat offsets of the union bits
+#pragma unroll
+  for (int b = 0; b < 4; ++b) pb[b] = b < R.u ? 1ull << R.bits[b] : 0;
+  const uint64_t groups = per >> R.u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + t; g < groups; g += stride) {
+    uint64_t j0 = g;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (b < R.u) j0 = insert0_64(j0, R.bits[b]);
+    for (int e = 0; e < D; ++e) {
+      const uint64_t j = j0 | ((e & 1) ? pb[0] : 0) | ((e & 2) ? pb[1] : 0) | ((e & 4) ? pb[2] : 0) |
+                         ((e & 8) ? pb[3] : 0);
+      xs[e * 256 + t] = s[j];
+    }
+    for (int i = 0; i < R.nops; ++i) {
+      const int sel = table[(size_t)srow[i] * n_states + copy];
+      if (sel < 0 || sel >= snm[i]) continue;
+      const float2* M = mats + smat[i] + (size_t)sel * (sk[i] == 1 ? 4 : 16);
+      if (sk[i] == 1)
+        run_op1(xs, D, sp0[i], M);
+      else
+        run_op2(xs, D, sp0[i], sp1[i], M);
+    }
+    for (int e = 0; e < D; ++e) {
+      const uint64_t j = j0 | ((e & 1) ? pb[0] : 0) | ((e & 2) ? pb[1] : 0) | ((e & 4) ? pb[2] : 0) |
+                         ((e & 8) ? pb[3] : 0);
+      s[j] = xs[e * 256 + t];
+    }
+  }
+}
+
 constexpr int kMaxKraus = 16;
 
 // partial[(r * gridDim.x + blockIdx.x) * n_mats + i] = sum over this workgroup's groups of copy r
@@ -289,6 +392,61 @@ int cqs_kraus_choose_batch(const double* weights, const double* uniforms, int32_
   CQS_REQUIRE(n_states >= 1 && n_mats >= 1 && n_mats <= kMaxKraus, "bad sizes");
   hipLaunchKernelGGL(k_kraus_choose, dim3((n_states + 255) / 256), dim3(256), 0, as_stream(stream), weights, uniforms,
                      (int)n_states, (int)n_mats, select, scales);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+
+int cqs_select_run_batch(void* amps, int32_t n_bits, int32_t n_states, int32_t n_ops, const int32_t* op_k,
+                         const int32_t* op_targets, const int32_t* op_mat, const int32_t* op_n_mats,
+                         const int32_t* op_row, const void* mats, int64_t mat_entries, const int32_t* table,
+                         int32_t table_rows, void* stream) {
+  CQS_REQUIRE(amps != nullptr && mats != nullptr && table != nullptr, "NULL pointer");
+  CQS_REQUIRE(op_k && op_targets && op_mat && op_n_mats && op_row, "NULL op arrays");
+  CQS_REQUIRE(n_bits >= 1 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
+  CQS_REQUIRE(n_states >= 1 && n_states <= 65535, "n_states=%d out of range (1..65535)", n_states);
+  CQS_REQUIRE(n_ops >= 1 && n_ops <= kRunOps, "n_ops=%d out of range (1..%d)", n_ops, kRunOps);
+  SelRun R;
+  R.u = 0;
+  R.nops = n_ops;
+  uint64_t un = 0;
+  for (int i = 0; i < n_ops; ++i) {
+    CQS_REQUIRE(op_k[i] == 1 || op_k[i] == 2, "op %d acts on %d bits (1 or 2)", i, op_k[i]);
+    CQS_REQUIRE(op_n_mats[i] >= 1 && op_mat[i] >= 0 && op_row[i] >= 0 && op_row[i] < table_rows,
+                "op %d: bad matrix set or table row", i);
+    const int64_t d = op_k[i] == 1 ? 2 : 4;
+    CQS_REQUIRE((int64_t)op_mat[i] + (int64_t)op_n_mats[i] * d * d <= mat_entries, "op %d: matrices past the end", i);
+    for (int m = 0; m < op_k[i]; ++m) {
+      const int b = op_targets[2 * i + m];
+      CQS_REQUIRE(b >= 0 && b < n_bits, "op %d: target bit %d out of range", i, b);
+      un |= 1ull << b;
+    }
+    CQS_REQUIRE(op_k[i] == 1 || op_targets[2 * i] != op_targets[2 * i + 1], "op %d: repeated target", i);
+  }
+  for (int b = 0; b < n_bits; ++b)
+    if ((un >> b) & 1ull) {
+      CQS_REQUIRE(R.u < 4, "the run's targets span more than 4 bits");
+      R.bits[R.u++] = b;
+    }
+  for (int b = R.u; b < 4; ++b) R.bits[b] = 0;
+  auto pos = [&](int bit) {
+    for (int q = 0; q < R.u; ++q)
+      if (R.bits[q] == bit) return q;
+    return 0;
+  };
+  for (int i = 0; i < kRunOps; ++i) {
+    const bool on = i < n_ops;
+    R.k[i] = on ? op_k[i] : 1;
+    R.p0[i] = on ? pos(op_targets[2 * i]) : 0;
+    R.p1[i] = on && op_k[i] == 2 ? pos(op_targets[2 * i + 1]) : 0;
+    R.mat[i] = on ? op_mat[i] : 0;
+    R.n_mats[i] = on ? op_n_mats[i] : 1;
+    R.row[i] = on ? op_row[i] : 0;
+  }
+  const uint64_t per = 1ull << n_bits;
+  const int gx = blocks_per_copy(per >> R.u, n_states);
+  hipLaunchKernelGGL(k_select_run, dim3(gx, n_states), dim3(256), 0, as_stream(stream), (float2*)amps, per, R,
+                     (const float2*)mats, table, (int)n_states);
   CQS_HIP(hipGetLastError());
   return CQS_OK;
 }

@@ -266,6 +266,29 @@ def kraus_weights_batch_dev(states: torch.Tensor, n_bits: int, n_states: int, ta
     return out.cpu().numpy() if to_host else out
 
 
+def select_run_batch(states: torch.Tensor, n_bits: int, n_states: int, ops, mats_dev: torch.Tensor,
+                     table_dev: torch.Tensor):
+    """A run of per-copy mixture choices in one sweep of the copies.  ops: (k, targets, offset of
+    the op's matrices in mats_dev in complex entries, their count, row of the op in table_dev)."""
+    _check_copies(states, n_bits, n_states)
+    if mats_dev.dtype != torch.float32 or table_dev.dtype != torch.int32 or table_dev.dim() != 2:
+        raise ValueError("float32 matrix buffer and int32 [rows, copies] table expected")
+    if table_dev.shape[1] != n_states or not table_dev.is_contiguous():
+        raise ValueError("the table needs one contiguous column per copy")
+    n = len(ops)
+    k = np.array([o[0] for o in ops], dtype=np.int32)
+    t = np.array([list(o[1]) + [0] * (2 - len(o[1])) for o in ops], dtype=np.int32).reshape(-1)
+    mo = np.array([o[2] for o in ops], dtype=np.int32)
+    nm = np.array([o[3] for o in ops], dtype=np.int32)
+    row = np.array([o[4] for o in ops], dtype=np.int32)
+    check(
+        lib().cqs_select_run_batch(ptr(states), n_bits, n_states, n, k.ctypes.data, t.ctypes.data, mo.ctypes.data,
+                                   nm.ctypes.data, row.ctypes.data, ptr(mats_dev), mats_dev.numel() // 2,
+                                   ptr(table_dev), int(table_dev.shape[0]), stream_handle()),
+        "cqs_select_run_batch",
+    )
+
+
 def kraus_choose_batch(weights_dev: torch.Tensor, uniforms_dev: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     """Per copy, the Kraus operator the channel walk picks for its uniform and 1/sqrt(its weight),
     on the device (int32 selections, float32 scales)."""

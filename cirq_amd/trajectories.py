@@ -99,6 +99,8 @@ class TrajectoryBatch:
         self._programs: dict = {}
         self.n_draws = sum(1 for kind, _ in actions if kind == "op")
         self._channels: dict = {}  # stochastic op index -> its matrices on the device
+        self._mix_mats: dict = {}  # stochastic op index -> its matrices (host complex64, flat)
+        self._plan = None  # the actions as steps, with runs of mixtures merged (built on first use)
 
     @staticmethod
     def supported(actions, dm: bool) -> bool:
@@ -174,47 +176,94 @@ class TrajectoryBatch:
             m_dev, n_mats = engine.device_mats(mats, targets, self.device)
             hit = (targets, m_dev, n_mats, probs)
             self._channels[j] = hit
+            self._mix_mats[j] = mats.astype(np.complex64).reshape(-1)
         return hit
+
+    def _steps(self):
+        """The actions as steps: ('program', compiled), ('measure', j, op), ('kraus', j, op),
+        ('mix', j, op) for a mixture on 3 qubits, and ('run', [j, ...]) for consecutive mixtures on
+        1-2 qubits whose targets span at most 4 bits (one sweep of the copies for all of them).
+        Mixture j's matrices sit in one device buffer at an offset; row j of the selection table
+        holds every copy's choice for it."""
+        if self._plan is not None:
+            return self._plan
+        steps, run, run_bits = [], [], set()
+        offsets, bufs, total = {}, [], 0
+
+        def flush():
+            nonlocal run, run_bits
+            if run:
+                steps.append(("run", list(run)))
+            run, run_bits = [], set()
+
+        j = 0
+        for kind, a in self.actions:
+            if kind == "program":
+                flush()
+                steps.append(("program", a))
+            elif kind == "op":
+                if _is_measurement(a) or self.dm:
+                    flush()
+                    steps.append(("measure", j, a))
+                else:
+                    targets, _, n_mats, probs = self._channel_of(j, a)
+                    if probs is None:
+                        flush()
+                        steps.append(("kraus", j, a))
+                    elif len(targets) > 2:
+                        flush()
+                        steps.append(("mix", j, a))
+                    else:
+                        if len(run) >= 16 or len(run_bits | set(targets)) > 4:
+                            flush()
+                        run.append(j)
+                        run_bits |= set(targets)
+                        d = 1 << len(targets)
+                        offsets[j] = total
+                        total += n_mats * d * d
+                        bufs.append(self._mix_mats[j])
+                j += 1
+        flush()
+        self._run_mats = engine._dev_array(np.concatenate(bufs).view(np.float32), self.device) if bufs else None
+        self._run_offsets = offsets
+        self._plan = steps
+        return steps
 
     def _run_batch(self, state0: torch.Tensor, resolver, R: int, u: np.ndarray) -> list:
         copy_bits = R.bit_length() - 1
         states = state0.repeat(R)
+        steps = self._steps()
         # a mixture's draw does not depend on the state: every copy's choice for every mixture is
-        # known now and goes to the device in one copy
-        stoch = [a for kind, a in self.actions if kind == "op"]
-        mix_rows = {}
-        for j, a in enumerate(stoch):
-            if not _is_measurement(a) and not self.dm:
-                targets, _, n_mats, probs = self._channel_of(j, a)
-                if probs is not None:
-                    mix_rows[j] = len(mix_rows)
-        sel_dev = None
-        if mix_rows:
-            table = np.empty((len(mix_rows), R), dtype=np.int32)
-            for j, row in mix_rows.items():
-                probs = self._channel_of(j, stoch[j])[3]
-                table[row] = choice_rows(np.broadcast_to(probs, (R, len(probs))), u[:, j])
-            sel_dev = engine._dev_array(table, self.device)
+        # known now and goes to the device in one copy (row j: stochastic op j)
+        D = u.shape[1]
+        table = np.full((max(D, 1), R), -1, dtype=np.int32)
+        for j, (targets, _, n_mats, probs) in self._channels.items():
+            if probs is not None:
+                table[j] = choice_rows(np.broadcast_to(probs, (R, len(probs))), u[:, j])
+        table_dev = engine._dev_array(table, self.device)
         ones = torch.ones(R, dtype=torch.float32, device=self.device)
-        u_dev = engine._dev_array(np.ascontiguousarray(u.T), self.device) if u.shape[1] else None  # row j: op j
+        u_dev = engine._dev_array(np.ascontiguousarray(u.T), self.device) if D else None  # row j: op j
         recs: list = []
-        j = 0
-        for kind, a in self.actions:
+        for step in steps:
+            kind = step[0]
             if kind == "program":
-                packed, prog, ev = self._program(a, copy_bits)
+                packed, prog, ev = self._program(step[1], copy_bits)
                 if packed.param_entries:
                     prog.set_mats(fill_param_mats(packed, ev, ev.symbol_values([resolver]))[0])
                 prog.run(states)
-            elif kind == "op":
-                if _is_measurement(a):
-                    recs.append(self._measure(states, R, a, u[:, j]))
-                elif j in mix_rows:
-                    targets, m_dev, n_mats, _ = self._channel_of(j, a)
-                    engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, sel_dev[mix_rows[j]],
-                                                  ones)
-                else:
-                    self._kraus(states, R, j, a, u_dev[j])
-                j += 1
+            elif kind == "measure":
+                recs.append(self._measure(states, R, step[2], u[:, step[1]]))
+            elif kind == "kraus":
+                self._kraus(states, R, step[1], step[2], u_dev[step[1]])
+            elif kind == "mix":
+                targets, m_dev, n_mats, _ = self._channel_of(step[1], step[2])
+                engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, table_dev[step[1]], ones)
+            else:
+                ops = []
+                for j in step[1]:
+                    targets, _, n_mats, _ = self._channels[j]
+                    ops.append((len(targets), targets, self._run_offsets[j], n_mats, j))
+                engine.select_run_batch(states, self.n_bits, R, ops, self._run_mats, table_dev)
         return recs
 
     def _measure(self, states, R: int, op, u: np.ndarray):

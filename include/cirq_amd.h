@@ -374,6 +374,19 @@ int cqs_apply_select_batch(void* amps, int32_t n_bits, int32_t n_states, int32_t
  * walks to pick a Kraus operator.  workspace: DEVICE, cqs_kraus_workspace_bytes bytes. */
 size_t cqs_kraus_workspace_bytes(int32_t n_bits, int32_t n_states, int32_t k, int32_t n_mats);
 
+/* A run of per-copy channel choices in one sweep of the copies (mixtures whose draws are known:
+ * cirq-core/cirq/sim/state_vector_simulation_state.py:183-203 for each in order).  Op i acts on
+ * op_k[i] (1 or 2) flat bits op_targets[2i], op_targets[2i + 1] (matrix index bit m <-> bit
+ * op_targets[2i + m]); copy r applies matrix table[op_row[i] * n_states + r] of the op's set of
+ * op_n_mats[i] row-major 2^k x 2^k complex64 matrices at mats + op_mat[i] (complex entries).  All
+ * targets together span at most 4 bits; 1 <= n_ops <= 16.  op_*: HOST int32 arrays; mats
+ * (mat_entries complex64 entries) and table (table_rows x n_states int32): DEVICE.  Copies whose
+ * entry is outside the op's set skip that op. */
+int cqs_select_run_batch(void* amps, int32_t n_bits, int32_t n_states, int32_t n_ops, const int32_t* op_k,
+                         const int32_t* op_targets, const int32_t* op_mat, const int32_t* op_n_mats,
+                         const int32_t* op_row, const void* mats, int64_t mat_entries, const int32_t* table,
+                         int32_t table_rows, void* stream);
+
 /* Copy r's Kraus operator from its weights (DEVICE double[n_states * n_mats], as
  * cqs_kraus_weights_batch writes them) and uniform (DEVICE double[n_states]), by the walk of
  * cirq-core/cirq/sim/state_vector_simulation_state.py:224-250: select[r] (DEVICE int32) and
