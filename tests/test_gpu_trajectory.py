@@ -289,3 +289,41 @@ def test_kraus_choose_on_device_matches_the_walk(gpu):
     with np.errstate(divide="ignore"):
         want_scale = (1.0 / np.sqrt(want_w)).astype(np.float32)
     assert np.array_equal(scl.cpu().numpy(), want_scale)
+
+
+def test_select_run_matches_one_select_per_op(gpu):
+    """cqs_select_run_batch applies a run of per-copy choices (1- and 2-qubit, overlapping
+    targets) as the one-op-at-a-time select kernel does, within complex64 rounding."""
+    _log("select run vs one select per op")
+    import torch
+
+    from cirq_amd import engine
+
+    rng = np.random.RandomState(12)
+    n, R = 9, 64
+    dev = torch.device("cuda")
+    st = torch.from_numpy((rng.randn(R << n) + 1j * rng.randn(R << n)).astype(np.complex64)).to(dev)
+    st /= torch.linalg.vector_norm(st.view(R, -1), dim=1).repeat_interleave(1 << n)
+    runs = [((2,), 3), ((5,), 4), ((2, 7), 2), ((7,), 3), ((5, 2), 5), ((0,), 2)]  # targets span bits 0,2,5,7
+    mats, offs, rows, total = [], [], [], 0
+    table = np.zeros((len(runs), R), dtype=np.int32)
+    for i, (targets, cnt) in enumerate(runs):
+        d = 1 << len(targets)
+        ms = np.stack([np.linalg.qr(rng.randn(d, d) + 1j * rng.randn(d, d))[0] for _ in range(cnt)])
+        mats.append(ms.astype(np.complex64).reshape(-1))
+        offs.append(total)
+        total += cnt * d * d
+        table[i] = rng.randint(0, cnt, size=R)
+    mats_dev = torch.from_numpy(np.concatenate(mats).view(np.float32)).to(dev)
+    table_dev = torch.from_numpy(table).to(dev)
+    ops = [(len(t), t, offs[i], c, i) for i, (t, c) in enumerate(runs)]
+    a = st.clone()
+    engine.select_run_batch(a, n, R, ops, mats_dev, table_dev)
+    b = st.clone()
+    ones = torch.ones(R, dtype=torch.float32, device=dev)
+    for i, (targets, cnt) in enumerate(runs):
+        d = 1 << len(targets)
+        m_dev, _ = engine.device_mats(np.concatenate(mats)[offs[i]:offs[i] + cnt * d * d].reshape(cnt, d, d),
+                                      targets, dev)
+        engine.apply_select_batch_dev(b, n, R, targets, m_dev, cnt, table_dev[i], ones)
+    assert torch.max(torch.abs(a - b)).item() < 2e-6
