@@ -126,6 +126,8 @@ _SIGNATURES = {
     "cqs_kraus_workspace_bytes": ([_i32, _i32, _i32, _i32], _sz),
     "cqs_kraus_weights_batch": ([_vp, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _vp], ctypes.c_int),
     "cqs_kraus_choose_batch": ([_vp, _vp, _i32, _i32, _vp, _vp, _vp], ctypes.c_int),
+    "cqs_select_then_kraus_weights": ([_vp, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp],
+                                      ctypes.c_int),
     "cqs_select_run_batch": ([_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _vp],
                              ctypes.c_int),
     "cqs_indices_to_bits": ([_vp, _i64, _i32, _vp, _vp], ctypes.c_int),

@@ -236,6 +236,90 @@ __global__ __launch_bounds__(256) void k_kraus_weights(const float2* __restrict_
   }
 }
 
+// Two single-qubit Kraus channels in a row, A then B, in one sweep of the copies: copy r applies
+// A's chosen operator (matsA[selA[r]] times scaleA[r], as k_select_batch<1> does) on bit a, stores
+// the result, and accumulates ||K_i x||^2 of B's operators on bit b for the weights of B's draw
+// (partial sums as k_kraus_weights writes them).  Groups span the union of a and b.
+__global__ __launch_bounds__(256) void k_select_then_weights(float2* __restrict__ amps, uint64_t per, int bit_a,
+                                                             const float2* __restrict__ matsA, int n_matsA,
+                                                             const int32_t* __restrict__ selA,
+                                                             const float* __restrict__ scaleA, int bit_b,
+                                                             const float2* __restrict__ matsB, int n_matsB,
+                                                             double* __restrict__ partial) {
+  __shared__ float2 sm[kMaxKraus * 4];
+  __shared__ double red[4][kMaxKraus];
+  for (int i = threadIdx.x; i < n_matsB * 4; i += blockDim.x) sm[i] = matsB[i];
+  __syncthreads();
+  const int sel = selA[blockIdx.y];
+  const bool apply = sel >= 0 && sel < n_matsA;
+  const float2* MA = matsA + (apply ? sel : 0) * 4;
+  const float sc = scaleA[blockIdx.y];
+  float2* s = amps + (uint64_t)blockIdx.y * per;
+  const bool same = bit_a == bit_b;
+  const int lo = bit_a < bit_b ? bit_a : bit_b, hi = bit_a < bit_b ? bit_b : bit_a;
+  const uint64_t A = 1ull << bit_a, B = 1ull << bit_b;
+  const uint64_t groups = same ? per >> 1 : per >> 2;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  double acc[kMaxKraus];
+#pragma unroll
+  for (int i = 0; i < kMaxKraus; ++i) acc[i] = 0.0;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
+    const uint64_t j0 = same ? insert0_64(g, lo) : insert0_64(insert0_64(g, lo), hi);
+    // x[e]: e bit 0 <-> bit a, e bit 1 <-> bit b (unused when a == b)
+    const int ne = same ? 2 : 4;
+    uint64_t id[4];
+    float2 x[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      id[e] = j0 | ((e & 1) ? A : 0) | ((e & 2) ? B : 0);
+      x[e] = e < ne ? s[id[e]] : make_float2(0.f, 0.f);
+    }
+    if (apply) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {  // the pairs along a: (x[2h], x[2h + 1])
+        if (h == 1 && same) break;
+        const float2 a0 = x[2 * h], a1 = x[2 * h + 1];
+        float2 y0 = cfma(MA[0], a0, make_float2(0.f, 0.f)), y1 = cfma(MA[2], a0, make_float2(0.f, 0.f));
+        y0 = cfma(MA[1], a1, y0);
+        y1 = cfma(MA[3], a1, y1);
+        x[2 * h] = make_float2(y0.x * sc, y0.y * sc);
+        x[2 * h + 1] = make_float2(y1.x * sc, y1.y * sc);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (e < ne) s[id[e]] = x[e];
+    }
+    // B's weights: pairs along b (along a again when a == b)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && same) break;
+      const float2 b0 = same ? x[0] : x[h], b1 = same ? x[1] : x[h + 2];
+#pragma unroll
+      for (int i = 0; i < kMaxKraus; ++i) {
+        if (i >= n_matsB) break;
+        const float2* M = sm + i * 4;
+        const float2 z0 = cfma(M[1], b1, cfma(M[0], b0, make_float2(0.f, 0.f)));
+        const float2 z1 = cfma(M[3], b1, cfma(M[2], b0, make_float2(0.f, 0.f)));
+        acc[i] += ((double)z0.x * (double)z0.x + (double)z0.y * (double)z0.y) +
+                  ((double)z1.x * (double)z1.x + (double)z1.y * (double)z1.y);
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kMaxKraus; ++i) {
+    if (i >= n_matsB) break;
+    const double v = wave_sum(acc[i]);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < n_matsB) {
+    const int i = threadIdx.x;
+    partial[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * n_matsB + i] =
+        ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+  }
+}
+
 // out[r * n_mats + i] = sum over b of partial[(r * nblk + b) * n_mats + i], b ascending.
 __global__ void k_kraus_finish(const double* __restrict__ partial, int nblk, int n_mats, int n_states,
                                double* __restrict__ out) {
@@ -447,6 +531,32 @@ int cqs_select_run_batch(void* amps, int32_t n_bits, int32_t n_states, int32_t n
   const int gx = blocks_per_copy(per >> R.u, n_states);
   hipLaunchKernelGGL(k_select_run, dim3(gx, n_states), dim3(256), 0, as_stream(stream), (float2*)amps, per, R,
                      (const float2*)mats, table, (int)n_states);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
+}
+
+
+int cqs_select_then_kraus_weights(void* amps, int32_t n_bits, int32_t n_states, int32_t bit_a, const void* mats_a,
+                                  int32_t n_mats_a, const int32_t* select_a, const float* scales_a, int32_t bit_b,
+                                  const void* mats_b, int32_t n_mats_b, double* out, void* workspace, void* stream) {
+  CQS_REQUIRE(amps && mats_a && select_a && scales_a && mats_b && out && workspace, "NULL pointer");
+  CQS_REQUIRE(n_bits >= 2 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
+  CQS_REQUIRE(n_states >= 1 && n_states <= 65535, "n_states=%d out of range (1..65535)", n_states);
+  CQS_REQUIRE(bit_a >= 0 && bit_a < n_bits && bit_b >= 0 && bit_b < n_bits, "bad target bits");
+  CQS_REQUIRE(n_mats_a >= 1 && n_mats_b >= 1 && n_mats_b <= kMaxKraus, "bad operator counts");
+  const uint64_t per = 1ull << n_bits;
+  // the same workgroup count as cqs_kraus_weights_batch for a one-qubit channel, so the
+  // workspace size and cqs_kraus_workspace_bytes agree
+  const int gx = blocks_per_copy(per >> 1, n_states);
+  hipStream_t s = as_stream(stream);
+  double* partial = (double*)workspace;
+  hipLaunchKernelGGL(k_select_then_weights, dim3(gx, n_states), dim3(256), 0, s, (float2*)amps, per, (int)bit_a,
+                     (const float2*)mats_a, (int)n_mats_a, select_a, scales_a, (int)bit_b, (const float2*)mats_b,
+                     (int)n_mats_b, partial);
+  CQS_HIP(hipGetLastError());
+  const int total = n_states * n_mats_b;
+  hipLaunchKernelGGL(k_kraus_finish, dim3((total + 255) / 256), dim3(256), 0, s, (const double*)partial, gx, n_mats_b,
+                     (int)n_states, out);
   CQS_HIP(hipGetLastError());
   return CQS_OK;
 }

@@ -289,6 +289,27 @@ def select_run_batch(states: torch.Tensor, n_bits: int, n_states: int, ops, mats
     )
 
 
+def select_then_kraus_weights(states: torch.Tensor, n_bits: int, n_states: int, bit_a: int, mats_a: torch.Tensor,
+                              n_mats_a: int, sel_a: torch.Tensor, scl_a: torch.Tensor, bit_b: int,
+                              mats_b: torch.Tensor, n_mats_b: int, ws: Workspace) -> torch.Tensor:
+    """One sweep of the copies: channel A's chosen operators on bit_a, then channel B's weights on
+    bit_b (device float64 [n_states, n_mats_b]).  Both channels act on one qubit."""
+    _check_copies(states, n_bits, n_states)
+    if mats_a.numel() != n_mats_a * 8 or mats_b.numel() != n_mats_b * 8:
+        raise ValueError("one-qubit operator sets expected")
+    if sel_a.dtype != torch.int32 or scl_a.dtype != torch.float32 or sel_a.numel() < n_states or scl_a.numel() < n_states:
+        raise ValueError("one int32 selection and one float32 scale per copy")
+    out = torch.empty((n_states, n_mats_b), dtype=torch.float64, device=states.device)
+    w = ws.get(lib().cqs_kraus_workspace_bytes(n_bits, n_states, 1, n_mats_b))
+    check(
+        lib().cqs_select_then_kraus_weights(ptr(states), n_bits, n_states, int(bit_a), ptr(mats_a), n_mats_a,
+                                            ptr(sel_a), ptr(scl_a), int(bit_b), ptr(mats_b), n_mats_b, ptr(out),
+                                            ptr(w), stream_handle()),
+        "cqs_select_then_kraus_weights",
+    )
+    return out
+
+
 def kraus_choose_batch(weights_dev: torch.Tensor, uniforms_dev: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     """Per copy, the Kraus operator the channel walk picks for its uniform and 1/sqrt(its weight),
     on the device (int32 selections, float32 scales)."""

@@ -180,9 +180,10 @@ class TrajectoryBatch:
         return hit
 
     def _steps(self):
-        """The actions as steps: ('program', compiled), ('measure', j, op), ('kraus', j, op),
-        ('mix', j, op) for a mixture on 3 qubits, and ('run', [j, ...]) for consecutive mixtures on
-        1-2 qubits whose targets span at most 4 bits (one sweep of the copies for all of them).
+        """The actions as steps: ('program', compiled), ('measure', j, op), ('kraus', [j, ...]) for
+        Kraus channels in a row (one-qubit ones chained), ('mix', j, op) for a mixture on 3 qubits,
+        and ('run', [j, ...]) for consecutive mixtures on 1-2 qubits whose targets span at most 4
+        bits (one sweep of the copies for all of them).
         Mixture j's matrices sit in one device buffer at an offset; row j of the selection table
         holds every copy's choice for it."""
         if self._plan is not None:
@@ -209,7 +210,12 @@ class TrajectoryBatch:
                     targets, _, n_mats, probs = self._channel_of(j, a)
                     if probs is None:
                         flush()
-                        steps.append(("kraus", j, a))
+                        prev = steps[-1] if steps else None
+                        if (len(targets) == 1 and n_mats <= MAX_KRAUS and prev is not None and prev[0] == "kraus"
+                                and len(self._channels[prev[1][-1]][0]) == 1):
+                            prev[1].append(j)  # one-qubit channels in a row: one sweep per channel
+                        else:
+                            steps.append(("kraus", [j]))
                     elif len(targets) > 2:
                         flush()
                         steps.append(("mix", j, a))
@@ -254,7 +260,7 @@ class TrajectoryBatch:
             elif kind == "measure":
                 recs.append(self._measure(states, R, step[2], u[:, step[1]]))
             elif kind == "kraus":
-                self._kraus(states, R, step[1], step[2], u_dev[step[1]])
+                self._kraus_chain(states, R, step[1], u_dev)
             elif kind == "mix":
                 targets, m_dev, n_mats, _ = self._channel_of(step[1], step[2])
                 engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, table_dev[step[1]], ones)
@@ -295,11 +301,23 @@ class TrajectoryBatch:
             inv[i] = 1 if x else 0
         return m.key, bits ^ inv[None, :]
 
-    def _kraus(self, states, R: int, j: int, op, u_dev: torch.Tensor):
-        """Weights, the walk and the chosen operator, all on the device: no round trip to the host."""
-        targets, m_dev, n_mats, _ = self._channel_of(j, op)
-        W = engine.kraus_weights_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, self.ws, to_host=False)
-        sel, scl = engine.kraus_choose_batch(W, u_dev)
+    def _kraus_chain(self, states, R: int, chain: list, u_dev: torch.Tensor):
+        """Kraus channels in a row: each one's weights, the walk and the chosen operator, all on the
+        device.  For one-qubit channels after the first, applying the previous channel's choice and
+        computing this channel's weights share one sweep of the copies."""
+        prev = None  # (targets, device matrices, count, selections, scales) of the last channel
+        for j in chain:
+            targets, m_dev, n_mats, _ = self._channels[j]
+            if prev is None:
+                W = engine.kraus_weights_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, self.ws,
+                                                   to_host=False)
+            else:
+                pt, pm, pn, ps, pc = prev
+                W = engine.select_then_kraus_weights(states, self.n_bits, R, pt[0], pm, pn, ps, pc, targets[0],
+                                                     m_dev, n_mats, self.ws)
+            sel, scl = engine.kraus_choose_batch(W, u_dev[j])
+            prev = (targets, m_dev, n_mats, sel, scl)
+        targets, m_dev, n_mats, sel, scl = prev
         engine.apply_select_batch_dev(states, self.n_bits, R, targets, m_dev, n_mats, sel, scl)
 
 

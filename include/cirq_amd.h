@@ -387,6 +387,16 @@ int cqs_select_run_batch(void* amps, int32_t n_bits, int32_t n_states, int32_t n
                          const int32_t* op_row, const void* mats, int64_t mat_entries, const int32_t* table,
                          int32_t table_rows, void* stream);
 
+/* Two one-qubit Kraus channels in a row, A on flat bit bit_a then B on bit_b, in one sweep of the
+ * copies: applies A's chosen operators (as cqs_apply_select_batch with select_a / scales_a, DEVICE)
+ * and writes B's weights ||K_i psi_r||^2 (as cqs_kraus_weights_batch) to out (DEVICE
+ * double[n_states * n_mats_b]).  workspace: cqs_kraus_workspace_bytes(n_bits, n_states, 1,
+ * n_mats_b) bytes.  The walk of cirq-core/cirq/sim/state_vector_simulation_state.py:205-257 for a
+ * run of channels, one read and write of the copies per channel. */
+int cqs_select_then_kraus_weights(void* amps, int32_t n_bits, int32_t n_states, int32_t bit_a, const void* mats_a,
+                                  int32_t n_mats_a, const int32_t* select_a, const float* scales_a, int32_t bit_b,
+                                  const void* mats_b, int32_t n_mats_b, double* out, void* workspace, void* stream);
+
 /* Copy r's Kraus operator from its weights (DEVICE double[n_states * n_mats], as
  * cqs_kraus_weights_batch writes them) and uniform (DEVICE double[n_states]), by the walk of
  * cirq-core/cirq/sim/state_vector_simulation_state.py:224-250: select[r] (DEVICE int32) and

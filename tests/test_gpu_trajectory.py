@@ -327,3 +327,35 @@ def test_select_run_matches_one_select_per_op(gpu):
                                       targets, dev)
         engine.apply_select_batch_dev(b, n, R, targets, m_dev, cnt, table_dev[i], ones)
     assert torch.max(torch.abs(a - b)).item() < 2e-6
+
+
+@pytest.mark.parametrize("same_bit", [False, True])
+def test_select_then_kraus_weights_matches_two_kernels(gpu, same_bit):
+    """The fused sweep leaves the copies as cqs_apply_select_batch does (bit for bit) and returns
+    cqs_kraus_weights_batch's weights up to FP64 summation order."""
+    _log(f"select then kraus weights same_bit={same_bit}")
+    import torch
+
+    import cirq_amd as cirq
+    from cirq_amd import engine
+    from cirq_amd.tape import kraus_of
+
+    rng = np.random.RandomState(5)
+    n, R = 10, 32
+    dev = torch.device("cuda")
+    st = torch.from_numpy((rng.randn(R << n) + 1j * rng.randn(R << n)).astype(np.complex64)).to(dev)
+    ka = np.stack(kraus_of(cirq.amplitude_damp(0.3)))
+    kb = np.stack(kraus_of(cirq.generalized_amplitude_damp(0.4, 0.2)))
+    a_bit, b_bit = 3, (3 if same_bit else 7)
+    ma, na = engine.device_mats(ka, [a_bit], dev)
+    mb, nb = engine.device_mats(kb, [b_bit], dev)
+    sel = torch.from_numpy(rng.randint(0, na, size=R).astype(np.int32)).to(dev)
+    scl = torch.from_numpy(rng.uniform(0.5, 2.0, size=R).astype(np.float32)).to(dev)
+    ws = engine.Workspace(dev)
+    x = st.clone()
+    w_fused = engine.select_then_kraus_weights(x, n, R, a_bit, ma, na, sel, scl, b_bit, mb, nb, ws).cpu().numpy()
+    y = st.clone()
+    engine.apply_select_batch_dev(y, n, R, [a_bit], ma, na, sel, scl)
+    w_two = engine.kraus_weights_batch_dev(y, n, R, [b_bit], mb, nb, ws)
+    assert torch.equal(x, y)
+    assert np.max(np.abs(w_fused - w_two) / np.maximum(np.abs(w_two), 1e-300)) < 1e-12
