@@ -97,12 +97,13 @@ def test_permuted_probabilities_and_sampling(gpu):
     sampling every qubit in a permuted order matches numpy's choice on that p."""
     _log("permuted probabilities")
     rng = np.random.RandomState(9)
-    for n, seed in [(5, 1), (13, 2), (17, 3)]:
-        c = generators.grid_circuit(3, 6, 6, seed=seed, n_qubits=n)
+    for n, seed in [(5, 1), (13, 2), (17, 3), (20, 4)]:
+        c = generators.grid_circuit(4, 5, 6, seed=seed, n_qubits=n)
         sim = cirq.Simulator()
         st = sim.simulate(c)._state
         psi = st.cpu().numpy()
-        axes = list(rng.permutation(n))
+        # n = 20: the reversed order, whose tiles hold the lowest and the highest bits
+        axes = list(range(n))[::-1] if n == 20 else list(rng.permutation(n))
         want_p = (psi.reshape((2,) * n).transpose(axes).reshape(-1) * np.conj(
             psi.reshape((2,) * n).transpose(axes).reshape(-1))).real.astype(np.float32)
         got_p = sim._sampler.permuted_probs(st, n, False, axes).cpu().numpy()
