@@ -401,17 +401,18 @@ __device__ __forceinline__ void ew_scale(float2 (&v)[PER], const Prog& P, sop_t&
 // Dense ops on register bits stay inside one thread.  Register bit r is tile bit log2(NT) + r, i.e.
 // bit r of the element index e; targets come sorted (R0 < R1 < R2) with matrix index bit m <->
 // target m.  Local controls are tested on the group's base element (controls are never targets).
-template <int PER, int NT, int K, int R0, int R1, int R2>
+// Without local controls (kCtl false) the groups are straight-line code: no per-group branch, so
+// results land in their own registers instead of being copied back at every branch join.
+template <int PER, int NT, int K, int R0, int R1, int R2, bool kCtl>
 __device__ __forceinline__ void reg_dense_k(float2 (&v)[PER], smat_t* M, int cm, int cv) {
   constexpr int D = 1 << K;
   constexpr int RS[3] = {R0, R1, R2};
   constexpr int TMASK = (1 << R0) | (K > 1 ? 1 << R1 : 0) | (K > 2 ? 1 << R2 : 0);
-  const bool ctl = cm != 0;  // uniform: the per-group test below is skipped by a scalar branch
   const int tm = (int)threadIdx.x & cm;
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     if (e & TMASK) continue;
-    if (ctl && (tm | ((e * NT) & cm)) != cv) continue;
+    if (kCtl && (tm | ((e * NT) & cm)) != cv) continue;
     int idx[D];
 #pragma unroll
     for (int m = 0; m < D; ++m) {
@@ -453,7 +454,11 @@ __device__ __forceinline__ void reg_dense(float2 (&v)[PER], const Prog& P, sop_t
   const int cm = op.cl_mask, cv = op.cl_val;
   const int r0 = op.t[0] - TB, r1 = op.t[1] - TB, r2 = op.t[2] - TB;
   const int code = op.k == 1 ? r0 : (op.k == 2 ? 16 + r0 * 4 + r1 : 32 + r0 * 16 + r1 * 4 + r2);
-#define CQS_RK(K, A, B, C) reg_dense_k<PER, NT, K, A, B, C>(v, M, cm, cv);
+#define CQS_RK(K, A, B, C)                                  \
+  if (cm == 0)                                              \
+    reg_dense_k<PER, NT, K, A, B, C, false>(v, M, 0, 0);    \
+  else                                                      \
+    reg_dense_k<PER, NT, K, A, B, C, true>(v, M, cm, cv);
 #define CQS_R1(A) \
   case A: if constexpr (NR > A) { CQS_RK(1, A, 0, 0) } return;
 #define CQS_R2(A, B) \
