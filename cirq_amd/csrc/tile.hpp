@@ -169,7 +169,8 @@ __device__ __forceinline__ void split_f16(float xr, float xi, float s, half2_t& 
 // bits of r deposited at the targets.  pad() is additive over disjoint bit sets, so the padded
 // index is a per-lane column part + a per-lane part from the lane's half (h) + a part that depends
 // only on compile-time row bits and the op's targets (scalar).
-template <int K, int NT>
+// kFull: every 32-column block is whole (L - K >= 5), so no column needs a bounds test.
+template <int K, int NT, bool kFull>
 __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, const half8_t* A,
                                                 float scale, float inv_scale) {
   constexpr int D = 1 << K;
@@ -215,7 +216,7 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, 
   }
   for (int cb = wave; cb < ncb; cb += NW) {
     const int col = cb * 32 + c32;
-    const bool valid = col < ncol;
+    const bool valid = kFull || col < ncol;
     int jc = col;
 #pragma unroll
     for (int m = 0; m < K; ++m) jc = insert0(jc, ts[m]);
@@ -275,10 +276,18 @@ __device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, cons
   const int var = variant_of(op, base);
   if ((op.kind & 0xff) == CQS_OP_BLOCK) {  // bits 8-9 may carry an adjoint tile selector
     const half8_t* A = reinterpret_cast<const half8_t*>(P.mats + op.mat + var * (2 << (2 * op.k)));
-    if (op.k == 4)
-      tile_block_mfma<4, NT>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
-    else if (op.k == 5)
-      tile_block_mfma<5, NT>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
+    const bool full = P.L - op.k >= 5;
+    if (op.k == 4) {
+      if (full)
+        tile_block_mfma<4, NT, true>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
+      else
+        tile_block_mfma<4, NT, false>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
+    } else if (op.k == 5) {
+      if (full)
+        tile_block_mfma<5, NT, true>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
+      else
+        tile_block_mfma<5, NT, false>(tile, P.L, op, A, P.mscale, 1.f / P.mscale);
+    }
     return;
   }
   smat_t* M = scalar_ptr(P.mats) + op.mat + var * (1 << (2 * op.k));
