@@ -178,7 +178,8 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, 
   constexpr int KS = R / 16;
   constexpr int RB = R / 32;
   constexpr int NW = NT / 64;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, c32 = lane & 31;
+  const int lane = threadIdx.x & 63, h = lane >> 5, c32 = lane & 31;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);  // uniform: column blocks in SGPRs
   int pbit[K];  // pad(1 << t[m])
   int ts[K];    // targets in ascending order
 #pragma unroll
@@ -214,14 +215,21 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, 
 #pragma unroll
     for (int f = 0; f < KS * RB * 2; ++f) ahold[f] = A[f * 64 + lane];
   }
+  // Inserting zeros at the targets and pad() are both additive over disjoint bits: the lane's
+  // column bits (c32) are placed once, the column block's (cb * 32) per block on the scalar unit.
+  int jl = c32;
+#pragma unroll
+  for (int m = 0; m < K; ++m) jl = insert0(jl, ts[m]);
+  const int pl = jl + (jl >> kPadShift);
+  const int b_in0 = pl + h_in, b_out0 = pl + h_out;
   for (int cb = wave; cb < ncb; cb += NW) {
     const int col = cb * 32 + c32;
     const bool valid = kFull || col < ncol;
-    int jc = col;
+    int jh = cb * 32;
 #pragma unroll
-    for (int m = 0; m < K; ++m) jc = insert0(jc, ts[m]);
-    const int pc = jc + (jc >> kPadShift);
-    const int b_in = pc + h_in, b_out = pc + h_out;
+    for (int m = 0; m < K; ++m) jh = insert0(jh, ts[m]);
+    const int ph = jh + (jh >> kPadShift);
+    const int b_in = b_in0 + ph, b_out = b_out0 + ph;
     floatx16_t acc[RB];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
