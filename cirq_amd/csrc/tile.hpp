@@ -122,20 +122,32 @@ __device__ __forceinline__ void tile_dense(float2* tile, int L, sop_t& op, MPtr 
   }
   const int G = (1 << L) >> K;
   const int cm = op.cl_mask, cv = op.cl_val;
-  for (int g = threadIdx.x; g < G; g += NT) {
-    int j0 = g;
+  // Group g = tid + g0 (g0 a multiple of NT: disjoint bits).  Inserting zeros at the targets and
+  // pad() are additive over disjoint bits, so the lane's part is placed once and the uniform
+  // part g0 and the row offsets pat[m] on the scalar unit.
+  int jt = threadIdx.x;
 #pragma unroll
-    for (int s = 0; s < K; ++s) j0 = insert0(j0, ts[s]);
-    if ((j0 & cm) != cv) continue;
+  for (int s = 0; s < K; ++s) jt = insert0(jt, ts[s]);
+  const int pt = pad(jt);
+  int ppat[D];
+#pragma unroll
+  for (int m = 0; m < D; ++m) ppat[m] = pad(pat[m]);
+  if ((int)threadIdx.x >= G) return;  // tiles with fewer groups than threads
+  for (int g0 = 0; g0 < G; g0 += NT) {
+    int jh = g0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) jh = insert0(jh, ts[s]);
+    if (((jt | jh) & cm) != cv) continue;
+    const int b = pt + pad(jh);
     float2 v[D];
 #pragma unroll
-    for (int m = 0; m < D; ++m) v[m] = tile[pad(j0 + pat[m])];
+    for (int m = 0; m < D; ++m) v[m] = tile[b + ppat[m]];
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
       for (int c = 0; c < D; ++c) acc = cfma(ldm(M, r * D + c), v[c], acc);
-      tile[pad(j0 + pat[r])] = acc;
+      tile[b + ppat[r]] = acc;
     }
   }
 }
