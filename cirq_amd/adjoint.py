@@ -222,7 +222,7 @@ class AdjointGradient:
         self.symbols = self.prog.symbols
         self.n = self.prog.n
         self.dev_fwd = engine.DeviceProgram(self.prog.fwd, self.device)
-        self.dev_rev = engine.DeviceProgram(self.prog.rev, self.device)
+        self.dev_rev = engine.DeviceProgram(self.prog.rev, self.device, adjoint=True)
         self.ws = engine.Workspace(self.device)
         self.psi = engine.new_state(self.n, self.device)
         self.phi = torch.empty_like(self.psi)

@@ -115,11 +115,182 @@ bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const 
   return true;
 }
 
+// A DIAG / PAIRPHASE position code: a tile bit (< L) or kOuter + a flat position outside the tile.
+static bool valid_code(int c, int L, int n, uint64_t local) {
+  if (c >= 0 && c < L) return true;
+  const int g = c - kOuter;
+  return c >= kOuter && g < n && !((local >> g) & 1ull);
+}
+
+// Every op of one pass: kinds, target bits, matrix / aux extents, controls and outer-bit variants
+// (the checks the kernels' indexing assumes).  mats_c: complex entries in the matrix buffer.
+bool validate_pass_ops(int n, const cqs_pass& ps, const cqs_op* ops, int32_t n_ops, int64_t mats_c,
+                       const int32_t* aux, int64_t n_aux, bool adjoint) {
+  const int L = ps.L;
+  if (ps.op_begin < 0 || ps.op_count < 0 || (int64_t)ps.op_begin + ps.op_count > n_ops) {
+    set_error("pass ops [%d, %d) outside the %d-op table", ps.op_begin, ps.op_begin + ps.op_count, n_ops);
+    return false;
+  }
+  uint64_t local = 0;
+  for (int b = 0; b < L; ++b) local |= 1ull << ps.lpos[b];
+  const uint64_t nmask = n >= 64 ? ~0ull : ((1ull << n) - 1);
+  for (int i = ps.op_begin; i < ps.op_begin + ps.op_count; ++i) {
+    const cqs_op& o = ops[i];
+    const int kind = o.kind & 0xff, sel = (o.kind >> 8) & 3;
+    if ((o.kind & ~0x3ff) != 0 || sel == 3) {
+      set_error("op %d: kind word 0x%x has bits set outside the kind and tile selector", i, o.kind);
+      return false;
+    }
+    if (sel != 0 && !adjoint) {
+      set_error("op %d: tile selector bits (kind word 0x%x) belong to adjoint programs only", i, o.kind);
+      return false;
+    }
+    if ((kind == CQS_OP_GRAD_DENSE || kind == CQS_OP_GRAD_DIAG) && !adjoint) {
+      set_error("op %d: gradient ops belong to adjoint programs only", i);
+      return false;
+    }
+    const uint64_t com = ((uint64_t)o.co_mask_hi << 32) | o.co_mask_lo;
+    const uint64_t cov = ((uint64_t)o.co_val_hi << 32) | o.co_val_lo;
+    if ((com & ~nmask) || (com & local) || (cov & ~com)) {
+      set_error("op %d: outer controls must name bits outside the tile (mask 0x%llx)", i, (unsigned long long)com);
+      return false;
+    }
+    if (L < 31 && ((o.cl_mask & ~((1 << L) - 1)) || (o.cl_val & ~o.cl_mask))) {
+      set_error("op %d: local controls outside the %d-bit tile", i, L);
+      return false;
+    }
+    auto need_mats = [&](int64_t count) {
+      if (o.mat < 0 || (int64_t)o.mat + count > mats_c) {
+        set_error("op %d: matrix entries [%d, %lld) outside the %lld-entry buffer", i, o.mat,
+                  (long long)(o.mat + count), (long long)mats_c);
+        return false;
+      }
+      return true;
+    };
+    switch (kind) {
+      case CQS_OP_DENSE:
+      case CQS_OP_BLOCK:
+      case CQS_OP_GRAD_DENSE: {
+        const bool block = kind == CQS_OP_BLOCK;
+        if (block ? (o.k < 4 || o.k > 5) : (o.k < 1 || o.k > 3)) {
+          set_error("op %d: k=%d out of range for its kind %d", i, o.k, kind);
+          return false;
+        }
+        if (kind == CQS_OP_DENSE && o.k > ps.max_k) {
+          set_error("op %d: dense k=%d exceeds the pass's max_k=%d (the LDS matrix stage)", i, o.k, ps.max_k);
+          return false;
+        }
+        int seen = 0;
+        for (int m = 0; m < o.k; ++m) {
+          if (o.t[m] < 0 || o.t[m] >= L || ((seen >> o.t[m]) & 1)) {
+            set_error("op %d: targets must be distinct tile bits < %d", i, L);
+            return false;
+          }
+          seen |= 1 << o.t[m];
+        }
+        if (seen & o.cl_mask) {
+          set_error("op %d: a local control is also a target", i);
+          return false;
+        }
+        const int code = o.t[5], cnt = (code >> 24) & 3;
+        if (code & ~((3 << 24) | ((1 << 21) - 1))) {
+          set_error("op %d: bad variant code 0x%x", i, code);
+          return false;
+        }
+        for (int v = 0; v < cnt; ++v) {
+          const int pos = (code >> (7 * v)) & 127;
+          if (pos >= n || ((local >> pos) & 1ull)) {
+            set_error("op %d: variant bit %d is not an outer bit", i, pos);
+            return false;
+          }
+        }
+        const int64_t per = block ? (int64_t)2 << (2 * o.k) : (int64_t)1 << (2 * o.k);
+        if (block && (o.mat & 1)) {
+          set_error("op %d: MFMA block operands must start 16-byte aligned", i);
+          return false;
+        }
+        if (!need_mats(per << cnt)) return false;
+        break;
+      }
+      case CQS_OP_DIAG:
+      case CQS_OP_GRAD_DIAG:
+        if (o.k < 1 || o.k > 6) {
+          set_error("op %d: diagonal on k=%d bits", i, o.k);
+          return false;
+        }
+        for (int m = 0; m < o.k; ++m)
+          if (!valid_code(o.t[m], L, n, local)) {
+            set_error("op %d: diagonal bit code %d", i, o.t[m]);
+            return false;
+          }
+        if (!need_mats((int64_t)1 << o.k)) return false;
+        break;
+      case CQS_OP_PAIRPHASE:
+        if (o.k < 1 || aux == nullptr || o.t[0] < 0 || (int64_t)o.t[0] + 2 * o.k > n_aux) {
+          set_error("op %d: %d pairs at aux[%d] outside the %lld-entry aux table", i, o.k, o.t[0], (long long)n_aux);
+          return false;
+        }
+        for (int m = 0; m < 2 * o.k; ++m)
+          if (!valid_code(aux[o.t[0] + m], L, n, local)) {
+            set_error("op %d: pair bit code %d", i, aux[o.t[0] + m]);
+            return false;
+          }
+        if (!need_mats(o.k)) return false;
+        break;
+      case CQS_OP_SCALE:
+        if (!need_mats(1)) return false;
+        break;
+      case CQS_OP_PERMUTE: {
+        if (adjoint || o.k != L || aux == nullptr || o.t[0] < 0 || (int64_t)o.t[0] + L > n_aux) {
+          set_error("op %d: PERMUTE needs k = L and L aux entries (forward programs only)", i);
+          return false;
+        }
+        int seen = 0;
+        for (int b = 0; b < L; ++b) {
+          const int d = aux[o.t[0] + b];
+          if (d < 0 || d >= L || ((seen >> d) & 1)) {
+            set_error("op %d: PERMUTE destinations are not a permutation of the tile bits", i);
+            return false;
+          }
+          seen |= 1 << d;
+        }
+        break;
+      }
+      default:
+        set_error("op %d: unknown kind %d", i, kind);
+        return false;
+    }
+  }
+  return true;
+}
+
 }  // namespace cqs
 
 using namespace cqs;
 
 extern "C" {
+
+int cqs_program_validate(int32_t n_bits, const cqs_pass* passes, int32_t n_passes, const cqs_op* ops, int32_t n_ops,
+                         int64_t mats_complex, const int32_t* aux, int64_t n_aux, int32_t adjoint) {
+  CQS_REQUIRE(n_bits >= 0 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
+  CQS_REQUIRE(n_passes >= 0 && (n_passes == 0 || passes != nullptr), "bad pass list");
+  CQS_REQUIRE(n_ops >= 0 && (n_ops == 0 || ops != nullptr), "bad op table");
+  for (int i = 0; i < n_passes; ++i) {
+    const cqs_pass& ps = passes[i];
+    if (ps.op_count <= 0) continue;
+    TileProg P;
+    if (!make_prog(P, n_bits, ps, ops, nullptr, aux)) return CQS_ERR_ARG;
+    if (adjoint)
+      for (int b = 0; b < ps.L; ++b)
+        CQS_REQUIRE(ps.lpos_out[b] == ps.lpos[b], "pass %d: adjoint passes cannot re-lay their tiles", i);
+    if (!validate_pass_ops(n_bits, ps, ops, n_ops, mats_complex, aux, n_aux, adjoint != 0)) {
+      std::string msg = "pass " + std::to_string(i) + ": " + g_last_error;
+      g_last_error = msg;
+      return CQS_ERR_ARG;
+    }
+  }
+  return CQS_OK;
+}
 
 int cqs_abi_version(void) { return CQS_ABI_VERSION; }
 

@@ -238,7 +238,7 @@ class DMAdjointGradient:
         self.n = self.prog.n
         nb = self.prog.n_bits
         self.dev_fwd = engine.DeviceProgram(self.prog.fwd, self.device)
-        self.dev_rev = engine.DeviceProgram(self.prog.rev, self.device)
+        self.dev_rev = engine.DeviceProgram(self.prog.rev, self.device, adjoint=True)
         self.ws = engine.Workspace(self.device)
         self.rho = engine.new_state(nb, self.device)
         self.lam = torch.empty_like(self.rho)

@@ -25,13 +25,25 @@ def stream_handle() -> int:
     return torch.cuda.current_stream().cuda_stream
 
 
-class DeviceProgram:
-    """A packed pass program resident on the device; parameter matrices can be swapped."""
+def validate_program(packed: PackedProgram, adjoint: bool = False) -> None:
+    """cqs_program_validate on the host arrays (ValueError naming the first bad pass / op)."""
+    pa = np.ascontiguousarray(packed.pass_arr)
+    ops = np.ascontiguousarray(packed.ops_arr)
+    aux = np.ascontiguousarray(packed.aux_arr, dtype=np.int32)
+    check(lib().cqs_program_validate(int(packed.n_bits), pa.ctypes.data, int(packed.n_passes), ops.ctypes.data,
+                                     int(len(ops)), int(packed.mats_arr.size), aux.ctypes.data, int(aux.size),
+                                     int(bool(adjoint))), "cqs_program_validate")
 
-    def __init__(self, packed: PackedProgram, device):
+
+class DeviceProgram:
+    """A packed pass program resident on the device; parameter matrices can be swapped.  The
+    program is checked on the host before upload (validate_program)."""
+
+    def __init__(self, packed: PackedProgram, device, adjoint: bool = False):
         self.packed = packed
         self.n_bits = packed.n_bits
         self.device = device
+        validate_program(packed, adjoint)
         self.ops = torch.from_numpy(packed.ops_arr.view(np.int32).copy()).to(device)
         self.mats = torch.from_numpy(packed.mats_arr.view(np.float32).copy()).to(device)
         self.aux = torch.from_numpy(packed.aux_arr.copy()).to(device)
@@ -73,12 +85,26 @@ class DeviceProgram:
         )
 
 
+def check_state_fits(n_bits: int, device) -> None:
+    """MemoryError ("Unable to allocate", NumPy's wording, which the reference's simulators
+    raise) when 2^n_bits complex64 amplitudes exceed the device's memory."""
+    nbytes = 8 << n_bits
+    total = torch.cuda.get_device_properties(device).total_memory
+    if nbytes > total:
+        raise MemoryError(f"Unable to allocate {nbytes / 2**30:.3g} GiB for a state of 2^{n_bits} complex64 "
+                          f"amplitudes (the device holds {total / 2**30:.3g} GiB)")
+
+
 def empty_state(n_bits: int, device) -> torch.Tensor:
-    """Uninitialised storage for 2^n_bits complex64 amplitudes."""
+    """Uninitialised storage for 2^n_bits complex64 amplitudes; MemoryError when the device
+    cannot hold it."""
+    nbytes = 8 << n_bits
+    check_state_fits(n_bits, device)
     try:
         return torch.empty(1 << n_bits, dtype=torch.complex64, device=device)
-    except torch.OutOfMemoryError as e:  # the reference raises MemoryError
-        raise MemoryError(f"cannot allocate a state of 2^{n_bits} complex64 amplitudes") from e
+    except torch.OutOfMemoryError as e:
+        raise MemoryError(f"Unable to allocate {nbytes / 2**30:.3g} GiB for a state of 2^{n_bits} complex64 "
+                          "amplitudes") from e
 
 
 def new_state(n_bits: int, device, basis: int = 0) -> torch.Tensor:

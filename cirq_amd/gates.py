@@ -81,7 +81,10 @@ class Gate:
         return out
 
     def _has_unitary_(self) -> bool:
-        return not self._is_parameterized_() and hasattr(self, "_param_exprs")
+        """True when the gate defines its own matrix (a subclass that only decomposes does not)."""
+        cls = type(self)
+        own = cls._matrix_batch is not Gate._matrix_batch or cls._unitary_ is not Gate._unitary_
+        return own and not self._is_parameterized_()
 
     def _is_parameterized_(self) -> bool:
         return any(is_parameterized(v) for v in self._param_exprs())

@@ -24,7 +24,7 @@ import torch
 
 from . import engine
 from .channels import NoiseModel
-from .circuits import Circuit, Moment
+from .circuits import Circuit, Moment, expand_for_simulation
 from .gates import is_parameterized
 from .paulis import PauliSum, pauli_masks
 from .qubits import QubitOrder
@@ -132,6 +132,69 @@ def product_state_factors(circuit, qubits) -> list[list]:
     return out
 
 
+def valid_state_vector(state, n: int, atol: float = 1e-7):
+    """An initial state vector checked as Cirq checks it (to_valid_state_vector /
+    validate_normalized_state_vector, cirq-core/cirq/qis/states.py:766-897): a 1-D array of
+    per-qubit basis values (integer or bool dtype, length n) names a basis state (its index is
+    returned); otherwise the amplitudes, cast to complex64, must number 2^n and have
+    sum |a|^2 close to 1 (np.isclose with atol 1e-7), else ValueError.  The pass kernels rely on
+    this: split-FP16 MFMA blocks scale amplitudes by 2^15 and need |a| <= 1."""
+    data = np.asarray(state)
+    if data.ndim == 1 and data.dtype.kind != "c":
+        if n == 2**n:
+            raise ValueError("Because len(qid_shape) == product(qid_shape), it is ambiguous whether the given state "
+                             "contains state vector amplitudes or per-qudit computational basis values. In this "
+                             "situation you are required to pass in a state vector that is a numpy array with a "
+                             "complex dtype.")
+        if data.shape == (n,):
+            if data.dtype.kind not in "?bBiu":
+                raise ValueError(f"Expected a bool or int entry for each qudit in `state`, because len(state) == "
+                                 f"len(qid_shape), but got dtype {data.dtype}.")
+            idx = 0
+            for i, v in enumerate(data.tolist()):
+                if not 0 <= int(v) < 2:
+                    raise ValueError(f"Qudit value {v} at index {i} is out of bounds for qudit dimension 2.")
+                idx = 2 * idx + int(v)
+            return idx
+    data = data.astype(np.complex64, casting="unsafe").reshape(-1)
+    if data.size != 2**n:
+        raise ValueError(f"state_vector has incorrect size. Expected {2**n} but was {data.size}.")
+    norm = np.sum(np.abs(data) ** 2)
+    if not np.isclose(norm, 1, atol=atol):
+        raise ValueError(f"State_vector is not normalized instead had norm {norm}")
+    return data
+
+
+def valid_density_matrix(state, n: int, atol: float = 1e-7):
+    """An initial density matrix checked as Cirq checks it (to_valid_density_matrix /
+    validate_density_matrix, cirq-core/cirq/qis/states.py:933-1035): a (2^n, 2^n) (or (2,)*2n)
+    array must be Hermitian, of trace 1 and positive semidefinite (eigenvalues > -atol); anything
+    else is taken as a state vector (or per-qubit basis values) and becomes |psi><psi|.  The
+    eigenvalue check is a host eigendecomposition, skipped above 2^12 x 2^12."""
+    N = 2**n
+    if isinstance(state, np.ndarray):
+        rho = state
+        if n > 1 and rho.shape == (2,) * (2 * n):
+            rho = rho.reshape(N, N)
+        if rho.shape == (N, N):
+            rho = rho.astype(np.complex64, casting="unsafe")
+            if not np.allclose(rho, rho.conj().T, atol=atol):
+                raise ValueError("The density matrix is not Hermitian.")
+            trace = np.trace(rho)
+            if not np.isclose(trace, 1.0, atol=atol):
+                raise ValueError(f"Density matrix does not have trace 1. Instead, it has trace {trace}.")
+            if N <= 4096 and not np.all(np.linalg.eigvalsh(rho) > -atol):
+                raise ValueError("The density matrix is not positive semidefinite. This may happen for simulations "
+                                 "using lower-precision `dtype` such as `np.complex64`. Try using `np.complex128` in "
+                                 "such a case.")
+            return rho
+    v = valid_state_vector(state, n, atol)
+    if not isinstance(v, np.ndarray):
+        return v
+    v = v.astype(np.complex128)
+    return np.outer(v, np.conj(v))
+
+
 class _Compiled:
     """A circuit lowered, scheduled and uploaded for one qubit order."""
 
@@ -213,6 +276,7 @@ class _SimulatorBase:
         self._traj_batch = trajectory_batch
         self._cache: "collections.OrderedDict" = collections.OrderedDict()
         self._split_memo: dict = {}
+        self._noisy_memo: dict = {}
         self._device = None
         self._sampler = None
         self._ws = None
@@ -226,10 +290,24 @@ class _SimulatorBase:
         return self._device
 
     def _noisy(self, circuit) -> Circuit:
-        if self.noise is None or type(self.noise).__name__ == "_NoNoise":
-            return circuit
-        qubits = sorted(circuit.all_qubits())
-        return Circuit.from_moments(*self.noise.noisy_moments(list(circuit), qubits))
+        """The circuit as simulated: sub-circuits and ops that only decompose expanded
+        (circuits.expand_for_simulation), then the noise model's moments.  Memoised per circuit
+        object, so repeated calls on one program reuse its compiled form."""
+        hit = self._noisy_memo.get(id(circuit))
+        if hit is not None and hit[0]() is circuit:
+            return hit[1]
+        out = expand_for_simulation(circuit)
+        if not (self.noise is None or type(self.noise).__name__ == "_NoNoise"):
+            qubits = sorted(out.all_qubits())
+            out = Circuit.from_moments(*self.noise.noisy_moments(list(out), qubits))
+        if out is not circuit:
+            try:
+                self._noisy_memo[id(circuit)] = (weakref.ref(circuit), out)
+            except TypeError:
+                return out
+            while len(self._noisy_memo) > 16:
+                self._noisy_memo.pop(next(iter(self._noisy_memo)))
+        return out
 
     def _tile_params(self, n_bits: int) -> tuple[int, int]:
         if self._tile_bits is not None:
@@ -246,6 +324,7 @@ class _SimulatorBase:
             self._cache.move_to_end(key)
             return hit[1]
         nb = 2 * len(qubits) if self._dm else len(qubits)
+        engine.check_state_fits(nb, self._dev())
         L, c = self._tile_params(nb)
         comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse, self._blocks, self._tuner)
         try:
@@ -271,16 +350,12 @@ class _SimulatorBase:
         dev = self._dev()
         n = comp.n
         if initial_state is not None and not isinstance(initial_state, (int, np.integer)):
-            arr = np.asarray(initial_state)
             if self._dm:
-                if arr.shape == (2**n,):
-                    v = arr.astype(np.complex128)
-                    arr = np.outer(v, np.conj(v))
-                arr = arr.reshape(2**n, 2**n)
+                arr = valid_density_matrix(initial_state, n)
             else:
-                arr = arr.reshape(-1)
-                if arr.size != 2**n:
-                    raise ValueError(f"initial_state has {arr.size} amplitudes, expected {2**n}")
+                arr = valid_state_vector(initial_state, n)
+            if not isinstance(arr, np.ndarray):  # per-qubit basis values
+                return self._prepare(comp, int(arr))
             return torch.from_numpy(np.ascontiguousarray(arr.reshape(-1), dtype=np.complex64)).to(dev)
         b = self._initial_index(initial_state, n)
         if self._dm:
@@ -289,16 +364,18 @@ class _SimulatorBase:
 
     def _run_program(self, comp: _Compiled, resolvers: list, initial_state) -> Iterator[torch.Tensor]:
         mats = comp.mats_for(resolvers) if comp.has_params else None
-        basis = None
+        basis = init = None
         if initial_state is None or isinstance(initial_state, (int, np.integer)):
             basis = self._initial_index(initial_state, comp.n)
             if self._dm:
                 basis = basis * (2**comp.n) + basis
+        else:  # validated and uploaded once for all resolvers
+            init = self._prepare(comp, initial_state)
         for i, _ in enumerate(resolvers):
             if mats is not None:
                 comp.dev.set_mats(mats[i])
             if basis is None:
-                st = self._prepare(comp, initial_state)
+                st = init.clone() if i < len(resolvers) - 1 else init
                 comp.dev.run(st)
             else:  # the first pass builds the basis state itself
                 st = engine.empty_state(comp.n_bits, self._dev())
@@ -810,14 +887,21 @@ class _SimulatorBase:
         return self.compute_amplitudes_sweep(program, bitstrings, param_resolver, qubit_order)[0]
 
     def compute_amplitudes_sweep(self, program, bitstrings, params, qubit_order=QubitOrder.DEFAULT) -> list:
+        return list(self.compute_amplitudes_sweep_iter(program, bitstrings, params, qubit_order))
+
+    def compute_amplitudes_sweep_iter(self, program, bitstrings, params, qubit_order=QubitOrder.DEFAULT):
+        """Amplitudes of the final state at the given big-endian bitstrings, per resolver
+        (cirq-core/cirq/sim/state_vector_simulator.py:74-98): read on the GPU from the final
+        state, with the same renormalisation final_state_vector applies."""
         if self._dm:
             raise NotImplementedError("amplitudes are defined for state vectors only")
-        out = []
+        if isinstance(bitstrings, np.ndarray) and len(bitstrings.shape) > 1:
+            raise ValueError("The list of bitstrings must be input as a 1-dimensional array of ints. Got an array "
+                             f"with shape {bitstrings.shape}.")
         idx = torch.tensor([int(b) for b in bitstrings], dtype=torch.int64)
         for res in self.simulate_sweep_iter(program, params, qubit_order):
             amps = res._state[idx.to(res._state.device)].cpu().numpy()
-            out.append(list(amps))
-        return out
+            yield (amps * np.float32(res._renorm_factor())).tolist()
 
 
 def _split_terminal_measurements(circuit):
@@ -852,25 +936,36 @@ class StateVectorTrialResult:
         self.qubit_map = {q: i for i, q in enumerate(self._qubits)}
         self._ws = ws
         self._fsv = None
+        self._scale = None
+        self._scaled = None
+
+    def _renorm_factor(self) -> float:
+        """1/norm when rescaling brings the total probability closer to 1, else 1
+        (cirq-core/cirq/sim/state_vector_simulator.py:131-146); computed once, in FP64 on the GPU."""
+        if self._scale is None:
+            n = len(self._qubits)
+            n2 = engine.norm2(self._state, n, self._ws)
+            norm = math.sqrt(n2)
+            eps = math.sqrt(np.finfo(np.complex64).eps)
+            self._scale = 1.0
+            if abs(norm - 1) > eps:
+                warnings.warn(f"final state vector's norm={norm} is too far from 1, {abs(norm - 1)} > {eps}.")
+            else:
+                t = self._state.clone()
+                engine.scale(t, n, 1.0 / norm)
+                n2b = engine.norm2(t, n, self._ws)
+                if abs(n2b - 1) < abs(n2 - 1):
+                    self._scale = 1.0 / norm
+                    self._scaled = t
+        return self._scale
 
     @property
     def final_state_vector(self) -> np.ndarray:
         """Copied to the host; renormalised only when that improves the total probability."""
         if self._fsv is None:
-            n = len(self._qubits)
-            n2 = engine.norm2(self._state, n, self._ws)
-            norm = math.sqrt(n2)
-            eps = math.sqrt(np.finfo(np.complex64).eps)
-            ret = self._state
-            if abs(norm - 1) > eps:
-                warnings.warn(f"final state vector's norm={norm} is too far from 1, {abs(norm - 1)} > {eps}.")
-            else:
-                t = ret.clone()
-                engine.scale(t, n, 1.0 / norm)
-                n2b = engine.norm2(t, n, self._ws)
-                if abs(n2b - 1) < abs(n2 - 1):
-                    ret = t
+            ret = self._scaled if self._renorm_factor() != 1.0 else self._state
             self._fsv = ret.cpu().numpy()
+            self._scaled = None
         return self._fsv
 
     def state_vector(self, copy: bool = False) -> np.ndarray:

@@ -174,6 +174,14 @@ class Lowering:
         if _is_pauli_string(op) and _gate_of(op) is None:
             self._lower_pauli_string(op)
             return
+        from .circuits import _decomposed, _needs_decompose
+
+        if _needs_decompose(op):  # sub-circuits, ops that only decompose, wide decomposable gates
+            subs = _decomposed(op)
+            if subs != [op]:
+                for sub in subs:
+                    self.lower_op(sub)
+                return
         g = _gate_of(op)
         if g is None:
             raise UnsupportedOperation(f"cannot simulate {op!r}: not a gate operation")

@@ -145,6 +145,20 @@ int cqs_run_passes(void* amps, int32_t n_bits, const cqs_pass* passes, int32_t n
                    const cqs_op* ops, const void* mats, const int32_t* aux, void* stream);
 
 /*
+ * Host-side check of a pass program (HOST copies of passes, ops and aux) before it is uploaded
+ * for cqs_run_passes / cqs_run_passes_from_basis (adjoint = 0) or cqs_adjoint_passes
+ * (adjoint = 1): pass layouts, op kinds, target bits (distinct, inside the tile), local and
+ * outer controls, outer-bit matrix variants, DENSE k against the pass's max_k, matrix extents
+ * against mats_complex entries, PAIRPHASE / PERMUTE entries against n_aux.  The kernels trust
+ * every one of these; a program that fails here would read or write out of bounds.  Returns
+ * CQS_ERR_ARG with the first problem in cqs_last_error.  The program takes the place of the
+ * op-by-op dispatch of cirq-core/cirq/sim/simulator_base.py:198-212, whose Python objects need
+ * no such check.
+ */
+int cqs_program_validate(int32_t n_bits, const cqs_pass* passes, int32_t n_passes, const cqs_op* ops, int32_t n_ops,
+                         int64_t mats_complex, const int32_t* aux, int64_t n_aux, int32_t adjoint);
+
+/*
  * cqs_init_basis(amps, n_bits, basis) followed by cqs_run_passes, with the initial state never
  * written or read: the first pass that runs builds its tiles of |basis> in registers (amps may
  * hold anything on entry).  With no pass to run it is cqs_init_basis.  The same initial states as

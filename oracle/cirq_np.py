@@ -263,6 +263,96 @@ def apply_channel(rho: np.ndarray, kraus: list, axes, n: int) -> np.ndarray:
     return out
 
 
+# ------------------------------------------------------------------------------ gates' own strategies
+def subspace_index(axes, little_endian_bits: int, ndim: int):
+    """Index selecting the target axes equal to the bits of `little_endian_bits`, the LEAST
+    significant bit for the FIRST axis (cirq-core/cirq/protocols/apply_unitary_protocol.py:184-214)."""
+    idx = [slice(None)] * ndim
+    for j, a in enumerate(axes):
+        idx[a] = (little_endian_bits >> j) & 1
+    return tuple(idx)
+
+
+def _global_phase(exponent: float, global_shift: float):
+    return 1j ** (2 * exponent * global_shift)
+
+
+def gate_apply_unitary(t: np.ndarray, family: str, params: dict, axes):
+    """A gate's own ``_apply_unitary_`` where Cirq defines one for the exponents it accepts; None
+    (NotImplemented) otherwise, and the caller falls back to the matrix strategy.
+
+    XPowGate   cirq-core/cirq/ops/common_gates.py:123-133 (exponent 1: swap of the slices)
+    YPowGate   cirq-core/cirq/ops/common_gates.py:454-464 (exponent 1)
+    ZPowGate   cirq-core/cirq/ops/common_gates.py:658-669 (any exponent: phase on |1>); Rz is one
+    HPowGate   cirq-core/cirq/ops/common_gates.py:977-987 (exponent 1)
+    CZPowGate  cirq-core/cirq/ops/common_gates.py:1072-1082 (any exponent: phase on |11>)
+    CXPowGate  cirq-core/cirq/ops/common_gates.py:1257-1268 (exponent 1)
+    ZZPowGate  cirq-core/cirq/ops/parity_gates.py:253-265 (any exponent: phase on |01>, |10>)
+    The result is a new array when Cirq writes its ``available_buffer``, else ``t`` in place.
+    """
+    nd = t.ndim
+    axes = list(axes)
+    if family == "Rz":
+        family, params = "ZPowGate", {"exponent": params["rads"] / np.pi, "global_shift": -0.5}
+    e = float(params.get("exponent", 1.0))
+    s = float(params.get("global_shift", 0.0))
+    p = _global_phase(e, s)
+    if family in ("XPowGate", "YPowGate") and e == 1:
+        zero, one = subspace_index(axes, 0, nd), subspace_index(axes, 1, nd)
+        buf = np.empty_like(t)
+        if family == "XPowGate":
+            buf[zero] = t[one]
+            buf[one] = t[zero]
+        else:
+            buf[zero] = -1j * t[one]
+            buf[one] = 1j * t[zero]
+        if p != 1:
+            buf *= p
+        return buf
+    if family == "ZPowGate":
+        t[subspace_index(axes, 1, nd)] *= 1j ** (e * 4 * 1 / 2)
+        if p != 1:
+            t *= p
+        return t
+    if family == "HPowGate" and e == 1:
+        zero, one = subspace_index(axes, 0, nd), subspace_index(axes, 1, nd)
+        t[one] -= t[zero]
+        t[one] *= -0.5
+        t[zero] -= t[one]
+        t *= np.sqrt(2) * p
+        return t
+    if family == "CZPowGate":
+        t[subspace_index(axes, 0b11, nd)] *= 1j ** (2 * e)
+        if p != 1:
+            t *= p
+        return t
+    if family == "CXPowGate" and e == 1:
+        oo, zo = subspace_index(axes, 0b11, nd), subspace_index(axes, 0b01, nd)
+        keep = np.copy(t[oo])
+        t[oo] = t[zo]
+        t[zo] = keep
+        if p != 1:
+            t *= p
+        return t
+    if family == "ZZPowGate":
+        if p != 1:
+            t *= p
+        rel = 1j ** (2 * e)
+        t[subspace_index(axes, 0b10, nd)] *= rel
+        t[subspace_index(axes, 0b01, nd)] *= rel
+        return t
+    return None
+
+
+def apply_gate(t: np.ndarray, family: str, params: dict, axes) -> np.ndarray:
+    """Cirq's strategy order for a gate (cirq-core/cirq/protocols/apply_unitary_protocol.py:375-389):
+    the gate's own ``_apply_unitary_`` first, then its matrix."""
+    r = gate_apply_unitary(t, family, params, axes)
+    if r is not None:
+        return r
+    return apply_unitary(t, gate_unitary(family, params), list(axes))
+
+
 # ------------------------------------------------------------------------------ simulation
 def op_matrix(op) -> np.ndarray:
     kind, family, params, axes = op
@@ -280,7 +370,7 @@ def simulate_state_vector(ops, n: int, dtype=np.complex128, initial_state: int =
         if family == "GlobalPhase":
             t = t * params["coefficient"]
             continue
-        t = apply_unitary(t, gate_unitary(family, params), list(axes))
+        t = apply_gate(t, family, params, axes)
     return t.reshape(-1)
 
 
