@@ -42,18 +42,19 @@ def parse():
                     help="compile the main circuit with the default tile / fusion settings instead of timing candidates")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-qubits", type=int, default=26)
-    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
     ap.add_argument("--adjoint-qubits", type=int, default=20)
     ap.add_argument("--adjoint-layers", type=int, default=10)
     ap.add_argument("--adjoint-reps", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (reference for HBM byte counters) and exit")
-    ap.add_argument("--layout-probe", action="store_true",
-                    help="time load/store-only passes over tile layouts and the grid program's load/store floor")
+    ap.add_argument("--no-qaoa", action="store_true", help="skip the QAOA gradient leg")
+    ap.add_argument("--qaoa-nodes", type=int, default=24)
+    ap.add_argument("--qaoa-p", type=int, default=8)
     ap.add_argument("--no-dm", action="store_true", help="skip the noisy density-matrix leg")
     ap.add_argument("--dm-qubits", type=int, default=16)
     ap.add_argument("--dm-layers", type=int, default=20)
+    ap.add_argument("--dm-cpu-qubits", type=int, default=12)
     ap.add_argument("--no-q32", action="store_true", help="skip the 32-qubit depth-20 grid leg")
     ap.add_argument("--q32-steps", type=int, default=2)
     ap.add_argument("--no-dm-adjoint", action="store_true", help="skip the noisy density-matrix gradient leg")
@@ -110,7 +111,7 @@ def dm_leg(args):
     bytes_per_pass = (2 * n_pass - 1) * 8 * 4**n / max(n_pass, 1)  # the first pass only writes
     del rho
     torch.cuda.empty_cache()
-    return {
+    out = {
         "workload": f"noisy line {n} qubits x {args.dm_layers} layers, density matrix 2^{2 * n} complex64, "
         f"{sum(1 for _ in circuit.all_operations())} operations incl. channels",
         "layers_per_s": args.dm_layers / wall,
@@ -119,6 +120,43 @@ def dm_leg(args):
         "pass_avg_ms": ms / max(n_pass, 1),
         "pass_hbm_gbs": bytes_per_pass / (ms / 1e3 / max(n_pass, 1)) / 1e9,
         "tr_rho_sum_z": ev,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = dm_cpu_baseline(args.dm_cpu_qubits)
+    return out
+
+
+def dm_cpu_baseline(n: int) -> dict:
+    """The oracle's density-matrix step (apply_channel with left / right Kraus products, complex64,
+    one NumPy thread) at n qubits: each op kind of the noisy-line layer (Rz, Ry, CZ, depolarize,
+    amplitude damping) timed once on the first layer's ops (median of 3 after a warm-up); a
+    layer's time is their sum weighted by the layer's op counts."""
+    from oracle import cirq_np as O
+    from tests.helpers import to_oracle_ops
+    from cirq_amd import generators
+
+    circuit = generators.noisy_line_circuit(n, 1, seed=0)
+    ops = to_oracle_ops(circuit, sorted(circuit.all_qubits()))
+    rho = np.zeros((2**n, 2**n), dtype=np.complex64)
+    rho[0, 0] = 1
+    state = {"t": rho.reshape((2,) * (2 * n))}
+    kinds, counts = {}, {}
+    for op in ops:
+        kinds.setdefault(op[1], op)
+        counts[op[1]] = counts.get(op[1], 0) + 1
+
+    def apply(op):
+        kind, fam, params, axes = op
+        ks = [O.gate_unitary(fam, params)] if kind == "gate" else O.channel_kraus(fam, params)
+        state["t"] = O.apply_channel(state["t"], [k.astype(np.complex64) for k in ks], list(axes), n)
+
+    per = {fam: _median_s(lambda op=op: apply(op), 3) for fam, op in kinds.items()}
+    layer = sum(per[f] * counts[f] for f in per)
+    return {
+        "value": 1.0 / layer, "unit": "layers/s", "cores": 1, "kind": "port",
+        "sample": f"noisy-line layer at {n} qubits (2^{2 * n}-entry density matrix) through the oracle's "
+        f"apply_channel: per-op medians {', '.join(f'{f} {t * 1e3:.0f} ms x{counts[f]}' for f, t in per.items())}; "
+        f"{layer:.2f} s per layer ({_cpu_threads()})",
     }
 
 
@@ -189,7 +227,8 @@ def sweep_leg(args):
     # looped: the first point of each of the first K circuits, one at a time
     K = min(args.sweep_looped_circuits, len(circuits))
     ags = [AdjointGradient(circuits[c], PauliSum.from_pauli_strings(observables[c])) for c in range(K)]
-    ags[0].value_and_grad(values[0][0])
+    for c in range(K):  # every looped circuit warmed (first launches, workspace growth) before timing
+        ags[c].value_and_grad(values[c][1 % values[c].shape[0]])
     torch.cuda.synchronize()
     diffs = []
     t0 = time.perf_counter()
@@ -242,7 +281,7 @@ def adjoint_leg(args):
     torch.cuda.synchronize()
     t_ps = time.perf_counter() - t0
     G = len(syms)
-    return {
+    out = {
         "workload": f"HEA {args.adjoint_qubits} qubits x {args.adjoint_layers} layers, {G} symbols, "
         f"{len(list(psum))} Pauli terms",
         "adjoint_ms": t_adj * 1e3,
@@ -255,6 +294,95 @@ def adjoint_leg(args):
         "forward_passes": int(ag.prog.fwd.n_passes),
         "reverse_passes": int(ag.prog.rev.n_passes),
     }
+    if not args.no_cpu_baseline:
+        # Cirq has no adjoint: its gradient is the shift rule, 2 simulations per symbol
+        n_layers = args.adjoint_layers
+        layer = _oracle_layer_seconds(circuit, syms, vals, moments=slice(0, len(circuit) // n_layers))
+        t_cpu = 2 * G * n_layers * layer
+        out["cpu_baseline"] = {
+            "value": G / t_cpu, "unit": "gradients/s", "cores": 1, "kind": "port",
+            "sample": f"one ansatz layer at {args.adjoint_qubits} qubits through the oracle (median of 3 after a "
+            f"warm-up): {layer:.3f} s; the shift rule's {2 * G} simulations of {n_layers} layers scaled from it "
+            f"(expectation values not counted; {_cpu_threads()})",
+        }
+    return out
+
+
+def _oracle_layer_seconds(circuit, syms, vals, moments: slice) -> float:
+    """Median seconds (3 runs after a warm-up) of the oracle applying the ops of circuit[moments]
+    at the given symbol values, complex64, from |0...0>."""
+    from oracle import cirq_np as O
+    from tests.helpers import to_oracle_ops
+    from cirq_amd.study import ParamResolver
+
+    qs = sorted(circuit.all_qubits())
+    n = len(qs)
+    ops = to_oracle_ops(circuit[moments], qs, ParamResolver(dict(zip(syms, vals))))
+    psi = np.zeros(2**n, dtype=np.complex64)
+    psi[0] = 1
+    state = {"t": psi.reshape((2,) * n)}
+
+    def run(sel=ops):
+        t = state["t"]
+        for _, fam, params, axes in sel:
+            t = O.apply_gate(t, fam, params, axes)
+        state["t"] = t
+
+    return _median_s(run, 3, warm=lambda: run(ops[:2]))
+
+
+def qaoa_leg(args):
+    """QAOA MaxCut on a seeded random 3-regular graph of 24 nodes, p = 8 (16 symbols, 504 gates),
+    H = sum_edges (1 - Z_u Z_v) / 2: gradients/s of one adjoint sweep vs the GPU parameter-shift
+    rule (two simulations per parameter occurrence: 960), and their max difference; the CPU
+    baseline is the oracle's central differences (2 simulations per symbol, 32)."""
+    import torch
+
+    from cirq_amd import generators
+    from cirq_amd.adjoint import AdjointGradient
+
+    circuit, syms, vals, ham, edges = generators.qaoa_maxcut(args.qaoa_nodes, args.qaoa_p, seed=0)
+    ag = AdjointGradient(circuit, ham)
+    point = dict(zip(syms, vals))
+    ag.value_and_grad(point)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.adjoint_reps):
+        v, g = ag.value_and_grad(point)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / args.adjoint_reps
+    ag.parameter_shift(point)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v2, g2 = ag.parameter_shift(point)
+    torch.cuda.synchronize()
+    t_ps = time.perf_counter() - t0
+    G = len(syms)
+    out = {
+        "workload": f"QAOA MaxCut, random 3-regular graph of {args.qaoa_nodes} nodes ({len(edges)} edges), "
+        f"p = {args.qaoa_p}, {G} symbols, {sum(1 for _ in circuit.all_operations())} gates",
+        "value": v,
+        "adjoint_ms": t_adj * 1e3,
+        "adjoint_grads_per_s": G / t_adj,
+        "param_shift_ms": t_ps * 1e3,
+        "param_shift_evaluations": 2 * int(ag.prog.n_slots),
+        "speedup": t_ps / t_adj,
+        "max_abs_grad_diff_vs_param_shift": float(np.abs(g - g2).max()),
+        "forward_passes": int(ag.prog.fwd.n_passes),
+        "reverse_passes": int(ag.prog.rev.n_passes),
+    }
+    if not args.no_cpu_baseline:
+        per_layer = (len(circuit) - 1) // args.qaoa_p  # moments of one (ZZ layer, Rx layer) round
+        layer = _oracle_layer_seconds(circuit, syms, vals, moments=slice(1, 1 + per_layer))
+        t_sim = (args.qaoa_p + 1.0 / 3) * layer  # the H layer costs about a third of a round
+        t_cpu = 2 * G * t_sim
+        out["cpu_baseline"] = {
+            "value": G / t_cpu, "unit": "gradients/s", "cores": 1, "kind": "port",
+            "sample": f"one QAOA round (36 ZZ + {args.qaoa_nodes} Rx) at {args.qaoa_nodes} qubits through the oracle "
+            f"(median of 3 after a warm-up): {layer:.2f} s; central differences' {2 * G} simulations scaled from it "
+            f"(expectation values not counted; {_cpu_threads()})",
+        }
+    return out
 
 
 def dm_adjoint_leg(args):
@@ -324,41 +452,66 @@ def dm_adjoint_leg(args):
     }
 
 
-def cpu_baseline(args, circuit_ops_fn, n_gpu: int, layers: int, ops_per_layer: float):
-    """The oracle (Cirq's algorithm restated in NumPy, complex64) on a bounded sample."""
+def _median_s(fn, runs: int = 3, warm=None) -> float:
+    """Median wall time of `runs` calls after one warm-up call (`warm`, default `fn`;
+    time.perf_counter)."""
+    (warm or fn)()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def _cpu_threads() -> str:
+    return (f"{len(os.sched_getaffinity(0))} cores visible, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}, "
+            f"OPENBLAS_NUM_THREADS={os.environ.get('OPENBLAS_NUM_THREADS', 'unset')}")
+
+
+def cpu_baseline(args, rows: int, cols: int, depth: int, n_gpu: int):
+    """Cirq's algorithm restated in NumPy (the oracle: each gate's own _apply_unitary_ first, then
+    its matrix through slices or einsum), complex64, one thread of NumPy elementwise work, on a
+    bounded sample: the first cycle (one single-qubit layer and CZ layer A) of the same circuit
+    at args.cpu_qubits qubits, a warm-up (one single-qubit gate and one CZ) then the median of 3
+    runs, scaled by the state size to n_gpu qubits."""
     from oracle import cirq_np as O
+    from tests.helpers import to_oracle_ops
+    from cirq_amd import generators
 
     n = args.cpu_qubits
-    ops = circuit_ops_fn(n)
+    c = generators.grid_circuit(rows, cols, depth, seed=0, n_qubits=n)
+    moments = list(c)
+    cycle = to_oracle_ops(c[:2], sorted(c.all_qubits()))  # moments 0 and 1: one cycle
     psi = np.zeros(2**n, dtype=np.complex64)
     psi[0] = 1
-    t = psi.reshape((2,) * n)
-    # warm-up op
-    O.apply_unitary(t, O.gate_unitary(ops[0][1], ops[0][2]), list(ops[0][3]))
-    done = 0
-    t0 = time.perf_counter()
-    for op in ops:
-        t = O.apply_unitary(t, O.gate_unitary(op[1], op[2]), list(op[3]))
-        done += 1
-        if time.perf_counter() - t0 > args.cpu_budget_s:
-            break
-    dt = time.perf_counter() - t0
-    ops_per_s = done / dt
-    layers_per_s_n = ops_per_s / ops_per_layer
-    # per-op cost scales with the state size: 2^(n_gpu - n) more work per op at n_gpu qubits
-    layers_per_s = layers_per_s_n / (2 ** (n_gpu - n))
-    cores = len(os.sched_getaffinity(0))
+    state = {"t": psi.reshape((2,) * n)}
+
+    def one_cycle(ops=cycle):
+        t = state["t"]
+        for _, fam, params, axes in ops:
+            t = O.apply_gate(t, fam, params, axes)
+        state["t"] = t
+
+    # warm-up: one single-qubit gate and one CZ (touches every page of the state and buffers)
+    warm = [cycle[0], next(op for op in cycle if op[1] == "CZPowGate")]
+    sec = _median_s(one_cycle, 3, warm=lambda: one_cycle(warm))
+    layers_per_s_n = 1.0 / sec
+    layers_per_s = layers_per_s_n / (2 ** (n_gpu - n))  # per-op cost scales with the state size
+    del state
     return {
         "value": layers_per_s,
         "unit": "layers/s",
         "cores": 1,
         "kind": "port",
         "sample": (
-            f"first {done} ops ({dt:.1f} s) of the same circuit at {n} qubits through the NumPy restatement "
-            f"of Cirq's apply_unitary (complex64, single-threaded NumPy; {cores} cores visible, "
-            f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}): {layers_per_s_n:.4f} layers/s at "
-            f"{n} qubits, scaled by 2^-{n_gpu - n} to {n_gpu} qubits"
+            f"one cycle ({len(cycle)} ops: {sum(1 for op in cycle if op[1] != 'CZPowGate')} single-qubit gates + "
+            f"{sum(1 for op in cycle if op[1] == 'CZPowGate')} CZ) of the same circuit at {n} qubits through the oracle "
+            f"(Cirq's gate strategies then matrices, complex64, single-threaded NumPy; {_cpu_threads()}): median of 3 "
+            f"runs after a warm-up gate pair, {sec:.2f} s per cycle = {layers_per_s_n:.4f} layers/s at {n} qubits, scaled by "
+            f"2^-{n_gpu - n} to {n_gpu} qubits"
         ),
+        "moments_total": len(moments),
     }
 
 
@@ -454,73 +607,6 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
 
 
 
-def layout_probe(args):
-    """How fast HBM streams for the access patterns the pass kernel makes, on the rows x cols state.
-
-    Times (HIP events) a device-to-device copy of the state and passes that only load and store it
-    (one SCALE by 1) with a 12-bit tile: 4 contiguous low bits plus 8 bits placed in different
-    ranges of the flat index.  Bits >= 18 are 2 MiB apart or more in memory.  Then, for the grid
-    circuit compiled with 12- and 13-bit tiles, the sum of such load/store-only passes on the
-    program's own pass layouts (its streaming floor) beside the program's real pass time."""
-    import torch
-
-    from cirq_amd import engine, generators, tape
-
-    n = args.rows * args.cols
-    dev = engine.require_gpu()
-    st = engine.new_state(n, dev)
-    gb = 2 * 8 * 2**n / 1e9
-
-    def timed(fn, reps=4):
-        fn()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            fn()
-        e1.record()
-        e1.synchronize()
-        return e0.elapsed_time(e1) / reps
-
-    def load_store(L, lpos):
-        one = tape.LOp("scale", (), np.complex128(1.0))
-        dp = engine.DeviceProgram(tape.PackedProgram([tape.Pass(L, tuple(lpos), [one])], n), dev)
-        return timed(lambda: dp.run(st))
-
-    out = {"n_qubits": n, "state_gb": gb / 2}
-    y = torch.empty_like(st)
-    ms = timed(lambda: y.copy_(st))
-    out["device_copy"] = {"ms": ms, "tb_s": gb / ms}
-    del y
-    torch.cuda.empty_cache()
-    low = list(range(4))
-    pick = lambda lo, hi, seed: sorted(np.random.RandomState(seed).choice(np.arange(lo, hi), 8, replace=False).tolist())
-    layouts = {
-        "contiguous 0-11": [list(range(12))],
-        "4 low + 8 in [4,18)": [low + pick(4, 18, s) for s in range(3)],
-        "4 low + 8 in [18,n)": [low + pick(18, n, s) for s in range(3)],
-        "4 low + top 8": [low + list(range(n - 8, n))],
-    }
-    out["load_store_L12"] = {}
-    for name, lps in layouts.items():
-        ts = [load_store(12, lp) for lp in lps]
-        out["load_store_L12"][name] = {"ms": float(np.mean(ts)), "tb_s": gb / float(np.mean(ts)), "layouts": lps}
-    circuit = generators.grid_circuit(args.rows, args.cols, args.depth, seed=0)
-    lw = tape.Lowering(sorted(circuit.all_qubits()), dm=False)
-    lw.lower_circuit(circuit)
-    out["grid_program"] = {}
-    for L in (12, 13):
-        p = tape.compile_ops(lw.ops, n, L=L, coalesce=4, kmax=4)
-        dp = engine.DeviceProgram(p, dev)
-        real = timed(lambda: dp.run(st), reps=2)
-        floor = sum(load_store(int(r["L"]), [int(x) for x in r["lpos"][: int(r["L"])]]) for r in p.pass_arr)
-        high = [sum(1 for x in r["lpos"][: int(r["L"])] if int(x) >= 18) for r in p.pass_arr]
-        out["grid_program"][f"L{L}"] = {"passes": int(p.n_passes), "pass_ms": real, "load_store_floor_ms": floor,
-                                       "mean_local_bits_ge_18": float(np.mean(high))}
-    print(json.dumps(out))
-
-
 def q32_leg(args):
     """The larger configuration's 32-qubit variant: the random grid circuit on a 4x8 grid, depth 20
     (32 GiB complex64 state), with the same step as the main leg (10^6 samples, 52 <ZZ>)."""
@@ -561,10 +647,6 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n, "copies": 5}))
-        return
-
-    if args.layout_probe:
-        layout_probe(args)
         return
 
     from cirq_amd import generators
@@ -632,6 +714,8 @@ def main():
         out["extra"]["q32"] = q32_leg(args)
     if not args.no_adjoint:
         out["extra"]["adjoint"] = adjoint_leg(args)
+    if not args.no_qaoa:
+        out["extra"]["qaoa"] = qaoa_leg(args)
     if not args.no_dm_adjoint:
         out["extra"]["dm_adjoint"] = dm_adjoint_leg(args)
     if not args.no_sweep:
@@ -641,13 +725,7 @@ def main():
     if not args.no_dm:
         out["extra"]["density_matrix"] = dm_leg(args)
     if not args.no_cpu_baseline:
-        from tests.helpers import to_oracle_ops
-
-        def ops_fn(nq):
-            c = generators.grid_circuit(args.rows, args.cols, args.depth, seed=0, n_qubits=nq)
-            return to_oracle_ops(c, sorted(c.all_qubits()))
-
-        out["cpu_baseline"] = cpu_baseline(args, ops_fn, n, args.depth, n_ops / args.depth)
+        out["cpu_baseline"] = cpu_baseline(args, args.rows, args.cols, args.depth, n)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

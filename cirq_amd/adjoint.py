@@ -133,8 +133,11 @@ class AdjointProgram:
                 if q not in qmap:
                     raise KeyError(q)
         self.terms = [pauli_masks(t, qmap, n) for t in psum]
-        self.phi_scale = adjoint_scale(c for _, _, c in self.terms)
-        self.phi_terms = [(x, z, c / self.phi_scale) for x, z, c in self.terms]  # phi = (H / s) psi
+        # phi = (H' / s) psi with H' = H minus its identity terms: c I adds c d<psi|psi>/dt = 0
+        # to every derivative in exact arithmetic and only rounding noise in complex64
+        varying = [(x, z, c) for x, z, c in self.terms if x or z]
+        self.phi_scale = adjoint_scale(c for _, _, c in varying)
+        self.phi_terms = [(x, z, c / self.phi_scale) for x, z, c in varying]
         self.evaluator = ParamEvaluator(low.param_refs)
         self.symbols = list(self.evaluator.symbols)
         L = min(tile_bits if tile_bits is not None else 12, n)

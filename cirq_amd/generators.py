@@ -74,22 +74,14 @@ SQRT_Y = g.YPowGate(exponent=0.5)
 SQRT_W = g.PhasedXPowGate(phase_exponent=0.25, exponent=0.5)
 
 
-def random_rotations_between_grid_interaction_layers_circuit(
-    qubits: Sequence[GridQubit],
-    depth: int,
-    *,
-    two_qubit_op_factory: Callable = lambda a, b, _: g.CZPowGate().on(a, b),
-    pattern: Sequence[GridInteractionLayer] = GRID_STAGGERED_PATTERN,
-    single_qubit_gates: Sequence = (SQRT_X, SQRT_Y, SQRT_W),
-    add_final_single_qubit_layer: bool = True,
-    seed=None,
-) -> Circuit:
-    prng = seed if isinstance(seed, np.random.RandomState) else np.random.RandomState(seed)
-    qubits = list(qubits)
-    pairs = _coupled_qubit_pairs(qubits)
-    circuit = Circuit()
-    previous = Moment()
-    fixed = len(set(map(id, single_qubit_gates))) == 1
+def _single_qubit_layer_factory(single_qubit_gates: Sequence, qubits: Sequence, prng: np.random.RandomState):
+    """_single_qubit_gates_arg_to_factory (random_quantum_circuit_generation.py:640-687): one
+    distinct gate (by equality) is used everywhere; otherwise each layer draws a gate per qubit
+    with prng.randint, redrawing while it is the qubit's previous gate (by identity)."""
+    try:
+        fixed = len(set(single_qubit_gates)) == 1
+    except TypeError:  # unhashable gates: compare by identity
+        fixed = len(set(map(id, single_qubit_gates))) == 1
 
     def new_layer(prev: Moment) -> Moment:
         if fixed:
@@ -104,6 +96,29 @@ def random_rotations_between_grid_interaction_layers_circuit(
             ops.append(gate.on(q))
         return Moment(ops)
 
+    return new_layer
+
+
+def _prng(seed) -> np.random.RandomState:
+    return seed if isinstance(seed, np.random.RandomState) else np.random.RandomState(seed)
+
+
+def random_rotations_between_grid_interaction_layers_circuit(
+    qubits: Sequence[GridQubit],
+    depth: int,
+    *,
+    two_qubit_op_factory: Callable = lambda a, b, _: g.CZPowGate().on(a, b),
+    pattern: Sequence[GridInteractionLayer] = GRID_STAGGERED_PATTERN,
+    single_qubit_gates: Sequence = (SQRT_X, SQRT_Y, SQRT_W),
+    add_final_single_qubit_layer: bool = True,
+    seed=None,
+) -> Circuit:
+    prng = _prng(seed)
+    qubits = list(qubits)
+    pairs = _coupled_qubit_pairs(qubits)
+    circuit = Circuit()
+    previous = Moment()
+    new_layer = _single_qubit_layer_factory(single_qubit_gates, qubits, prng)
     for i in range(depth):
         layer = new_layer(previous)
         circuit += layer
@@ -113,6 +128,38 @@ def random_rotations_between_grid_interaction_layers_circuit(
     if add_final_single_qubit_layer:
         circuit += new_layer(previous)
     return circuit
+
+
+def random_rotations_between_two_qubit_circuit(
+    q0,
+    q1,
+    depth: int,
+    two_qubit_op_factory: Callable = lambda a, b, _: g.CZPowGate().on(a, b),
+    single_qubit_gates: Sequence = (SQRT_X, SQRT_Y, SQRT_W),
+    add_final_single_qubit_layer: bool = True,
+    seed=None,
+) -> Circuit:
+    """random_rotations_between_two_qubit_circuit (random_quantum_circuit_generation.py:171-231):
+    `depth` cycles of a random single-qubit layer on (q0, q1) then the two-qubit op, with the
+    same single-qubit layer factory (and draw order) as the grid generator."""
+    prng = _prng(seed)
+    circuit = Circuit()
+    previous = Moment()
+    new_layer = _single_qubit_layer_factory(single_qubit_gates, (q0, q1), prng)
+    for _ in range(depth):
+        layer = new_layer(previous)
+        circuit += layer
+        circuit += two_qubit_op_factory(q0, q1, prng)
+        previous = layer
+    if add_final_single_qubit_layer:
+        circuit += new_layer(previous)
+    return circuit
+
+
+def grid_interaction_layer_pairs(qubits: Sequence[GridQubit], layer: GridInteractionLayer) -> list:
+    """The nearest-neighbour pairs of `qubits` that `layer` activates, sorted (the pairs of one
+    moment of get_grid_interaction_layer_circuit, random_quantum_circuit_generation.py:511-535)."""
+    return sorted((a, b) for a, b in _coupled_qubit_pairs(list(qubits)) if (a, b) in layer or (b, a) in layer)
 
 
 def grid_circuit(rows: int, cols: int, depth: int, seed: int = 0, n_qubits: int | None = None) -> Circuit:

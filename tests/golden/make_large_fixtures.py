@@ -5,12 +5,15 @@
   tests/test_oracle_csim.py).  Stored: 4,098 amplitudes at seeded indices (plus the first and last),
   <Z_i Z_j> for every nearest-neighbour pair, and the overlaps <v_r|psi> with 4 seeded product
   states |v_r> = (x)_q V_rq |0> (each V_rq a Haar-random 2x2 unitary, written down too).
+* ``qaoa24_p8.npz``: QAOA MaxCut on the seeded 24-node 3-regular graph, p = 8 (16 symbols, seed 0):
+  <H> and the gradient by central differences (h = 1e-5) of the complex128 simulation, accurate to
+  about 1e-9.
 * ``noisy_line12_depth20.npz``: the noisy-line density-matrix circuit on 12 qubits, 20 layers,
   seed 0 (depolarize(1e-3) after every gate, amplitude_damp(1e-3) per qubit per layer), from
   |0...0><0...0|.  Stored: the diagonal of rho, 4,096 off-diagonal entries at seeded (row, col)
   positions and Tr(rho sum_i Z_i).
 
-Usage: ``python tests/golden/make_large_fixtures.py [grid|dm|all]`` from the repository root.
+Usage: ``python tests/golden/make_large_fixtures.py [grid|dm|qaoa|all]`` from the repository root.
 The 28-qubit run holds 8 GiB and takes about 10 minutes on 8 cores.
 """
 
@@ -101,9 +104,45 @@ def make_dm(path: str):
     print(f"wrote {path}: Tr(rho sum Z) = {sz:.12f}, trace {np.real(np.trace(rho)):.15f}", flush=True)
 
 
+def make_qaoa(path: str, n: int = 24, p: int = 8, seed: int = 0, h: float = 1e-5):
+    from cirq_amd.study import ParamResolver
+
+    circuit, syms, vals, ham, edges = generators.qaoa_maxcut(n, p, seed=seed)
+    qs = sorted(circuit.all_qubits())
+    qmap = {q: i for i, q in enumerate(qs)}
+    const = 0.0
+    masks, coefs = [], []
+    for t in ham:
+        items = list(t.items())
+        if not items:
+            const += complex(t.coefficient).real
+            continue
+        masks.append(sum(1 << (n - 1 - qmap[q]) for q, _ in items))
+        coefs.append(complex(t.coefficient).real)
+
+    def value(v):
+        psi = csim.simulate_state_vector(to_oracle_ops(circuit, qs, ParamResolver(dict(zip(syms, v)))), n)
+        return const + float(np.dot(coefs, csim.expect_z(psi, n, masks)))
+
+    t0 = time.time()
+    v0 = value(vals)
+    grad = np.zeros(len(syms))
+    for k in range(len(syms)):
+        up, dn = vals.copy(), vals.copy()
+        up[k] += h
+        dn[k] -= h
+        grad[k] = (value(up) - value(dn)) / (2 * h)
+        print(f"  symbol {k}: {grad[k]:+.10f} ({time.time() - t0:.0f} s)", flush=True)
+    np.savez_compressed(path, value=v0, grad=grad, values=vals, symbols=np.array([s.name for s in syms]),
+                        edges=np.array(edges), n=n, p=p, seed=seed, h=h)
+    print(f"wrote {path}: <H> = {v0:.12f}", flush=True)
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     if what in ("dm", "all"):
         make_dm(os.path.join(HERE, "noisy_line12_depth20.npz"))
     if what in ("grid", "all"):
         make_grid(os.path.join(HERE, "grid28_depth40.npz"))
+    if what in ("qaoa", "all"):
+        make_qaoa(os.path.join(HERE, "qaoa24_p8.npz"))

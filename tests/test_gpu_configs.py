@@ -285,6 +285,28 @@ def test_qaoa24_p8_adjoint_vs_parameter_shift(gpu):
     np.testing.assert_allclose(grad, g2, atol=1e-5 * 36 / 10)
 
 
+def test_qaoa24_p8_adjoint_matches_oracle_fixture(gpu):
+    """The 24-node p = 8 gradient against central differences of the complex128 simulation
+    (tests/golden/qaoa24_p8.npz, accurate to ~1e-8).  Bar: 1e-5 relative to the largest gradient
+    component (12.9 here), i.e. 1.3e-4 absolute.  complex64 states after ~1,000 gate applications
+    (forward sweep and un-computation) carry ~2e-6 relative error, and these gradients sum 36-60
+    such terms of size up to |H| = 18 (the identity part of H is dropped from phi); measured:
+    2.5e-5 absolute."""
+    from cirq_amd.adjoint import AdjointGradient
+
+    fx = np.load(os.path.join(GOLDEN, "qaoa24_p8.npz"))
+    circuit, syms, vals, ham, edges = generators.qaoa_maxcut(24, 8, seed=0)
+    assert np.array_equal(vals, fx["values"]) and [s.name for s in syms] == list(fx["symbols"])
+    ag = AdjointGradient(circuit, ham)
+    value, grad = ag.value_and_grad(dict(zip(syms, vals)))
+    want = dict(zip(syms, fx["grad"]))
+    err = np.abs(np.asarray(grad) - np.array([want[s] for s in ag.symbols]))
+    _log(f"QAOA 24q p=8 vs oracle: max|grad err| {err.max():.3e} (max|grad| {np.abs(fx['grad']).max():.2f}), "
+         f"<H> err {abs(value - float(fx['value'])):.3e}")
+    assert err.max() <= 1e-5 * max(1.0, np.abs(fx["grad"]).max())
+    assert abs(value - float(fx["value"])) < 1e-5
+
+
 def test_qaoa10_p8_matches_oracle_finite_differences(gpu):
     from cirq_amd.adjoint import AdjointGradient
     from tests.test_adjoint_host import _oracle_ev, _oracle_grad
