@@ -111,6 +111,7 @@ _SIGNATURES = {
     "cqs_program_validate": ([_i32, _vp, _i32, _vp, _i32, _i64, _vp, _i64, _i32], ctypes.c_int),
     "cqs_expect_workspace_bytes": ([_i32], _sz),
     "cqs_expect_paulis": ([_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cqs_expect_paulis_noalloc": ([_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cqs_apply_paulisum": ([_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cqs_norm2": ([_vp, _i32, _vp, _vp, _vp], ctypes.c_int),
     "cqs_scale": ([_vp, _i32, _f32, _f32, _vp], ctypes.c_int),

@@ -1,5 +1,6 @@
 // Pauli-string expectations, PauliSum application and norms (FP64 accumulation).
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 #include <vector>
 
@@ -208,51 +209,40 @@ __global__ __launch_bounds__(256) void k_apply_paulisum(const float2* __restrict
   }
 }
 
-}  // namespace cqs
-
-using namespace cqs;
-
-extern "C" {
-
-size_t cqs_expect_workspace_bytes(int32_t n_terms) {
-  (void)n_terms;
-  return sizeof(double) * std::max(2 * kTermsPerLaunch * kRedBlocks, kZTerms * kZBlocks);
-}
-
-int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms,
-                      const uint64_t* xmask, const uint64_t* zmask, const double* coef, double* out,
-                      void* workspace, void* stream) {
-  CQS_REQUIRE(amps != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
-  CQS_REQUIRE(n_qubits >= 0 && n_qubits <= (is_dm ? 20 : CQS_MAX_BITS), "n_qubits=%d out of range",
-              n_qubits);
+// Term checks shared by the expectation entry points.
+static int check_terms(int32_t n_qubits, int32_t is_dm, int32_t n_terms, const uint64_t* xmask, const uint64_t* zmask,
+                       const double* coef) {
+  CQS_REQUIRE(n_qubits >= 0 && n_qubits <= (is_dm ? 20 : CQS_MAX_BITS), "n_qubits=%d out of range", n_qubits);
   CQS_REQUIRE(n_terms >= 0, "n_terms < 0");
   if (n_terms == 0) return CQS_OK;
   CQS_REQUIRE(xmask && zmask && coef, "NULL term arrays");
   const uint64_t valid = n_qubits >= 64 ? ~0ull : ((1ull << n_qubits) - 1);
-  std::vector<int> order(n_terms);
-  std::iota(order.begin(), order.end(), 0);
   for (int t = 0; t < n_terms; ++t)
     CQS_REQUIRE(!(xmask[t] & ~valid) && !(zmask[t] & ~valid), "term %d has masks beyond %d qubits", t,
                 n_qubits);
-  // by X mask (terms sharing one launch); Z-only terms also by their low 4 z bits (k_expect_z)
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    if (xmask[a] != xmask[b]) return xmask[a] < xmask[b];
-    return xmask[a] == 0 && (zmask[a] & 15) < (zmask[b] & 15);
-  });
-  hipStream_t s = as_stream(stream);
-  double* partials = (double*)workspace;
+  return CQS_OK;
+}
+
+// Launches over the terms in `order` (identity when NULL): runs of terms with one X mask share a
+// k_expect launch (up to 16 terms), runs of Z-only terms a k_expect_z launch (up to 32).  Every
+// term's sum has a fixed reduction order whatever batch it lands in, so any order of the terms
+// gives the same bits.  `partials` holds one launch's per-block partial sums.
+static int expect_batches(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms, const int* order,
+                          const uint64_t* xmask, const uint64_t* zmask, const double* coef, double* out,
+                          double* partials, hipStream_t s) {
+  auto term = [&](int i) { return order ? order[i] : i; };
   const uint64_t N = 1ull << n_qubits;
   int grid = (int)std::min<uint64_t>(kRedBlocks, (N + 255) / 256);
   if (grid < 1) grid = 1;
   const int zgrid = n_qubits >= 4 ? (int)std::min<uint64_t>(kZBlocks, ((N >> 4) + 255) / 256) : 0;
   for (int i = 0; i < n_terms;) {
-    if (!is_dm && n_qubits >= 4 && xmask[order[i]] == 0) {
+    if (!is_dm && n_qubits >= 4 && xmask[term(i)] == 0) {
       ZBatch zb;
       ZFinal zf;
       zb.count = 0;
       zb.pad = 0;
-      while (i < n_terms && zb.count < kZTerms && xmask[order[i]] == 0) {
-        const int t = order[i];
+      while (i < n_terms && zb.count < kZTerms && xmask[term(i)] == 0) {
+        const int t = term(i);
         zb.z[zb.count] = zmask[t];
         zf.out_idx[zb.count] = t;
         zf.cr[zb.count] = coef[2 * t];
@@ -270,11 +260,11 @@ int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t
     }
     TermBatch tb;
     FinalBatch fb;
-    tb.x = xmask[order[i]];
+    tb.x = xmask[term(i)];
     tb.count = 0;
     tb.pad = 0;
-    while (i < n_terms && tb.count < kTermsPerLaunch && xmask[order[i]] == tb.x) {
-      const int t = order[i];
+    while (i < n_terms && tb.count < kTermsPerLaunch && xmask[term(i)] == tb.x) {
+      const int t = term(i);
       tb.z[tb.count] = zmask[t];
       fb.out_idx[tb.count] = t;
       fb.cr[tb.count] = coef[2 * t];
@@ -291,6 +281,78 @@ int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t
   }
   CQS_HIP(hipGetLastError());
   return CQS_OK;
+}
+
+// Static scratch for cqs_expect_paulis_noalloc: one launch's partial sums, the largest any batch
+// writes (2048 blocks x 32 Z terms, or 512 blocks x 16 terms x 2 doubles): 512 KiB per GPU, held
+// until the process ends.  In host code `scratch` is only a handle for hipGetSymbolAddress; the
+// kernels get the device address that call returns, kept per device.
+static __device__ __align__(256) double scratch[kZTerms * kZBlocks > 2 * kTermsPerLaunch * kRedBlocks
+                                                    ? kZTerms * kZBlocks
+                                                    : 2 * kTermsPerLaunch * kRedBlocks];
+constexpr int kMaxDevices = 64;
+static std::atomic<double*> g_scratch_ptr[kMaxDevices];
+
+// The current device's address of `scratch`, looked up once per device (hipGetSymbolAddress with
+// that device current).
+static int scratch_for_current_device(double** out) {
+  int dev = 0;
+  CQS_HIP(hipGetDevice(&dev));
+  CQS_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the scratch table covers", dev, kMaxDevices);
+  double* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (p == nullptr) {
+    void* addr = nullptr;
+    CQS_HIP(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+    CQS_REQUIRE(addr != nullptr, "hipGetSymbolAddress returned NULL for the expectation scratch");
+    p = (double*)addr;
+    g_scratch_ptr[dev].store(p, std::memory_order_release);
+  }
+  *out = p;
+  return CQS_OK;
+}
+
+}  // namespace cqs
+
+using namespace cqs;
+
+extern "C" {
+
+size_t cqs_expect_workspace_bytes(int32_t n_terms) {
+  (void)n_terms;
+  return sizeof(double) * std::max(2 * kTermsPerLaunch * kRedBlocks, kZTerms * kZBlocks);
+}
+
+int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms,
+                      const uint64_t* xmask, const uint64_t* zmask, const double* coef, double* out,
+                      void* workspace, void* stream) {
+  CQS_REQUIRE(amps != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
+  if (int rc = check_terms(n_qubits, is_dm, n_terms, xmask, zmask, coef)) return rc;
+  if (n_terms == 0) return CQS_OK;
+  std::vector<int> order(n_terms);
+  std::iota(order.begin(), order.end(), 0);
+  // by X mask (terms sharing one launch); Z-only terms also by their low 4 z bits (k_expect_z)
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (xmask[a] != xmask[b]) return xmask[a] < xmask[b];
+    return xmask[a] == 0 && (zmask[a] & 15) < (zmask[b] & 15);
+  });
+  return expect_batches(amps, n_qubits, is_dm, n_terms, order.data(), xmask, zmask, coef, out, (double*)workspace,
+                        as_stream(stream));
+}
+
+int cqs_expect_paulis_noalloc(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms,
+                              const uint64_t* xmask, const uint64_t* zmask, const double* coef, double* out,
+                              void* stream) {
+  CQS_REQUIRE(amps != nullptr && out != nullptr, "NULL pointer");
+  if (int rc = check_terms(n_qubits, is_dm, n_terms, xmask, zmask, coef)) return rc;
+  if (n_terms == 0) return CQS_OK;
+  CQS_REQUIRE(cqs_expect_workspace_bytes(n_terms) <= sizeof(scratch),
+              "expectation scratch of %zu bytes exceeds the static %zu-byte array", cqs_expect_workspace_bytes(n_terms),
+              sizeof(scratch));
+  double* partials = nullptr;
+  if (int rc = scratch_for_current_device(&partials)) return rc;
+  // terms in the caller's order: runs of equal X masks share launches (no host sort, no allocation)
+  return expect_batches(amps, n_qubits, is_dm, n_terms, nullptr, xmask, zmask, coef, out, partials,
+                        as_stream(stream));
 }
 
 int cqs_norm2(const void* amps, int32_t n_bits, double* out, void* workspace, void* stream) {

@@ -286,6 +286,22 @@ int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t
                       double* out, void* workspace, void* stream);
 
 /*
+ * cqs_expect_paulis without a workspace argument and without any allocation (no hipMalloc,
+ * hipFree or allocating library call, host or device): the per-block partial sums live in a
+ * static 512 KiB device array per GPU, located once per device with hipGetSymbolAddress and kept
+ * until the process ends.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * Terms are batched in the caller's order (runs of one X mask share a launch); the values are
+ * bit-identical to cqs_expect_paulis.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (different streams or threads) race and must be serialised by the caller.
+ * Sizes beyond the array are refused with CQS_ERR_ARG (the array holds the largest batch any
+ * n_terms needs, so this does not happen for valid arguments).
+ */
+int cqs_expect_paulis_noalloc(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms,
+                              const uint64_t* xmask, const uint64_t* zmask, const double* coef,
+                              double* out, void* stream);
+
+/*
  * phi = sum_t coef_t P_t psi (the PauliSum applied to a state vector), complex64 out; the
  * term arrays are HOST arrays as above.
  * Used by the adjoint gradient.  No direct reference counterpart: Cirq evaluates each term
