@@ -95,6 +95,12 @@ def _grad_op(op: LOp, k: int, slot: int) -> LOp:
     return LOp("grad_dense", tuple(bits), None, (), ParamRef(ref.gate, controlled, ref.exprs, ref.slot, k), 0, slot)
 
 
+def adjoint_waves(L: int) -> int:
+    """Waves per workgroup of the adjoint kernels at tile width L: each writes its own partial of
+    every GRAD op (256 threads up to L = 12, then 16 elements per thread)."""
+    return (256 if L <= 12 else (1 << L) // 16) // 64
+
+
 def shift_rule_ok(gate, k: int) -> bool:
     """True when parameter k of `gate` enters as exp(i pi t h) with two eigenvalues h one apart,
     the case where the two-term shift rule with shifts +-1/2 is exact."""
@@ -163,7 +169,7 @@ class AdjointProgram:
         passes = [order_tile_bits(p, coalesce_bits) for p in passes]
         self.rev = PackedProgram(passes, n)
         self.tile_bits_rev = La
-        self.partial_stride = 1 << (n - La)
+        self.partial_stride = (1 << (n - La)) * adjoint_waves(La)  # one FP64 partial per wave of each tile
         self._slot_exprs = [ref.exprs[k] for ref, k in self.grad_slots]
         self._chain = SlotChain(self._slot_exprs, self.symbols, self.evaluator)
 
@@ -244,6 +250,11 @@ class AdjointGradient:
         self._forward()
         return self.psi
 
+    def value(self, values) -> float:
+        """<H> alone: the forward passes and one expectation, no reverse sweep."""
+        psi = self.forward_state(values)
+        return float(engine.expect_terms(psi, self.n, False, self.prog.terms, self.ws).sum().real.item())
+
     def value_and_grad(self, values) -> tuple[float, np.ndarray]:
         """(<H>, d<H>/d symbol in `symbols` order) at the given symbol values."""
         prog, n = self.prog, self.n
@@ -299,20 +310,27 @@ class AdjointGradient:
 
 
 class _ExpectationFn(torch.autograd.Function):
+    """Rows of symbol values (B, S) -> <H> per row (B,); the adjoint sweep runs in forward and its
+    gradients are kept for backward."""
+
     @staticmethod
     def forward(ctx, values: torch.Tensor, grad_engine):
-        value, grad = grad_engine.value_and_grad(values.detach().cpu().numpy())
-        ctx.save_for_backward(torch.as_tensor(grad, dtype=values.dtype, device=values.device))
-        return torch.tensor(value, dtype=values.dtype, device=values.device)
+        rows = values.detach().cpu().numpy().reshape(-1, values.shape[-1])
+        vg = [grad_engine.value_and_grad(r) for r in rows]
+        grads = np.stack([g for _, g in vg])
+        ctx.save_for_backward(torch.as_tensor(grads, dtype=values.dtype, device=values.device).reshape(values.shape))
+        return torch.tensor([v for v, _ in vg], dtype=values.dtype, device=values.device).reshape(values.shape[:-1])
 
     @staticmethod
     def backward(ctx, gout):
         (grad,) = ctx.saved_tensors
-        return gout * grad, None
+        return gout.unsqueeze(-1) * grad, None
 
 
 class ExpectationLayer(torch.nn.Module):
-    """torch module: symbol values (1-D tensor, in `.symbols` order) -> <H>, differentiable.
+    """torch module: symbol values (in `.symbols` order; shape (S,) or a batch (B, S)) -> <H>
+    (a scalar, or (B,)), differentiable.  When no gradient is needed (torch.no_grad, or inputs
+    that do not require grad) only the forward passes run; otherwise one adjoint sweep per row.
 
     density_matrix=True evolves rho instead (the circuit may hold channels) and differentiates
     Tr(H rho) with the checkpointed density-matrix sweep of cirq_amd.dm_adjoint."""
@@ -328,4 +346,11 @@ class ExpectationLayer(torch.nn.Module):
         self.symbols = self.engine.symbols
 
     def forward(self, values: torch.Tensor) -> torch.Tensor:
-        return _ExpectationFn.apply(values, self.engine)
+        if values.dim() not in (1, 2) or values.shape[-1] != len(self.symbols):
+            raise ValueError(f"expected (S,) or (B, S) symbol values with S = {len(self.symbols)}, got {tuple(values.shape)}")
+        if torch.is_grad_enabled() and values.requires_grad:
+            return _ExpectationFn.apply(values, self.engine)
+        rows = values.detach().cpu().numpy().reshape(-1, values.shape[-1])
+        out = [self.engine.value(r) for r in rows]
+        return torch.tensor(out, dtype=values.dtype, device=values.device).reshape(values.shape[:-1])
+

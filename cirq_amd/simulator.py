@@ -846,7 +846,11 @@ class _SimulatorBase:
         initial_state=None,
         permit_terminal_measurements: bool = False,
     ):
-        if not permit_terminal_measurements and program.has_measurements() and program.are_all_measurements_terminal():
+        """Expectation values of the final state of every resolver's simulation, as
+        cirq-core/cirq/sim/sparse_simulator.py:193-218 computes them from simulate_sweep_iter:
+        ValueError if any measurement is terminal (unless permitted); circuits with mid-circuit
+        measurements or state-vector noise take one sampled trajectory per resolver."""
+        if not permit_terminal_measurements and program.are_any_measurements_terminal():
             raise ValueError(
                 "Provided circuit has terminal measurements, which may "
                 "skew expectation values. If this is intentional, set "
@@ -859,25 +863,34 @@ class _SimulatorBase:
         circuit = self._noisy(program)
         qubits = QubitOrder.as_qubit_order(qubit_order).order_for(circuit.all_qubits())
         qmap = {q: i for i, q in enumerate(qubits)}
-        body, meas = _split_terminal_measurements(circuit)
-        comp = self._compile(body, qubits)
-        self._check_resolved(resolvers, comp)
+        n = len(qubits)
         terms, owner = [], []
         for oi, ps in enumerate(pslist):
             for t in ps:
                 for q in t.qubits:
                     if q not in qmap:
                         raise KeyError(q)
-                terms.append(pauli_masks(t, qmap, comp.n))
+                terms.append(pauli_masks(t, qmap, n))
                 owner.append(oi)
         owner = np.array(owner, dtype=np.int64)
+
+        def values(st):
+            vals = engine.expect_terms(st, n, self._dm, terms, self._ws).cpu().numpy()
+            sums = np.zeros(len(pslist), dtype=np.complex128)
+            np.add.at(sums, owner, vals)
+            return [self._ev_value(v) for v in sums]
+
+        if self._needs_trajectory(circuit):
+            for res in self.simulate_sweep_iter(program, resolvers, qubits, initial_state):
+                yield values(res._state)
+            return
+        body, meas = _split_terminal_measurements(circuit)
+        comp = self._compile(body, qubits)
+        self._check_resolved(resolvers, comp)
         for r, st in zip(resolvers, self._run_program(comp, resolvers, initial_state)):
             if meas:
                 self._measure_terminal(st, comp, meas)
-            vals = engine.expect_terms(st, comp.n, self._dm, terms, self._ws).cpu().numpy()
-            sums = np.zeros(len(pslist), dtype=np.complex128)
-            np.add.at(sums, owner, vals)
-            yield [self._ev_value(v) for v in sums]
+            yield values(st)
 
     def _ev_value(self, v):
         return v

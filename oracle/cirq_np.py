@@ -163,8 +163,7 @@ def channel_kraus(family: str, params: dict) -> list:
             out.append(np.sqrt(pk) * m)
         return out
     if family == "asymmetric_depolarize":  # cirq-core/cirq/ops/common_channels.py:114-128
-        px, py, pz = params.get("p_x", 0.0), params.get("p_y", 0.0), params.get("p_z", 0.0)
-        return [np.sqrt(1 - px - py - pz) * _PI, np.sqrt(px) * _PX, np.sqrt(py) * _PY, np.sqrt(pz) * _PZ]
+        return [np.sqrt(p) * u for p, u in channel_mixture(family, params)]
     if family == "bit_flip":
         p = params["p"]
         return [np.sqrt(1 - p) * _PI, np.sqrt(p) * _PX]
@@ -463,3 +462,150 @@ def expectation_from_density_matrix(rho, term, n: int) -> float:
     while any(r.shape):
         r = np.trace(r, axis1=0, axis2=len(r.shape) // 2)
     return float(np.real(r * coef))
+
+
+# ------------------------------------------------------------------------------ trajectories
+def channel_mixture(family: str, params: dict):
+    """cirq.mixture of the mixture channels, [(p_i, U_i)] in Cirq's order, or None for channels
+    that only have Kraus operators.
+      depolarize             cirq-core/cirq/ops/common_channels.py:281-293 (I, then Paulis)
+      asymmetric_depolarize  cirq-core/cirq/ops/common_channels.py:114-128
+      bit_flip / phase_flip  cirq-core/cirq/ops/common_channels.py (BitFlipChannel._mixture_,
+                             PhaseFlipChannel._mixture_: identity first)"""
+    if family == "depolarize":
+        n = params.get("n_qubits", 1)
+        p = params["p"]
+        out = []
+        for tup in itertools.product("IXYZ", repeat=n):
+            m = np.identity(1, dtype=np.complex128)
+            for c in tup:
+                m = np.kron(m, _PM[c])
+            out.append((1 - p if set(tup) == {"I"} else p / (4**n - 1), m))
+        return out
+    if family == "asymmetric_depolarize":
+        ep = params.get("error_probabilities")
+        if ep is None:
+            px, py, pz = params.get("p_x", 0.0), params.get("p_y", 0.0), params.get("p_z", 0.0)
+            ep = {"I": 1 - px - py - pz, "X": px, "Y": py, "Z": pz}
+        out = []
+        for pauli, p in ep.items():  # the channel's own order of Pauli strings
+            m = np.identity(1, dtype=np.complex128)
+            for c in pauli:
+                m = np.kron(m, _PM[c])
+            out.append((p, m))
+        return out
+    if family == "bit_flip":
+        return [(1 - params["p"], _PI), (params["p"], _PX)]
+    if family == "phase_flip":
+        return [(1 - params["p"], _PI), (params["p"], _PZ)]
+    return None
+
+
+def split_prefix(moments, in_prefix):
+    """split_into_matching_protocol_then_general (cirq-core/cirq/sim/simulator.py:952-983) on
+    moments of neutral ops: an op joins the prefix when `in_prefix(op)` holds and no earlier op
+    sent to the suffix shares a qubit (or, for measurements, a key) with it."""
+    blocked_q, blocked_k = set(), set()
+    prefix, suffix = [], []
+    for moment in moments:
+        mp, gp = [], []
+        for op in moment:
+            qs = set(op[3])
+            keys = {op[1]} if op[0] == "measure" else set()
+            if in_prefix(op) and qs.isdisjoint(blocked_q) and keys.isdisjoint(blocked_k):
+                mp.append(op)
+            else:
+                blocked_q |= qs
+                blocked_k |= keys
+                gp.append(op)
+        if mp:
+            prefix.append(mp)
+        if gp:
+            suffix.append(gp)
+    return prefix, suffix
+
+
+def _measure(t: np.ndarray, axes, invert, prng):
+    """SimulationState.measure -> measure_state_vector (cirq-core/cirq/sim/state_vector.py:235-322):
+    probabilities in the state's precision, one prng.choice, projection, renormalisation, then
+    the invert mask (cirq-core/cirq/sim/simulation_state.py measure)."""
+    n = t.ndim
+    probs = (t * t.conj()).real
+    probs = state_probabilities_by_indices(probs, list(axes), (2,) * n)
+    result = int(prng.choice(len(probs), p=probs))
+    bits = big_endian_int_to_digits(result, (2,) * len(axes))
+    mask = np.ones(t.shape, dtype=bool)
+    mask[slice_for_qubits_equal_to(list(axes), result, n)] = False
+    out = np.copy(t)
+    out[mask] = 0
+    out /= np.sqrt(probs[result])
+    bits = [b ^ int(bool(invert[i])) if i < len(invert) else b for i, b in enumerate(bits)]
+    return bits, out
+
+
+def _apply_stochastic_channel(t: np.ndarray, family: str, params: dict, axes, prng) -> np.ndarray:
+    """act_on for a channel on a state vector (cirq-core/cirq/sim/state_vector_simulation_state.py:
+    183-257, strategy order :376-399): a mixture draws its unitary with prng.choice; otherwise
+    one prng.random() walks the Kraus operators by their weights ||K_i psi||^2, falling back to the
+    heaviest branch when rounding leaves the draw unassigned."""
+    mix = channel_mixture(family, params)
+    if mix is not None:
+        probs, unitaries = zip(*mix)
+        index = prng.choice(range(len(unitaries)), p=probs)
+        return apply_unitary(t, np.asarray(unitaries[index]), list(axes))
+    kraus = channel_kraus(family, params)
+    p = prng.random()
+    fallback_weight, fallback_index = 0.0, 0
+    buf = None
+    for index, k in enumerate(kraus):
+        buf = apply_unitary(t, k, list(axes))
+        weight = float(np.linalg.norm(buf) ** 2)
+        if weight > fallback_weight:
+            fallback_index, fallback_weight = index, weight
+        p -= weight
+        if p < 0:
+            break
+    if p >= 0 or weight == 0:
+        buf = apply_unitary(t, kraus[fallback_index], list(axes))
+        weight = fallback_weight
+    return buf / np.sqrt(weight)
+
+
+def run_trajectories(moments, n: int, repetitions: int, prng: np.random.RandomState, dtype=np.complex64) -> dict:
+    """SimulatorBase._run with a general suffix (cirq-core/cirq/sim/simulator_base.py:216-273) on
+    one merged state vector: the unitary prefix once, then per repetition, in order, the suffix's
+    gates, measurements (records per key, one row per instance) and channels on a copy of the
+    prefix state; records padded as pad_evenly does.  `moments`: lists of neutral ops, with
+    ('measure', key, {'invert': mask}, axes) for measurements."""
+    prefix, suffix = split_prefix(moments, lambda op: op[0] == "gate")
+    psi = np.zeros(2**n, dtype=dtype)
+    psi[0] = 1
+    t0 = psi.reshape((2,) * n)
+    for m in prefix:
+        for _, fam, params, axes in m:
+            t0 = apply_gate(t0, fam, params, axes) if fam != "GlobalPhase" else t0 * params["coefficient"]
+    records: dict = {}
+    for _ in range(repetitions):
+        t = np.copy(t0)
+        rec: dict = {}
+        for m in suffix:
+            for kind, fam, params, axes in m:
+                if kind == "measure":
+                    bits, t = _measure(t, axes, params.get("invert", ()), prng)
+                    rec.setdefault(fam, []).append(bits)
+                elif kind == "channel":
+                    t = _apply_stochastic_channel(t, fam, params, axes, prng)
+                elif fam == "GlobalPhase":
+                    t = t * params["coefficient"]
+                else:
+                    t = apply_gate(t, fam, params, axes)
+        for k, r in rec.items():
+            records.setdefault(k, []).append(r)
+    out = {}
+    for k, v in records.items():
+        largest = max(len(r) for r in v)
+        xs = np.zeros((len(v), largest, len(v[0][0])), dtype=np.uint8)
+        for i, r in enumerate(v):
+            xs[i, : len(r), :] = r
+        out[k] = xs
+    return out

@@ -57,6 +57,8 @@ def _channel_spec(gate):
     name = type(gate).__name__
     if name == "DepolarizingChannel":
         return "depolarize", {"p": gate.p, "n_qubits": gate.n_qubits}
+    if name == "AsymmetricDepolarizingChannel":
+        return "asymmetric_depolarize", {"error_probabilities": dict(gate.error_probabilities)}
     if name == "AmplitudeDampingChannel":
         return "amplitude_damp", {"gamma": gate.gamma}
     if name == "GeneralizedAmplitudeDampingChannel":
@@ -99,6 +101,24 @@ def to_oracle_ops(circuit, qubits, resolver=None):
                 continue
             fam, params = _gate_spec(gate)
             out.append(("gate", fam, params, axes))
+    return out
+
+
+def to_oracle_moments(circuit, qubits, resolver=None):
+    """The circuit as moments of neutral oracle ops, measurements included as
+    ('measure', key, {'invert': full invert mask}, axes)."""
+    idx = {q: i for i, q in enumerate(qubits)}
+    out = []
+    for moment in circuit:
+        ops = []
+        for op in moment:
+            gate = getattr(op, "gate", None)
+            if gate is not None and type(gate).__name__ == "MeasurementGate":
+                ops.append(("measure", gate.key, {"invert": list(gate.full_invert_mask())},
+                            tuple(idx[q] for q in op.qubits)))
+            else:
+                ops += to_oracle_ops([[op]], qubits, resolver)
+        out.append(ops)
     return out
 
 

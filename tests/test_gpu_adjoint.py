@@ -96,6 +96,30 @@ def test_expectation_layer_autograd(gpu):
     assert layer.symbols == syms
 
 
+def test_expectation_layer_batched_rows_and_no_grad(gpu):
+    """(B, S) input: one value per row and per-row gradients; without grad only forward passes
+    run and give the same values."""
+    _log("ExpectationLayer batched / no-grad")
+    circuit, syms, vals = generators.hardware_efficient_ansatz(6, 2, seed=4)
+    psum = generators.random_pauli_sum(sorted(circuit.all_qubits()), 5, 3, seed=4)
+    layer = ExpectationLayer(circuit, psum)
+    X = torch.tensor(np.stack([vals, 0.5 * vals, -vals]), dtype=torch.float64, requires_grad=True)
+    y = layer(X)
+    assert y.shape == (3,)
+    w = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64)
+    (w * y).sum().backward()
+    for i in range(3):
+        v, g = layer.engine.value_and_grad(X[i].detach().numpy())
+        assert abs(float(y[i]) - v) < 1e-12
+        np.testing.assert_allclose(X.grad[i].numpy(), float(w[i]) * g, atol=1e-12)
+    with torch.no_grad():
+        y2 = layer(X)
+    np.testing.assert_allclose(y2.numpy(), y.detach().numpy(), atol=1e-12)
+    assert layer(torch.tensor(vals, dtype=torch.float64)).shape == ()
+    with pytest.raises(ValueError):
+        layer(torch.zeros(3, 2, len(syms), dtype=torch.float64))
+
+
 def test_adjoint_rejects_non_unitary_scale(gpu):
     q = cirq.LineQubit.range(2)
     t = sympy.Symbol("t")

@@ -183,3 +183,38 @@ def test_json_loaded_circuit_on_the_gpu(gpu):
         want = O.simulate_state_vector(to_oracle_ops(c, qs, resolver=r), len(qs))
         _log(f"Circuit.json: {len(qs)} qubits, max|err| {np.max(np.abs(got - want)):.2e}")
         assert np.max(np.abs(got - want)) <= 1e-5 and fidelity(got, want) >= 1 - 1e-6
+
+
+def test_expectation_values_known_answers_and_measurements(gpu):
+    # cirq-core/cirq/sim/sparse_simulator_test.py:630-708
+    q0, q1, q2 = cirq.LineQubit.range(3)
+    sim = cirq.Simulator()
+    psum1 = cirq.Z(q0) + 3.2 * cirq.Z(q1)
+    psum2 = -1 * cirq.X(q0) + 2 * cirq.X(q1)
+    r = sim.simulate_expectation_values(cirq.Circuit(cirq.I(q0), cirq.X(q1)), [psum1, psum2])
+    assert abs(r[0] - -2.2) < 1e-6 and abs(r[1]) < 1e-6
+    r = sim.simulate_expectation_values(cirq.Circuit(cirq.H(q0), cirq.H(q1)), [psum1, psum2])
+    assert abs(r[0]) < 1e-6 and abs(r[1] - 1) < 1e-6
+    r = sim.simulate_expectation_values(cirq.Circuit(cirq.I(q0), cirq.H(q1)), cirq.Z(q0) + cirq.X(q1))
+    assert abs(r[0] - 2) < 1e-6
+    c = cirq.Circuit(cirq.H(q0), cirq.H(q1), cirq.X(q2))
+    obs = cirq.X(q0) + cirq.X(q1) - cirq.Z(q2)
+    assert abs(sim.simulate_expectation_values(c, obs)[0] - 3) < 1e-6
+    assert abs(sim.simulate_expectation_values(c, obs, qubit_order=[q1, q2, q0])[0] - 3) < 1e-6
+    # terminal measurement: refused unless permitted, then the collapsed state's value (+-1)
+    term = cirq.Circuit(cirq.H(q0), cirq.measure(q0))
+    with pytest.raises(ValueError):
+        sim.simulate_expectation_values(term, cirq.Z(q0))
+    seen = {round(float(np.real(sim.simulate_expectation_values(term, cirq.Z(q0), permit_terminal_measurements=True)[0])))
+            for _ in range(40)}
+    assert seen == {-1, 1}
+    # a mid-circuit measurement is not terminal: one sampled trajectory, the same as simulate's
+    mid = cirq.Circuit(cirq.H(q0), cirq.measure(q0, key="m"), cirq.H(q0), cirq.CNOT(q0, q1), cirq.ry(0.4).on(q1))
+    for seed in range(4):
+        ev = cirq.Simulator(seed=seed).simulate_expectation_values(mid, [cirq.Z(q0) * cirq.Z(q1), cirq.X(q1)])
+        psi = cirq.Simulator(seed=seed).simulate(mid).final_state_vector.astype(np.complex128)
+        qs = [q0, q1]
+        want = [sum(O.expectation_from_state_vector(psi, t, 2) for t in
+                    [(complex(x.coefficient), {qs.index(q): p._name for q, p in x.items()}) for x in cirq.PauliSum.wrap(o)])
+                for o in (cirq.Z(q0) * cirq.Z(q1), cirq.X(q1))]
+        np.testing.assert_allclose(np.array(ev, dtype=complex), want, atol=1e-6)

@@ -52,7 +52,7 @@ def test_grid_circuit_matches_oracle(gpu, n, depth, seed):
     assert fidelity(got, want) >= 1 - 1e-6
 
 
-@pytest.mark.parametrize("tile_bits", [12, 13])
+@pytest.mark.parametrize("tile_bits", [12, 13, 14])
 @pytest.mark.parametrize("kind", ["grid", "random"])
 def test_wide_tiles_match_oracle(gpu, tile_bits, kind):
     """Tiles of 12 and 13 bits (16 elements per thread, 4 register bits) on a 15-qubit state: the

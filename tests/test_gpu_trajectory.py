@@ -359,3 +359,31 @@ def test_select_then_kraus_weights_matches_two_kernels(gpu, same_bit):
     w_two = engine.kraus_weights_batch_dev(y, n, R, [b_bit], mb, nb, ws)
     assert torch.equal(x, y)
     assert np.max(np.abs(w_fused - w_two) / np.maximum(np.abs(w_two), 1e-300)) < 1e-12
+
+
+@pytest.mark.parametrize("batch", [None, 1])
+def test_trajectories_match_oracle_loop(gpu, batch):
+    """Records of run() with a mid-circuit measurement and mixtures / Kraus channels on a state
+    vector against the oracle's restatement of Cirq's per-repetition loop (complex64, merged state,
+    the same RandomState): every key's record equal in >= 99.9% of repetitions (a draw within
+    rounding of a probability boundary may differ), batched and one repetition at a time."""
+    from tests.helpers import to_oracle_moments
+
+    q = cirq.LineQubit.range(5)
+    c = cirq.Circuit(
+        cirq.H.on_each(*q), cirq.CNOT(q[0], q[1]), cirq.ry(0.7).on(q[2]), cirq.measure(q[0], key="a"),
+        cirq.depolarize(0.1).on(q[1]), cirq.amplitude_damp(0.25).on(q[2]), cirq.CZ(q[1], q[2]), cirq.X(q[0]) ** 0.3,
+        cirq.phase_damp(0.2).on(q[3]), cirq.asymmetric_depolarize(0.05, 0.1, 0.02).on(q[4]), cirq.CNOT(q[3], q[4]),
+        cirq.measure(q[1], q[0], key="c", invert_mask=(True,)), cirq.ry(0.3).on(q[0]), cirq.measure(*q, key="b"),
+    )
+    reps = 2000 if batch is None else 300
+    _log(f"trajectories vs oracle loop, batch={batch}, {reps} repetitions")
+    got = cirq.Simulator(seed=11, trajectory_batch=batch).run(c, repetitions=reps).records
+    want = O.run_trajectories(to_oracle_moments(c, q), 5, reps, np.random.RandomState(11))
+    assert set(got) == set(want)
+    same = np.ones(reps, dtype=bool)
+    for k in want:
+        assert got[k].shape == want[k].shape, k
+        same &= np.all(got[k] == want[k], axis=(1, 2))
+    _log(f"  {int(same.sum())} of {reps} repetitions identical")
+    assert same.mean() >= 0.999
