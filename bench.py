@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--sweep-points", type=int, default=16)
     ap.add_argument("--sweep-looped-circuits", type=int, default=256,
                     help="circuits whose first point the looped comparison runs (scaled to all points)")
+    ap.add_argument("--leg", choices=sorted(LEGS), default=None,
+                    help="run only this leg (no main workload) and print its record: for profiling one leg")
     return ap.parse_args()
 
 
@@ -627,6 +629,11 @@ def q32_leg(args):
     }
 
 
+LEGS = {"q32": lambda a: q32_leg(a), "adjoint": lambda a: adjoint_leg(a), "qaoa": lambda a: qaoa_leg(a),
+        "dm_adjoint": lambda a: dm_adjoint_leg(a), "sweep": lambda a: sweep_leg(a),
+        "trajectories": lambda a: trajectory_leg(a), "density_matrix": lambda a: dm_leg(a)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -649,7 +656,9 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n, "copies": 5}))
         return
 
-    from cirq_amd import generators
+    if args.leg:
+        print(json.dumps({args.leg: LEGS[args.leg](args)}), flush=True)
+        return
 
     g = grid_workload(args, args.rows, args.cols, args.depth, args.reps, args.steps, args.warmup,
                       autotune=not args.no_autotune)

@@ -1312,8 +1312,166 @@ def _batch_group(gate):
     return None
 
 
+class _ExprBatch:
+    """Values of a list of parameter expressions at B points, as one (len, B) array: affine
+    expressions (a * symbol + b, the usual case) in one vectorised step, the rest one by one."""
+
+    def __init__(self, exprs: Sequence, ev: ParamEvaluator):
+        self.n = len(exprs)
+        comp = []
+        for e in exprs:
+            c = ev._fns.get(e)
+            if c is None:
+                c = ev._fns[e] = ev._compile(e)
+            comp.append(c)
+        self.aff = np.array([i for i, c in enumerate(comp) if c[0] == "affine"], dtype=np.int64)
+        self.aff_sym = np.array([comp[i][1][0] for i in self.aff], dtype=np.int64)
+        self.aff_a = np.array([comp[i][1][1] for i in self.aff], dtype=np.float64)
+        self.aff_b = np.array([comp[i][1][2] for i in self.aff], dtype=np.float64)
+        self.const = np.array([i for i, c in enumerate(comp) if c[0] == "const"], dtype=np.int64)
+        self.const_v = np.array([comp[i][1] for i in self.const], dtype=np.float64)
+        self.other = [(i, c) for i, c in enumerate(comp) if c[0] not in ("affine", "const")]
+
+    def __call__(self, symvals: np.ndarray) -> np.ndarray:
+        out = np.empty((self.n, symvals.shape[1]))
+        if self.aff.size:
+            out[self.aff] = self.aff_a[:, None] * symvals[self.aff_sym] + self.aff_b[:, None]
+        if self.const.size:
+            out[self.const] = self.const_v[:, None]
+        for i, c in self.other:
+            out[i] = ParamEvaluator._eval(c, symvals)
+        return out
+
+
+class ParamPlan:
+    """`fill_param_mats` compiled once per (program, evaluator).
+
+    Every parameterised entry of a program's matrix buffer is a transform of one gate's matrix
+    (or of its derivative): the lowering's payload transforms (conjugate transposes, target
+    permutations, control embeddings, diagonal extraction, derivative payloads) are affine in the
+    gate matrix's real and imaginary parts.  Each entry's affine map is read off its transform
+    once, from the images of 0 and of the 2 d^2 real unit matrices, and checked on a random
+    matrix; the maps of all affine entries form one sparse matrix, so filling the buffer is one
+    batched matrix evaluation per gate family, one sparse product and one scatter.  Entries whose
+    transform is not affine (the FP16 split of an MFMA block payload, density-matrix diagonals
+    d (x) conj(d)) keep the per-entry call."""
+
+    def __init__(self, prog: PackedProgram, ev: ParamEvaluator):
+        import scipy.sparse
+
+        self.base = prog.mats_arr
+        refs: dict = {}
+        for _off, ref, _size in prog.param_entries:
+            refs.setdefault((id(ref.gate), ref.deriv), ref)
+        grouped: dict = {}
+        for key, ref in refs.items():
+            g = _batch_group(ref.gate)
+            grouped.setdefault(key if g is None else (g, ref.deriv), []).append(key)
+        # groups: (representative ref, keys, one _ExprBatch per parameter position)
+        self.groups = []
+        where: dict = {}  # key -> (group index, position in group)
+        for keys in grouped.values():
+            rep = refs[keys[0]]
+            exprs = [_ExprBatch([refs[k].exprs[p] for k in keys], ev) for p in range(len(rep.exprs))]
+            for j, k in enumerate(keys):
+                where[k] = (len(self.groups), j)
+            self.groups.append((rep, keys, exprs))
+        # input layout: group blocks of (G, d, d) complex, as float64 pairs
+        zero = np.zeros((len(ev.symbols), 1))
+        self.in_off, n_in = [], 0
+        dims = []
+        for rep, keys, exprs in self.groups:
+            mb = self._group_matrices(rep, exprs, zero)
+            d = mb.shape[-1]
+            dims.append(d)
+            self.in_off.append(n_in)
+            n_in += len(keys) * 2 * d * d
+        self.n_in = n_in
+        rng = np.random.default_rng(0)
+        rows, cols, vals, crow, cval = [], [], [], [], []
+        self.nonaffine = []
+        for off, ref, size in prog.param_entries:
+            gi, j = where[(id(ref.gate), ref.deriv)]
+            d = dims[gi]
+            nq = 2 * d * d
+            if d > 8:  # wide gates: not worth 2 d^2 probe calls
+                self.nonaffine.append((off, ref, size))
+                continue
+            probe = np.zeros((1 + nq, d, d), dtype=np.complex128)
+            for q in range(nq):
+                probe[1 + q].reshape(-1)[q // 2] = 1.0 if q % 2 == 0 else 1.0j
+            img = np.asarray(ref.transform(probe), dtype=np.complex128).reshape(1 + nq, -1)
+            c, A = img[0], img[1:] - img[0]
+            y = rng.standard_normal((2, d, d)) + 1j * rng.standard_normal((2, d, d))
+            want = np.asarray(ref.transform(y), dtype=np.complex128).reshape(2, -1)
+            yr = y.reshape(2, -1).view(np.float64)
+            got = c + yr @ A
+            if img.shape[1] != size or not np.allclose(got, want, rtol=0, atol=1e-12 * (1 + np.abs(want).max())):
+                self.nonaffine.append((off, ref, size))
+                continue
+            base_col = self.in_off[gi] + j * nq
+            for part, M in ((0, A.real), (1, A.imag)):
+                q, m = np.nonzero(M)
+                rows.append(2 * (off + m) + part)
+                cols.append(base_col + q)
+                vals.append(M[q, m])
+            crow.append(2 * (off + np.arange(size)))
+            cval.append(c.real)
+            crow.append(2 * (off + np.arange(size)) + 1)
+            cval.append(c.imag)
+        self.dst = np.concatenate(crow) if crow else np.zeros(0, dtype=np.int64)  # float32 positions written
+        self.const = np.concatenate(cval) if cval else np.zeros(0)
+        pos = np.full(2 * self.base.size, -1, dtype=np.int64)
+        pos[self.dst] = np.arange(self.dst.size)
+        r = np.concatenate(rows) if rows else np.zeros(0, dtype=np.int64)
+        self.S = scipy.sparse.csr_matrix(
+            (np.concatenate(vals) if vals else np.zeros(0), (pos[r], np.concatenate(cols) if cols else r)),
+            shape=(self.dst.size, n_in),
+        )
+
+    @staticmethod
+    def _group_matrices(rep, exprs, symvals):
+        """(G * B, d, d) matrices (or derivatives) of the group's gates, gate-major."""
+        vals = tuple(e(symvals).reshape(-1) for e in exprs)
+        if rep.deriv is None:
+            return np.asarray(rep.gate._matrix_batch(vals))
+        return np.asarray(rep.gate._dmatrix_batch(vals, rep.deriv))
+
+    def __call__(self, symvals: np.ndarray) -> np.ndarray:
+        B = symvals.shape[1]
+        out = np.broadcast_to(self.base, (B, self.base.size)).copy()
+        x = np.empty((B, self.n_in))
+        mats = {}
+        for (rep, keys, exprs), off in zip(self.groups, self.in_off):
+            mb = self._group_matrices(rep, exprs, symvals)
+            d = mb.shape[-1]
+            mb = mb.reshape(len(keys), B, d, d)
+            x[:, off : off + len(keys) * 2 * d * d] = (
+                np.ascontiguousarray(mb.transpose(1, 0, 2, 3)).reshape(B, -1).view(np.float64))
+            if self.nonaffine:
+                for j, k in enumerate(keys):
+                    mats[k] = mb[j]
+        if self.dst.size:
+            y = (self.S @ x.T).T + self.const
+            out.view(np.float32)[:, self.dst] = y
+        for off, ref, size in self.nonaffine:
+            payload = ref.transform(mats[(id(ref.gate), ref.deriv)])
+            out[:, off : off + size] = payload.reshape(B, -1)
+        return out
+
+
 def fill_param_mats(prog: PackedProgram, ev: ParamEvaluator, symvals: np.ndarray) -> np.ndarray:
-    """(B, mats) complex64: the program's matrix buffer for each of B resolvers."""
+    """(B, mats) complex64: the program's matrix buffer for each of B resolvers (through the
+    program's ParamPlan for this evaluator, built on first use)."""
+    plan = getattr(prog, "_param_plan", None)
+    if plan is None or plan[0] is not ev:
+        plan = (ev, ParamPlan(prog, ev))
+        prog._param_plan = plan
+    return plan[1](symvals)
+
+
+def fill_param_mats_direct(prog: PackedProgram, ev: ParamEvaluator, symvals: np.ndarray) -> np.ndarray:
+    """fill_param_mats by calling every entry's transform (the definition ParamPlan compiles)."""
     B = symvals.shape[1]
     out = np.broadcast_to(prog.mats_arr, (B, prog.mats_arr.size)).copy()
     refs: dict = {}
