@@ -144,6 +144,9 @@ class AdjointProgram:
         varying = [(x, z, c) for x, z, c in self.terms if x or z]
         self.phi_scale = adjoint_scale(c for _, _, c in varying)
         self.phi_terms = [(x, z, c / self.phi_scale) for x, z, c in varying]
+        # <H> = s Re<psi|phi> + c_I <psi|psi> (c_I: the identity terms' coefficient, real for a
+        # Hermitian H): two inner products instead of one reduction per group of terms
+        self.identity_coef = float(sum(complex(c).real for x, z, c in self.terms if not x and not z))
         self.evaluator = ParamEvaluator(low.param_refs)
         self.symbols = list(self.evaluator.symbols)
         L = min(tile_bits if tile_bits is not None else 12, n)
@@ -238,6 +241,7 @@ class AdjointGradient:
         G = self.prog.n_slots
         self.partial = torch.zeros(max(G, 1) * self.prog.partial_stride, dtype=torch.float64, device=self.device)
         self.gslot = torch.zeros(max(G, 1), dtype=torch.float64, device=self.device)
+        self.ev_buf = torch.zeros(4, dtype=torch.float64, device=self.device)
 
     def _forward(self):
         check(lib().cqs_init_basis(ptr(self.psi), self.n, 0, engine.stream_handle()), "cqs_init_basis")
@@ -251,9 +255,26 @@ class AdjointGradient:
         return self.psi
 
     def value(self, values) -> float:
-        """<H> alone: the forward passes and one expectation, no reverse sweep."""
-        psi = self.forward_state(values)
-        return float(engine.expect_terms(psi, self.n, False, self.prog.terms, self.ws).sum().real.item())
+        """<H> alone: the forward passes and the expectation, no reverse sweep (the same arithmetic
+        as value_and_grad's value)."""
+        self.forward_state(values)
+        return self._expectation_value(self._queue_expectation().cpu().numpy())
+
+    def _queue_expectation(self) -> torch.Tensor:
+        """Queues phi = (H' / s) psi and the inner products <psi|phi>, <psi|psi> into a device
+        float64[4]; _expectation_value turns them into <H> = s Re<psi|phi> + c_I <psi|psi>."""
+        prog, n = self.prog, self.n
+        engine.apply_paulisum(self.psi, self.phi, n, prog.phi_terms)
+        engine.inner(self.psi, self.phi, n, self.ws, out=self.ev_buf[0:2])
+        if prog.identity_coef:
+            engine.inner(self.psi, self.psi, n, self.ws, out=self.ev_buf[2:4])
+        return self.ev_buf
+
+    def _expectation_value(self, buf: np.ndarray) -> float:
+        ev = float(buf[0]) * self.prog.phi_scale
+        if self.prog.identity_coef:
+            ev += self.prog.identity_coef * float(buf[2])
+        return ev
 
     def value_and_grad(self, values) -> tuple[float, np.ndarray]:
         """(<H>, d<H>/d symbol in `symbols` order) at the given symbol values."""
@@ -263,9 +284,7 @@ class AdjointGradient:
         self.dev_fwd.set_mats(mf)
         self.dev_rev.set_mats(mr)
         self._forward()
-        evs = engine.expect_terms(self.psi, n, False, prog.terms, self.ws)
-        ev = evs.sum()
-        engine.apply_paulisum(self.psi, self.phi, n, prog.phi_terms)
+        evbuf = self._queue_expectation()
         G = prog.n_slots
         rv = prog.rev
         check(
@@ -278,9 +297,9 @@ class AdjointGradient:
         )
         check(lib().cqs_grad_reduce(ptr(self.partial), G, prog.partial_stride, ptr(self.gslot), engine.stream_handle()),
               "cqs_grad_reduce")
+        ev = self._expectation_value(evbuf.cpu().numpy())
         gs = self.gslot.cpu().numpy()[:G] * prog.phi_scale
-        self.last_term_values = evs.real.cpu().numpy()  # <coef_t P_t> of every term of H
-        return float(ev.real.item()), prog.chain(gs, symvals)
+        return ev, prog.chain(gs, symvals)
 
     def parameter_shift(self, values) -> tuple[float, np.ndarray]:
         """The same gradient by the two-term shift rule: two forward simulations and <H> per

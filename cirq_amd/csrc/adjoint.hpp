@@ -144,37 +144,20 @@ __device__ __forceinline__ double grad_reg_dense(const float2 (&x)[PER], const f
 template <int PER, int NT, class Prog>
 __device__ __forceinline__ double grad_reg_diag(const float2 (&x)[PER], const float2 (&f)[PER], const Prog& P,
                                                 sop_t& op, uint64_t base) {
-  constexpr int TB = __builtin_ctz(NT);
-  constexpr int NR = __builtin_ctz(PER);
-  const int tid = threadIdx.x;
-  int fixed = 0;
-  int w[NR > 0 ? NR : 1];
-#pragma unroll
-  for (int r = 0; r < (NR > 0 ? NR : 1); ++r) w[r] = 0;
-#pragma unroll
-  for (int m = 0; m < 6; ++m) {  // k <= 6
-    if (m >= op.k) break;
-    const int t = op.t[m];
-    if (t >= kOuter) {
-      fixed |= (int)((base >> (t - kOuter)) & 1ull) << m;
-    } else if (t < TB) {
-      fixed |= ((tid >> t) & 1) << m;
-    } else {
-#pragma unroll
-      for (int r = 0; r < NR; ++r)
-        if (t - TB == r) w[r] |= 1 << m;
-    }
-  }
-  const float2* Dg = P.mats + op.mat;
+  const DiagTable<PER, NT, Prog> D(P, op, base);
   double acc = 0.0;
+  if (D.small) {
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    int idx = fixed;
+    for (int e = 0; e < PER; ++e) {
+      const float2 y = cmul(D.pick(D.index(e)), x[e]);
+      acc += (double)f[e].x * y.x + (double)f[e].y * y.y;
+    }
+  } else {
 #pragma unroll
-    for (int r = 0; r < NR; ++r)
-      if ((e >> r) & 1) idx |= w[r];
-    const float2 y = cmul(Dg[idx], x[e]);
-    acc += (double)f[e].x * y.x + (double)f[e].y * y.y;
+    for (int e = 0; e < PER; ++e) {
+      const float2 y = cmul(D.Dg[D.index(e)], x[e]);
+      acc += (double)f[e].x * y.x + (double)f[e].y * y.y;
+    }
   }
   return 2.0 * acc;
 }
@@ -207,6 +190,7 @@ __device__ __forceinline__ void adjoint_tile(float2* __restrict__ psi, float2* _
   float2* tB = lds + tile_words(L);
   float2* mstage = lds + 2 * tile_words(L);
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
+  warm_program<NT>(P);
   float2 va[PER], vb[PER];
   {  // offsets are recomputed for the store, so they are not live across the program
     uint64_t off_lo, off_hi[PER];

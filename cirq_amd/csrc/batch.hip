@@ -41,6 +41,7 @@ __device__ __forceinline__ ProgView view_of(const cqs_batch_pass& B, const cqs_b
   P.op_count = B.op_count;
   P.max_k = B.max_k;
   P.mscale = 32768.f;
+  P.warm = 0;
   P.ops = ops + B.op_begin;
   P.mats = mats + I.mats;
   P.aux = aux;

@@ -72,10 +72,32 @@ __device__ __forceinline__ int insert0(int x, int b) {
   return ((x >> b) << (b + 1)) | (x & ((1 << b) - 1));
 }
 
-// FP64 block reduction of one value per thread (WG threads); result valid in thread 0.
+// v + (v moved by one DPP pattern), FP64 as two 32-bit DPP moves; lanes whose source is outside
+// the row, or whose row is masked off, add +0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, ROW_MASK, 0xf, true);
+  return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum of v over the 64 lanes of a wave, returned to every lane.  Row prefix sums by DPP row shifts
+// (1, 2, 4, 8: lane 15 of each 16-lane row holds its row), then row_bcast:15 and row_bcast:31 fold
+// the rows into lane 63, which is read back as a uniform value: register moves only, where a
+// butterfly of shuffles would make six dependent round trips through the LDS crossbar.  The
+// summation order is fixed, so results are deterministic.
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = dpp_add<0x111, 0xf>(v);  // row_shr:1
+  v = dpp_add<0x112, 0xf>(v);  // row_shr:2
+  v = dpp_add<0x114, 0xf>(v);  // row_shr:4
+  v = dpp_add<0x118, 0xf>(v);  // row_shr:8
+  v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)x, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 }  // namespace cqs

@@ -68,6 +68,47 @@ __global__ __launch_bounds__(256) void k_expect(const float2* __restrict__ amps,
   }
 }
 
+// partials[2 * blockIdx + {0,1}] = this block's share of sum_i conj(a_i) b_i (FP64, fixed order).
+__global__ __launch_bounds__(256) void k_inner(const float2* __restrict__ a, const float2* __restrict__ b,
+                                               uint64_t N, double* __restrict__ partials) {
+  __shared__ double red[4][2];
+  double sr = 0.0, si = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+    const float2 x = a[i], y = b[i];
+    const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+    sr += xr * yr + xi * yi;
+    si += xr * yi - xi * yr;
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[w][0] = sr;
+    red[w][1] = si;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int c = threadIdx.x;
+    partials[2 * blockIdx.x + c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_inner_final(const double* __restrict__ partials, int nblocks,
+                                                    double* __restrict__ out) {
+  double sr = 0.0, si = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 64) {
+    sr += partials[2 * b];
+    si += partials[2 * b + 1];
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if (threadIdx.x == 0) {
+    out[0] = sr;
+    out[1] = si;
+  }
+}
+
 struct FinalBatch {
   int count;
   int nblocks;
@@ -362,6 +403,18 @@ int cqs_norm2(const void* amps, int32_t n_bits, double* out, void* workspace, vo
   const uint64_t x = 0, z = 0;
   const double c[2] = {1.0, 0.0};
   return cqs_expect_paulis(amps, n_bits, 0, 1, &x, &z, c, out, workspace, stream);
+}
+
+int cqs_inner(const void* a, const void* b, int32_t n_bits, double* out, void* workspace, void* stream) {
+  CQS_REQUIRE(a != nullptr && b != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
+  CQS_REQUIRE(n_bits >= 0 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
+  const uint64_t N = 1ull << n_bits;
+  hipStream_t s = as_stream(stream);
+  const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(kRedBlocks, (N + 255) / 256));
+  hipLaunchKernelGGL(k_inner, dim3(grid), dim3(256), 0, s, (const float2*)a, (const float2*)b, N, (double*)workspace);
+  hipLaunchKernelGGL(k_inner_final, dim3(1), dim3(64), 0, s, (const double*)workspace, grid, out);
+  CQS_HIP(hipGetLastError());
+  return CQS_OK;
 }
 
 int cqs_apply_paulisum(const void* psi, void* phi, int32_t n_qubits, int32_t n_terms, const uint64_t* xmask,

@@ -75,6 +75,9 @@ static int grid_for(uint64_t n) {
 }
 
 // Fill a TileProg for one pass over a state of n bits.  Returns false on a bad pass.
+// Passes of at most 2^kWarmOuterBits tiles (a few workgroups per CU) warm their program first.
+constexpr int kWarmOuterBits = 11;
+
 bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const float2* mats,
                const int32_t* aux) {
   if (ps.L < 0 || ps.L > 14 || ps.L > n) {
@@ -114,6 +117,7 @@ bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const 
   }
   P.max_k = ps.max_k;
   P.mscale = 32768.f;
+  P.warm = P.nouter <= kWarmOuterBits;
   P.basis = -1;
   P.ops = ops + ps.op_begin;
   P.mats = mats;

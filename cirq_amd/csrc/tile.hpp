@@ -22,6 +22,7 @@ struct TileProg {
   int32_t op_count;  // ops in this pass
   int32_t max_k;     // largest VALU dense k the LDS matrix stage was sized for
   float mscale;      // amplitude scale of the split-FP16 MFMA path (2^15 for normalised states)
+  int32_t warm;      // nonzero: touch the pass's op descriptors and matrices first (warm_program)
   int32_t lpos[CQS_MAX_LOCAL];      // tile bit -> flat bit when the pass loads
   int32_t lpos_out[CQS_MAX_LOCAL];  // tile bit -> flat bit when it stores (after PERMUTE ops)
   int32_t opos[CQS_MAX_BITS];
@@ -37,6 +38,7 @@ struct ProgView {
   int32_t op_count;
   int32_t max_k;
   float mscale;
+  int32_t warm;
   const cqs_op* ops;
   const float2* mats;
   const int32_t* aux;
@@ -331,47 +333,87 @@ __device__ __forceinline__ void run_mixing_op(float2* tile, float2* mstage, cons
 // complex multiply with everything else decided once per op.
 
 // Diagonal over k bits (tile-local or outer): index bits from thread / outer bits form `fixed`
-// (per lane); index bits from register bit r contribute w[r] (uniform).
+// (per lane); index bits from register bit r contribute w[r] (uniform), so element e reads entry
+// fixed | sub(e).  Tables of at most 4 entries (one- and two-qubit diagonals: Rz, ZZ, CZPow) are
+// read through scalar loads and picked per lane with selects, so the op waits on no vector load
+// (a per-lane gather costs a memory latency per op, which few workgroups per CU cannot hide);
+// larger tables are gathered per lane.
 template <int PER, int NT, class Prog>
-__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, sop_t& op, uint64_t base) {
-  constexpr int TB = __builtin_ctz(NT);
-  constexpr int NR = __builtin_ctz(PER);
-  const int tid = threadIdx.x;
+struct DiagTable {
+  static constexpr int TB = __builtin_ctz(NT);
+  static constexpr int NR = __builtin_ctz(PER);
   int fixed = 0;
   int w[NR > 0 ? NR : 1];
-#pragma unroll
-  for (int r = 0; r < (NR > 0 ? NR : 1); ++r) w[r] = 0;
   int any_reg = 0;
+  bool small;
+  float2 d0, d1, d2, d3;
+  const float2* Dg;
+
+  __device__ __forceinline__ DiagTable(const Prog& P, sop_t& op, uint64_t base) {
+    const int tid = threadIdx.x;
 #pragma unroll
-  for (int m = 0; m < 6; ++m) {  // k <= 6; unrolled so the target loads issue together
-    if (m >= op.k) break;
-    const int t = op.t[m];
-    if (t >= kOuter) {
-      fixed |= (int)((base >> (t - kOuter)) & 1ull) << m;
-    } else if (t < TB) {
-      fixed |= ((tid >> t) & 1) << m;
-    } else {
-      any_reg = 1;
+    for (int r = 0; r < (NR > 0 ? NR : 1); ++r) w[r] = 0;
 #pragma unroll
-      for (int r = 0; r < NR; ++r)
-        if (t - TB == r) w[r] |= 1 << m;
+    for (int m = 0; m < 6; ++m) {  // k <= 6; unrolled so the target loads issue together
+      if (m >= op.k) break;
+      const int t = op.t[m];
+      if (t >= kOuter) {
+        fixed |= (int)((base >> (t - kOuter)) & 1ull) << m;
+      } else if (t < TB) {
+        fixed |= ((tid >> t) & 1) << m;
+      } else {
+        any_reg = 1;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          if (t - TB == r) w[r] |= 1 << m;
+      }
+    }
+    small = op.k <= 2;
+    Dg = P.mats + op.mat;
+    if (small) {
+      smat_t* Ds = scalar_ptr(P.mats) + op.mat;
+      d0 = ldm(Ds, 0);
+      d1 = ldm(Ds, 1);
+      d2 = d0;
+      d3 = d1;
+      if (op.k == 2) {
+        d2 = ldm(Ds, 2);
+        d3 = ldm(Ds, 3);
+      }
     }
   }
-  const float2* Dg = P.mats + op.mat;
-  if (!any_reg) {
-    const float2 d = Dg[fixed];
+  __device__ __forceinline__ int index(int e) const {
+    int idx = fixed;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if ((e >> r) & 1) idx |= w[r];
+    return idx;
+  }
+  __device__ __forceinline__ float2 pick(int idx) const {
+    const float2 lo = (idx & 1) ? d1 : d0;
+    const float2 hi = (idx & 1) ? d3 : d2;
+    return (idx & 2) ? hi : lo;
+  }
+  // entry of element e (uniform branch on `small`)
+  __device__ __forceinline__ float2 at(int e) const { return small ? pick(index(e)) : Dg[index(e)]; }
+};
+
+template <int PER, int NT, class Prog>
+__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, sop_t& op, uint64_t base) {
+  const DiagTable<PER, NT, Prog> D(P, op, base);
+  if (!D.any_reg) {
+    const float2 d = D.small ? D.pick(D.fixed) : D.Dg[D.fixed];
 #pragma unroll
     for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], d);
     return;
   }
+  if (D.small) {
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    int sub = 0;
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-      if ((e >> r) & 1) sub |= w[r];
-    v[e] = cmul(v[e], Dg[fixed | sub]);
+    for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], D.pick(D.index(e)));
+    return;
   }
+#pragma unroll
+  for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], D.Dg[D.index(e)]);
 }
 
 // Products of phases on bit pairs (a CZ layer).  Pairs on thread / outer bits only multiply one
@@ -621,6 +663,21 @@ __device__ __forceinline__ void tile_offsets(const int32_t* pos, uint64_t& off_l
   }
 }
 
+// With few workgroups per CU (small states) the interpreter's op reads miss every cache: each op
+// waits for its descriptor and then its matrix through dependent scalar loads from memory.  With
+// P.warm set, the workgroup first touches every op's descriptor and the first line of its matrix
+// with vector loads, all in flight together, so that those scalar loads find them in L2.
+template <int NT, class Prog>
+__device__ __forceinline__ void warm_program(const Prog& P) {
+  if (!P.warm) return;
+  for (int i = threadIdx.x; i < P.op_count; i += NT) {
+    const int32_t* o = (const int32_t*)(P.ops + i);
+    const int kind = *(volatile const int32_t*)(o + 0) & 0xff;
+    const int mat = *(volatile const int32_t*)(o + 2);
+    if (kind != CQS_OP_PERMUTE) (void)*(volatile const float*)(P.mats + mat);
+  }
+}
+
 // One workgroup's whole pass: load the tile from HBM (layout lpos) into registers, run the
 // program, store it (layout lpos_out).  `base` is the flat offset of the tile's outer bits.  With
 // basis >= 0 nothing is read: the tile starts as its part of the basis state |basis>.
@@ -632,6 +689,7 @@ __device__ __forceinline__ void pass_tile(float2* __restrict__ psi, const Prog& 
   float2* tile = lds;
   float2* mstage = lds + tile_words(L);
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
+  warm_program<NT>(P);
   float2 v[PER];
   {
     uint64_t off_lo, off_hi[PER];

@@ -134,6 +134,18 @@ def norm2(state: torch.Tensor, n_bits: int, ws: Workspace) -> float:
     return float(out[0].item())
 
 
+def inner(a: torch.Tensor, b: torch.Tensor, n_bits: int, ws: Workspace, out: torch.Tensor | None = None):
+    """Queues <a|b> over 2^n_bits entries (FP64 accumulation) into out[0:2] (re, im; a new device
+    float64 tensor when out is None) and returns out."""
+    if out is None:
+        out = torch.zeros(2, dtype=torch.float64, device=a.device)
+    if out.dtype != torch.float64 or out.numel() < 2 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float64 tensor of at least 2 entries")
+    w = ws.get(lib().cqs_expect_workspace_bytes(1))
+    check(lib().cqs_inner(ptr(a), ptr(b), n_bits, ptr(out), ptr(w), stream_handle()), "cqs_inner")
+    return out
+
+
 def scale(state: torch.Tensor, n_bits: int, c: complex):
     check(lib().cqs_scale(ptr(state), n_bits, float(c.real), float(c.imag), stream_handle()), "cqs_scale")
 

@@ -311,6 +311,13 @@ int cqs_apply_paulisum(const void* psi, void* phi, int32_t n_qubits, int32_t n_t
                        const uint64_t* xmask, const uint64_t* zmask, const double* coef,
                        void* stream);
 
+/* out[0] + i out[1] = sum_i conj(a[i]) b[i] over 2^n_bits complex64 entries, FP64 accumulation in
+ * a fixed order (DEVICE doubles); workspace: DEVICE scratch of cqs_expect_workspace_bytes(1) bytes.
+ * The adjoint gradient's <H> = s Re<psi|phi> once phi = (H / s) psi is formed, the expectation
+ * cirq-core/cirq/ops/linear_combinations.py:659-719 (PauliSum.expectation_from_state_vector)
+ * computes term by term. */
+int cqs_inner(const void* a, const void* b, int32_t n_bits, double* out, void* workspace, void* stream);
+
 /* out[0] = sum_i |amps[i]|^2 in FP64 (DEVICE double).  Norm check of
  * cirq-core/cirq/qis/states.py:865-897 and cirq-core/cirq/sim/state_vector_simulator.py:131-146. */
 int cqs_norm2(const void* amps, int32_t n_bits, double* out, void* workspace, void* stream);
