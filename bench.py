@@ -253,6 +253,36 @@ def sweep_leg(args):
         "max_abs_diff_batched_vs_looped": max(diffs),
         "compile_s": t_compile,
         "launches": len(be.host.launches),
+        "roofline": _sweep_bytes_roofline(
+            [(p.n, int(p.fwd.n_passes), int(p.rev.n_passes), int(v.shape[0])) for p, v in zip(be.host.progs, values)],
+            t_batch, "every point's forward and reverse passes over the batched wall time (one replay of the "
+            "captured launches); states of 10-22 qubits stay in the last-level cache"),
+    }
+
+
+def _adjoint_roofline(ag, n_bits: int, point) -> dict:
+    """Roofline of the reverse-sweep kernel (k_adjoint_pass, the dominant one of a gradient): its
+    algorithmic bytes per pass (psi and phi each read and written once: 4 * 8 * 2^n) over its
+    average pass time, timed with HIP events on the stream the passes run on; the forward passes
+    beside it (read + write once each, the first writes only)."""
+    t = {}
+    ag.value_and_grad(point, timings=t)
+    n_rev, n_fwd = int(ag.prog.rev.n_passes), int(ag.prog.fwd.n_passes)
+    rev_bytes = 4 * 8 * 2**n_bits
+    fwd_bytes = (2 * n_fwd - 1) * 8 * 2**n_bits / max(n_fwd, 1)
+    achieved = rev_bytes / (t["reverse_ms"] / 1e3 / max(n_rev, 1)) / 1e9
+    fwd_gbs = fwd_bytes / (t["forward_ms"] / 1e3 / max(n_fwd, 1)) / 1e9
+    state_mib = 8 * 2**n_bits / 2**20
+    where = ("the two states stay in the 256 MiB last-level cache" if 2 * state_mib <= 128 else
+             "the two states fill the 256 MiB last-level cache and spill to HBM")
+    return {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None, "kernel": f"k_adjoint_pass<{int(ag.prog.tile_bits_rev)}>",
+        "reverse_ms": t["reverse_ms"], "reverse_passes": n_rev, "forward_ms": t["forward_ms"],
+        "forward_gbs": fwd_gbs,
+        "note": f"{state_mib:.0f} MiB per state, {2**(n_bits - int(ag.prog.tile_bits_rev))} tiles per reverse pass: "
+        f"{where}; the passes are bound by VALU work per op (about 3 interpreted ops per parameterised gate) "
+        "and by latency at this tile count, not by HBM bandwidth",
     }
 
 
@@ -295,6 +325,7 @@ def adjoint_leg(args):
         "value_diff": abs(v - v2),
         "forward_passes": int(ag.prog.fwd.n_passes),
         "reverse_passes": int(ag.prog.rev.n_passes),
+        "roofline": _adjoint_roofline(ag, args.adjoint_qubits, vals),
     }
     if not args.no_cpu_baseline:
         # Cirq has no adjoint: its gradient is the shift rule, 2 simulations per symbol
@@ -372,6 +403,7 @@ def qaoa_leg(args):
         "max_abs_grad_diff_vs_param_shift": float(np.abs(g - g2).max()),
         "forward_passes": int(ag.prog.fwd.n_passes),
         "reverse_passes": int(ag.prog.rev.n_passes),
+        "roofline": _adjoint_roofline(ag, args.qaoa_nodes, point),
     }
     if not args.no_cpu_baseline:
         per_layer = (len(circuit) - 1) // args.qaoa_p  # moments of one (ZZ layer, Rx layer) round
@@ -451,7 +483,22 @@ def dm_adjoint_leg(args):
         "max_abs_grad_diff_vs_param_shift": float(np.abs(g[:K] - g_ps).max()),
         "max_abs_grad_diff_checkpointed": float(np.abs(g - g_ck).max()),
         "value_diff_checkpointed": abs(v - vc),
+        "roofline": _sweep_bytes_roofline(
+            [(2 * n, int(ga.prog.fwd.n_passes), int(ga.prog.rev.n_passes), 1)], t_adj,
+            "whole gradient: forward passes (rho read + written once) and reverse passes (rho and lambda "
+            f"each read + written once) over its wall time; 2^{2 * n} entries = {8 * 4**n / 2**20:.0f} MiB per "
+            "matrix, resident in the last-level cache"),
     }
+
+
+def _sweep_bytes_roofline(items, seconds: float, note: str) -> dict:
+    """Whole-job roofline of gradient sweeps: items (n_bits, forward passes, reverse passes,
+    points); algorithmic bytes = per point (2 f - 1) * 8 * 2^n (forward; the first pass writes
+    only) + 32 * r * 2^n (reverse: two states read and written), over the wall time."""
+    total = sum(pts * ((2 * f - 1) * 8 * 2**nb + 32 * r * 2**nb) for nb, f, r, pts in items)
+    achieved = total / seconds / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "algorithmic_bytes": total, "note": note}
 
 
 def _median_s(fn, runs: int = 3, warm=None) -> float:
@@ -687,7 +734,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex64 (fp32 pairs); fp64 reductions",
+        "dtype": "complex64 (fp32 pairs); fused 4-qubit blocks on MFMA with split-FP16 operands (hi + lo, three "
+        "products accumulated in fp32, ~2^-22 relative per block); fp64 reductions",
         "data": "synthetic: seeded random grid circuit (Cirq's generator restated)",
         "config": {
             "workload": f"random grid circuit {args.rows}x{args.cols} ({n} qubits), depth {args.depth}, "

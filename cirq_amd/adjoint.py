@@ -95,6 +95,20 @@ def _grad_op(op: LOp, k: int, slot: int) -> LOp:
     return LOp("grad_dense", tuple(bits), None, (), ParamRef(ref.gate, controlled, ref.exprs, ref.slot, k), 0, slot)
 
 
+def default_tile_bits(n_bits: int) -> tuple[int, int]:
+    """(forward, reverse) tile widths of a gradient over n_bits state bits.  Small states have few
+    tiles, one workgroup or less per CU at 12-bit tiles, so narrower tiles buy occupancy at the
+    price of more passes; the reverse kernel holds two states and runs best one bit narrower.
+    Timed on MI355X over ansatz gradients of 14-24 qubits (value_and_grad, ms): 18 qubits
+    1.83 (10/10) vs 2.96 (12/12); 20 qubits 2.50 (11/11) vs 3.38; 22 qubits 7.73 (12/11) vs 7.86;
+    24 qubits 20.7 (12/11) vs 21.8."""
+    if n_bits <= 18:
+        return min(10, n_bits), min(10, n_bits)
+    if n_bits <= 21:
+        return 11, 11
+    return 12, 11
+
+
 def adjoint_waves(L: int) -> int:
     """Waves per workgroup of the adjoint kernels at tile width L: each writes its own partial of
     every GRAD op (256 threads up to L = 12, then 16 elements per thread)."""
@@ -149,8 +163,9 @@ class AdjointProgram:
         self.identity_coef = float(sum(complex(c).real for x, z, c in self.terms if not x and not z))
         self.evaluator = ParamEvaluator(low.param_refs)
         self.symbols = list(self.evaluator.symbols)
-        L = min(tile_bits if tile_bits is not None else 12, n)
-        La = min(adjoint_tile_bits if adjoint_tile_bits is not None else 12, n)
+        dL, dLa = default_tile_bits(n)
+        L = min(tile_bits if tile_bits is not None else dL, n)
+        La = min(adjoint_tile_bits if adjoint_tile_bits is not None else dLa, n)
         self.fwd = compile_ops(low.ops, n, L=L, coalesce=coalesce_bits, blocks=mfma_blocks)
         rev: list[LOp] = []
         self.grad_slots: list[tuple[ParamRef, int]] = []  # slot -> (forward op's ref, parameter index)
@@ -260,15 +275,17 @@ class AdjointGradient:
         self.forward_state(values)
         return self._expectation_value(self._queue_expectation().cpu().numpy())
 
-    def _queue_expectation(self) -> torch.Tensor:
+    def _queue_expectation(self, out: torch.Tensor | None = None) -> torch.Tensor:
         """Queues phi = (H' / s) psi and the inner products <psi|phi>, <psi|psi> into a device
-        float64[4]; _expectation_value turns them into <H> = s Re<psi|phi> + c_I <psi|psi>."""
+        float64[4] (`out`, default this object's buffer); _expectation_value turns them into
+        <H> = s Re<psi|phi> + c_I <psi|psi>."""
         prog, n = self.prog, self.n
+        out = self.ev_buf if out is None else out
         engine.apply_paulisum(self.psi, self.phi, n, prog.phi_terms)
-        engine.inner(self.psi, self.phi, n, self.ws, out=self.ev_buf[0:2])
+        engine.inner(self.psi, self.phi, n, self.ws, out=out[0:2])
         if prog.identity_coef:
-            engine.inner(self.psi, self.psi, n, self.ws, out=self.ev_buf[2:4])
-        return self.ev_buf
+            engine.inner(self.psi, self.psi, n, self.ws, out=out[2:4])
+        return out
 
     def _expectation_value(self, buf: np.ndarray) -> float:
         ev = float(buf[0]) * self.prog.phi_scale
@@ -276,15 +293,24 @@ class AdjointGradient:
             ev += self.prog.identity_coef * float(buf[2])
         return ev
 
-    def value_and_grad(self, values) -> tuple[float, np.ndarray]:
-        """(<H>, d<H>/d symbol in `symbols` order) at the given symbol values."""
+    def value_and_grad(self, values, timings: dict | None = None) -> tuple[float, np.ndarray]:
+        """(<H>, d<H>/d symbol in `symbols` order) at the given symbol values.  With a `timings`
+        dict, also fills timings['forward_ms'] / ['reverse_ms']: the forward and reverse programs'
+        GPU time between events on the stream they run on."""
         prog, n = self.prog, self.n
         symvals = prog.symbol_values(values)
         mf, mr = prog.mats(symvals)
         self.dev_fwd.set_mats(mf)
         self.dev_rev.set_mats(mr)
+        ev_t = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if timings is not None else None
+        if ev_t:
+            ev_t[0].record()
         self._forward()
+        if ev_t:
+            ev_t[1].record()
         evbuf = self._queue_expectation()
+        if ev_t:
+            ev_t[2].record()
         G = prog.n_slots
         rv = prog.rev
         check(
@@ -295,9 +321,14 @@ class AdjointGradient:
             ),
             "cqs_adjoint_passes",
         )
+        if ev_t:
+            ev_t[3].record()
         check(lib().cqs_grad_reduce(ptr(self.partial), G, prog.partial_stride, ptr(self.gslot), engine.stream_handle()),
               "cqs_grad_reduce")
         ev = self._expectation_value(evbuf.cpu().numpy())
+        if ev_t:
+            timings["forward_ms"] = ev_t[0].elapsed_time(ev_t[1])
+            timings["reverse_ms"] = ev_t[2].elapsed_time(ev_t[3])
         gs = self.gslot.cpu().numpy()[:G] * prog.phi_scale
         return ev, prog.chain(gs, symvals)
 
@@ -315,13 +346,13 @@ class AdjointGradient:
         padded[:, : rows.shape[1]] = rows
         allm = torch.from_numpy(padded.view(np.float32)).to(self.device)
         own = self.dev_fwd.mats
-        evs = []
+        bufs = torch.zeros((allm.shape[0], 4), dtype=torch.float64, device=self.device)
         try:
-            for r in range(allm.shape[0]):
+            for r in range(allm.shape[0]):  # <H> as value_and_grad computes it
                 self.dev_fwd.set_mats_device(allm[r])
                 self._forward()
-                evs.append(engine.expect_terms(self.psi, n, False, prog.terms, self.ws).sum().real)
-            ev = torch.stack(evs).cpu().numpy()
+                self._queue_expectation(bufs[r])
+            ev = np.array([self._expectation_value(b) for b in bufs.cpu().numpy()])
         finally:
             self.dev_fwd.set_mats_device(own)
         gs = (np.pi / 2) * (ev[0:-1:2] - ev[1:-1:2])

@@ -38,7 +38,7 @@ import torch
 
 from . import _lib, engine
 from ._lib import check, lib, ptr
-from .adjoint import _dagger, _grad_op, _inverse, adjoint_waves
+from .adjoint import _dagger, _grad_op, _inverse, adjoint_waves, default_tile_bits
 from .gates import is_parameterized
 from .paulis import PauliSum, adjoint_scale, pauli_masks
 from .qubits import QubitOrder
@@ -151,8 +151,9 @@ class DMAdjointProgram:
         S = len(steps)
         self.checkpoint_every = max(1, int(checkpoint_every) if checkpoint_every else math.isqrt(S - 1) + 1)
 
-        L = min(tile_bits if tile_bits is not None else 12, nb)
-        La = min(adjoint_tile_bits if adjoint_tile_bits is not None else 12, nb)
+        dL, dLa = default_tile_bits(nb)
+        L = min(tile_bits if tile_bits is not None else dL, nb)
+        La = min(adjoint_tile_bits if adjoint_tile_bits is not None else dLa, nb)
         fwd_passes: list[Pass] = []
         rev_passes: list[Pass] = []
         self.fwd_range: list[tuple[int, int]] = []
