@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-blocks", action="store_true", help="disable MFMA block fusion")
     ap.add_argument("--no-autotune", action="store_true",
                     help="compile the main circuit with the default tile / fusion settings instead of timing candidates")
+    ap.add_argument("--no-specialize", action="store_true",
+                    help="run the grid legs' passes through the op interpreter instead of compiled per-pass kernels")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-qubits", type=int, default=26)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
@@ -595,6 +597,11 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
     t0 = time.perf_counter()
     comp = sim._compile(circuit, qubits)
     t_compile = time.perf_counter() - t0
+    t_spec = None
+    if not args.no_specialize:  # the tuned program's passes as straight-line kernels (cirq_amd.specialize)
+        t0 = time.perf_counter()
+        comp.dev.specialize()
+        t_spec = time.perf_counter() - t0
     tune_info = None
     if tuner is not None:
         import dataclasses
@@ -652,7 +659,8 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
     norm = engine.norm2(st, n, sim._ws)  # FP64 on the GPU: a size-independent check of the passes
     return dict(n=n, n_ops=n_ops, zz=zz, comp=comp, bits=bits, vals=vals, wall=wall, pass_ms=pass_ms, n_pass=n_pass,
                 avg_launch_s=avg_launch_s, bytes_per_launch=bytes_per_launch, achieved=achieved,
-                ms_per_step=ms_per_step, layers_per_s=layers_per_s, tune_info=tune_info, norm=norm)
+                ms_per_step=ms_per_step, layers_per_s=layers_per_s, tune_info=tune_info, norm=norm,
+                specialize_s=t_spec)
 
 
 
@@ -673,6 +681,8 @@ def q32_leg(args):
         "pass_roofline_frac": g["achieved"] / HBM_PEAK_GBS,
         "norm_error": abs(g["norm"] - 1.0),
         "autotune": g["tune_info"],
+        "specialized_passes": g["specialize_s"] is not None,
+        "specialize_s": g["specialize_s"],
     }
 
 
@@ -715,12 +725,13 @@ def main():
     ms_per_step, layers_per_s, tune_info = g["ms_per_step"], g["layers_per_s"], g["tune_info"]
 
     traffic = None  # measured HBM bytes per launch (rocprofv3 PMC), if measured for this tile width
-    tpath = os.path.join(ROOT, "profiles", f"traffic_k_pass_{n}q.json")
+    spec = g["specialize_s"] is not None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{'k_spec' if spec else 'k_pass'}_{n}q.json")
     tile_l = int(comp.packed.pass_arr["L"].max())
     if os.path.exists(tpath):
         with open(tpath) as f:
             rec = json.load(f)
-        if rec.get("kernel") == f"k_pass<{tile_l}>":
+        if rec.get("tile_bits") == tile_l and bool(rec.get("specialized")) == spec:
             traffic = rec.get("bytes_per_launch")
 
     out = {
@@ -748,6 +759,7 @@ def main():
             "passes": n_pass,
             "tile_bits": int(comp.packed.pass_arr["L"].max()) if n_pass else 0,
             "mfma_blocks": int((comp.packed.ops_arr["kind"] == 5).sum()),
+            "specialized_passes": g["specialize_s"] is not None,
         },
         "roofline": {
             "bound": "hbm",
@@ -765,6 +777,7 @@ def main():
             "sample_check": int(bits.shape[0]),
             "norm_error": abs(g["norm"] - 1.0),
             "autotune": tune_info,
+            "specialize_s": g["specialize_s"],
         },
     }
     if not args.no_q32:

@@ -2,6 +2,8 @@
 #include <stdarg.h>
 #include <string.h>
 
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "tile.hpp"
@@ -360,8 +362,24 @@ static int ensure_pass_lds(int L, size_t shmem) {
   return CQS_OK;
 }
 
+// Dynamic LDS above 64 KB must be granted per kernel; specialised kernels are granted once each.
+static int ensure_kernel_lds(const void* fn, size_t shmem) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, size_t> granted;
+  if (shmem <= 64 * 1024) return CQS_OK;
+  CQS_REQUIRE(shmem <= 160 * 1024, "pass needs %zu bytes of LDS (> 160 KB)", shmem);
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& g = granted[fn];
+  if (shmem > g) {
+    CQS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    g = shmem;
+  }
+  return CQS_OK;
+}
+
 static int run_passes(void* amps, int32_t n_bits, int64_t basis, const cqs_pass* passes, int32_t n_passes,
-                      const cqs_op* ops, const void* mats, const int32_t* aux, void* stream) {
+                      const cqs_op* ops, const void* mats, const int32_t* aux, void* stream,
+                      const void* const* kernels = nullptr) {
   CQS_REQUIRE(amps != nullptr, "amps is NULL");
   CQS_REQUIRE(n_bits >= 0 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
   CQS_REQUIRE(n_passes >= 0 && (n_passes == 0 || passes != nullptr), "bad pass list");
@@ -382,9 +400,17 @@ static int run_passes(void* amps, int32_t n_bits, int64_t basis, const cqs_pass*
     const int mk = ps.max_k;
     const size_t stage = mk >= 3 ? (size_t(1) << (2 * mk)) : 0;
     const size_t shmem = ((size_t)tile_words(ps.L) + stage) * sizeof(float2);
-    if (int rc = ensure_pass_lds(ps.L, shmem)) return rc;
-    hipLaunchKernelGGL(kPassKernels[ps.L], dim3((unsigned)tiles), dim3(threads_for_L(ps.L)), shmem, s,
-                       (float2*)amps, P);
+    if (kernels != nullptr) {
+      CQS_REQUIRE(kernels[i] != nullptr, "no specialised kernel for pass %d", i);
+      if (int rc = ensure_kernel_lds(kernels[i], shmem)) return rc;
+      float2* psi = (float2*)amps;
+      void* args[] = {&psi, &P};
+      CQS_HIP(hipLaunchKernel(kernels[i], dim3((unsigned)tiles), dim3(threads_for_L(ps.L)), args, shmem, s));
+    } else {
+      if (int rc = ensure_pass_lds(ps.L, shmem)) return rc;
+      hipLaunchKernelGGL(kPassKernels[ps.L], dim3((unsigned)tiles), dim3(threads_for_L(ps.L)), shmem, s,
+                         (float2*)amps, P);
+    }
     CQS_HIP(hipGetLastError());
   }
   if (start >= 0) return cqs_init_basis(amps, n_bits, (uint64_t)start, stream);  // no pass ran
@@ -401,6 +427,13 @@ int cqs_run_passes_from_basis(void* amps, int32_t n_bits, uint64_t basis, const 
                               void* stream) {
   CQS_REQUIRE(basis < (1ull << 62), "basis state out of range");
   return run_passes(amps, n_bits, (int64_t)basis, passes, n_passes, ops, mats, aux, stream);
+}
+
+int cqs_run_passes_specialized(void* amps, int32_t n_bits, int64_t basis, const cqs_pass* passes, int32_t n_passes,
+                               const cqs_op* ops, const void* mats, const int32_t* aux,
+                               const void* const* kernels, void* stream) {
+  CQS_REQUIRE(kernels != nullptr || n_passes == 0, "kernels is NULL");
+  return run_passes(amps, n_bits, basis < 0 ? -1 : basis, passes, n_passes, ops, mats, aux, stream, kernels);
 }
 
 }  // extern "C"

@@ -6,6 +6,8 @@ synchronise.  There is no CPU fallback: without the built library or a GPU these
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -48,6 +50,15 @@ class DeviceProgram:
         self.mats = torch.from_numpy(packed.mats_arr.view(np.float32).copy()).to(device)
         self.aux = torch.from_numpy(packed.aux_arr.copy()).to(device)
         self.pass_arr = np.ascontiguousarray(packed.pass_arr)
+        self.specialized = None  # SpecializedPasses: run() launches them instead of the interpreter
+
+    def specialize(self):
+        """Compiles this program's passes to straight-line kernels (cirq_amd.specialize) and runs
+        them from now on."""
+        from .specialize import specialize
+
+        self.specialized = specialize(self.packed)
+        return self
 
     def set_mats(self, mats_c64: np.ndarray):
         src = torch.from_numpy(np.ascontiguousarray(mats_c64, dtype=np.complex64).view(np.float32))
@@ -59,6 +70,16 @@ class DeviceProgram:
     def run(self, state: torch.Tensor, basis: int | None = None):
         """Runs the passes on state in place; with basis given, on |basis> instead (the first
         pass builds it, so state's contents on entry do not matter)."""
+        if self.specialized is not None:
+            check(
+                lib().cqs_run_passes_specialized(
+                    ptr(state), self.n_bits, -1 if basis is None else int(basis), self.pass_arr.ctypes.data,
+                    int(self.packed.n_passes), ptr(self.ops), ptr(self.mats), ptr(self.aux),
+                    ctypes.cast(self.specialized.handles, ctypes.c_void_p), stream_handle(),
+                ),
+                "cqs_run_passes_specialized",
+            )
+            return
         if basis is not None:
             check(
                 lib().cqs_run_passes_from_basis(

@@ -249,13 +249,17 @@ class _SimulatorBase:
         mfma_blocks: bool = True,
         autotune=False,
         trajectory_batch: int | None = None,
+        specialize_passes: bool = False,
     ):
         """Cirq's arguments (dtype, noise, seed, split_untangled_states) plus engine settings:
         tile_bits / coalesce_bits / mfma_blocks fix the pass compiler's tile and fusion choices;
         autotune=True (or an autotune.Autotuner) instead times candidate settings on the GPU the
         first time each circuit structure is compiled and keeps the fastest; trajectory_batch caps
         how many repetitions of a circuit with mid-circuit measurements or state-vector noise run
-        together (None: as many as a quarter of free device memory holds; 1: one at a time)."""
+        together (None: as many as a quarter of free device memory holds; 1: one at a time);
+        specialize_passes=True compiles each circuit's passes to straight-line kernels the first time
+        it runs (cirq_amd.specialize: seconds to a minute of hipcc, cached on disk), which pays for
+        circuits simulated many times on large states."""
         if np.dtype(dtype).kind != "c":
             raise ValueError(f"dtype must be a complex type but was {dtype}")
         if np.dtype(dtype) != np.complex64:
@@ -274,6 +278,7 @@ class _SimulatorBase:
             autotune = default_autotuner()
         self._tuner = autotune or None
         self._traj_batch = trajectory_batch
+        self._specialize = bool(specialize_passes)
         self._cache: "collections.OrderedDict" = collections.OrderedDict()
         self._split_memo: dict = {}
         self._noisy_memo: dict = {}
@@ -327,6 +332,8 @@ class _SimulatorBase:
         engine.check_state_fits(nb, self._dev())
         L, c = self._tile_params(nb)
         comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse, self._blocks, self._tuner)
+        if self._specialize and comp.packed.n_passes:
+            comp.dev.specialize()
         try:
             ref = weakref.ref(circuit)
         except TypeError:

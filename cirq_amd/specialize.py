@@ -1,0 +1,242 @@
+"""Pass programs compiled to straight-line kernels.
+
+The pass kernel (`k_pass`, `csrc/tile.hpp`) interprets a pass's op list: a loop that dispatches on
+each op's kind and bit positions while the tile sits in registers.  The dispatch costs more than
+the branches: an op's results are formed while its operands are live, so they land in fresh
+registers, and every trip of the loop copies all of a thread's amplitudes back into the registers
+the loop keeps them in (64 moves per op at 16 amplitudes per thread, measured in the ISA of
+`k_pass<13>`).  Here each pass becomes its own kernel that calls the same device functions in the
+pass's order, with the kind, the register positions, the matrix offsets and the outer-bit tests
+as constants; the compiler then renames instead of copying and resolves every dispatch.  MFMA
+blocks and the other LDS-phase ops keep their runtime dispatch (their data is in LDS, so it
+costs no copies).
+
+`specialize(packed)` writes the kernels' HIP source, compiles it with hipcc for gfx950 into a
+shared object under the cache directory (reused when the same source was compiled before) and
+returns the host launch handles, which `cqs_run_passes_specialized` launches with the same
+checked pass arguments as the interpreter.  Compiling takes seconds to a minute (parallel
+hipcc, a few kernels per file), so it pays for circuits that run many times (sampling loops,
+parameter sweeps, benchmarks) on states large enough for the passes to be compute-bound.
+
+Cirq has no counterpart: its NumPy path has no kernels to compile.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures
+import ctypes
+import hashlib
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from . import _build, _lib
+
+CSRC = Path(__file__).resolve().parent / "csrc"
+CACHE_DIR = Path(os.environ.get("CIRQ_AMD_SPECIALIZE_CACHE", Path(__file__).resolve().parent / "_specialized"))
+KERNELS_PER_FILE = 4
+
+_OP_DENSE, _OP_DIAG, _OP_PAIR, _OP_SCALE, _OP_BLOCK, _OP_PERMUTE = (
+    _lib.OP_DENSE, _lib.OP_DIAG, _lib.OP_PAIRPHASE, _lib.OP_SCALE, _lib.OP_BLOCK, _lib.OP_PERMUTE)
+
+
+def _shape(L: int) -> tuple[int, int]:
+    """Threads and amplitudes per thread of a tile of 2^L amplitudes (tile.hpp TileShape)."""
+    nt = 256 if L <= 12 else (1 << L) // 16
+    per = 1 if L <= 8 else (1 << L) // nt
+    return nt, per
+
+
+def _is_register_dense(op, nt: int, per: int) -> bool:
+    """tile.hpp is_register_dense: k <= 3 targets, all register bits, ascending."""
+    tb, nr = nt.bit_length() - 1, per.bit_length() - 1
+    k = int(op["k"])
+    if (int(op["kind"]) & 0xFF) != _OP_DENSE or k < 1 or k > 3 or k > nr:
+        return False
+    t = [int(x) for x in op["t"][:k]]
+    return t[0] >= tb and all(t[m] > t[m - 1] for m in range(1, k))
+
+
+def _mixing(op, nt: int, per: int) -> bool:
+    kind = int(op["kind"]) & 0xFF
+    return (kind == _OP_DENSE and not _is_register_dense(op, nt, per)) or kind == _OP_BLOCK
+
+
+def _outer_guard(op) -> str:
+    """`if (...) ` testing the op's outer controls against the tile's outer bits, or ''."""
+    mask = int(op["co_mask_lo"]) | (int(op["co_mask_hi"]) << 32)
+    if not mask:
+        return ""
+    val = int(op["co_val_lo"]) | (int(op["co_val_hi"]) << 32)
+    return f"if ((base & 0x{mask:x}ull) == 0x{val:x}ull) "
+
+
+def _variant_expr(op) -> str:
+    """The op's matrix variant from the tile's outer bits (tile.hpp variant_of), as constants."""
+    code = int(op["t"][5])
+    cnt = (code >> 24) & 3
+    if not cnt:
+        return ""
+    parts = [f"((int)((base >> {(code >> (7 * i)) & 127}) & 1ull) << {i})" for i in range(cnt)]
+    return "(" + " | ".join(parts) + ")"
+
+
+def pass_source(index: int, ps, ops: np.ndarray) -> str:
+    """HIP source of one pass as `extern "C" __global__ void k_spec_<index>(float2*, TileProg)`,
+    the same steps as tile.hpp pass_tile / run_program for this pass's ops."""
+    L = int(ps["L"])
+    nt, per = _shape(L)
+    tb = nt.bit_length() - 1
+    first, count = int(ps["op_begin"]), int(ps["op_count"])
+    body = []
+    j = 0
+    while j < count:
+        op = ops[first + j]
+        kind = int(op["kind"]) & 0xFF
+        if not _mixing(op, nt, per):
+            ref = f"ops[{j}]"
+            guard = _outer_guard(op)
+            if kind == _OP_PERMUTE:
+                body.append(f"  __syncthreads();\n  regs_to_tile_permuted<PER, NT>(v, tile, L, scalar_ptr(P.aux) + {int(op['t'][0])}, act);\n"
+                            "  __syncthreads();\n  tile_to_regs<PER, NT>(v, tile, act);")
+            elif kind == _OP_DENSE:
+                k = int(op["k"])
+                r = [int(x) - tb for x in op["t"][:k]] + [0, 0]
+                var = _variant_expr(op)
+                mat = f"scalar_ptr(P.mats) + {int(op['mat'])}" + (f" + {var} * {1 << (2 * k)}" if var else "")
+                ctl = "true" if int(op["cl_mask"]) else "false"
+                body.append(f"  {guard}reg_dense_k<PER, NT, {k}, {r[0]}, {r[1]}, {r[2]}, {ctl}>(v, {mat}, "
+                            f"{int(op['cl_mask'])}, {int(op['cl_val'])});")
+            elif kind == _OP_DIAG:
+                body.append(f"  {guard}ew_diag<PER, NT>(v, P, {ref}, base);")
+            elif kind == _OP_PAIR:
+                body.append(f"  {guard}ew_pairs<PER, NT>(v, P, {ref}, base);")
+            elif kind == _OP_SCALE:
+                body.append(f"  {guard}ew_scale<PER>(v, P, {ref});")
+            else:
+                raise ValueError(f"pass {index}: op kind {kind} has no specialised form")
+            j += 1
+            continue
+        body.append("  regs_to_tile<PER, NT>(v, tile, act);")
+        while j < count and _mixing(ops[first + j], nt, per):
+            op = ops[first + j]
+            body.append(f"  __syncthreads();\n  {_outer_guard(op)}run_mixing_op<NT>(tile, mstage, P, ops[{j}], base);")
+            j += 1
+        body.append("  __syncthreads();\n  tile_to_regs<PER, NT>(v, tile, act);")
+    lines = "\n".join(body)
+    return f"""
+extern "C" __global__ __launch_bounds__({nt}) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spec_{index}(
+    float2* __restrict__ psi, TileProg P) {{
+  constexpr int L = {L}, NT = {nt}, PER = {per};
+  extern __shared__ float2 lds[];
+  uint32_t b = blockIdx.x;  // XCD-aware tile order, as k_pass
+  const uint32_t G = gridDim.x;
+  if ((G & 7) == 0) b = (b & 7) * (G >> 3) + (b >> 3);
+  const uint64_t base = deposit_bounded<31>((uint64_t)b, P.opos, P.nouter);
+  float2* tile = lds;
+  float2* mstage = lds + tile_words(L);
+  sop_t* ops = scalar_ptr(P.ops);
+  const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
+  (void)ops;
+  (void)mstage;
+  warm_program<NT>(P);
+  float2 v[PER];
+  {{
+    uint64_t off_lo, off_hi[PER];
+    tile_offsets<L>(P.lpos, off_lo, off_hi);
+    if (act) {{
+      if (P.basis >= 0) {{
+#pragma unroll
+        for (int e = 0; e < PER; ++e)
+          v[e] = make_float2(base + off_lo + off_hi[e] == (uint64_t)P.basis ? 1.f : 0.f, 0.f);
+      }} else {{
+#pragma unroll
+        for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
+      }}
+    }}
+  }}
+{lines}
+  {{
+    uint64_t off_lo, off_hi[PER];
+    tile_offsets<L>(P.lpos_out, off_lo, off_hi);
+    if (act) {{
+#pragma unroll
+      for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = v[e];
+    }}
+  }}
+}}
+"""
+
+
+def program_sources(packed) -> list[str]:
+    """The program's kernels, KERNELS_PER_FILE to a translation unit."""
+    head = f'#include "{CSRC / "tile.hpp"}"\nusing namespace cqs;\n'
+    ops = packed.ops_arr
+    kernels = [pass_source(i, ps, ops) for i, ps in enumerate(packed.pass_arr)]
+    return [head + "".join(kernels[i : i + KERNELS_PER_FILE]) for i in range(0, len(kernels), KERNELS_PER_FILE)]
+
+
+def _digest(sources: list[str]) -> str:
+    h = hashlib.sha1()
+    for part in [" ".join(_build.COMMON_FLAGS)] + [(CSRC / f).read_text() for f in ("tile.hpp", "common.hpp")] + sources:
+        h.update(part.encode())
+    return h.hexdigest()[:20]
+
+
+def _compile(sources: list[str], out: Path, jobs: int | None = None) -> None:
+    work = out.with_suffix(".d")
+    work.mkdir(parents=True, exist_ok=True)
+
+    def one(i_src):
+        i, src = i_src
+        path = work / f"part{i}.hip"
+        path.write_text(src)
+        obj = work / f"part{i}.o"
+        cmd = [_build.HIPCC, *_build.COMMON_FLAGS, "-c", str(path), "-o", str(obj)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            err = "\n".join(ln for ln in res.stderr.splitlines() if "packed-fp32-ops" not in ln)
+            raise RuntimeError(f"hipcc failed on a specialised pass file:\n{err[-4000:]}")
+        return obj
+
+    jobs = jobs or max(1, min(len(sources), (os.cpu_count() or 2), 16))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(one, enumerate(sources)))
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [_build.HIPCC, f"--offload-arch={_build.ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link of specialised passes failed:\n{res.stderr[-4000:]}")
+    os.replace(tmp, out)
+
+
+class SpecializedPasses:
+    """The launch handles of a program's compiled passes (kept alive with their shared object)."""
+
+    def __init__(self, path: Path, n_passes: int):
+        self.path = path
+        self.lib = ctypes.CDLL(str(path))
+        handles = [ctypes.cast(getattr(self.lib, f"k_spec_{i}"), ctypes.c_void_p).value for i in range(n_passes)]
+        self.handles = (ctypes.c_void_p * max(n_passes, 1))(*handles)
+        self.n_passes = n_passes
+
+
+_loaded: dict[str, SpecializedPasses] = {}
+
+
+def specialize(packed, cache_dir: Path | None = None) -> SpecializedPasses:
+    """Compiles (or finds in the cache) the straight-line kernels of a PackedProgram."""
+    sources = program_sources(packed)
+    key = _digest(sources)
+    if key in _loaded:
+        return _loaded[key]
+    cache = Path(cache_dir) if cache_dir is not None else CACHE_DIR
+    cache.mkdir(parents=True, exist_ok=True)
+    out = cache / f"passes_{key}.so"
+    if not out.exists():
+        _compile(sources, out)
+    sp = SpecializedPasses(out, int(packed.n_passes))
+    _loaded[key] = sp
+    return sp

@@ -169,6 +169,19 @@ int cqs_run_passes_from_basis(void* amps, int32_t n_bits, uint64_t basis, const 
                               int32_t n_passes, const cqs_op* ops, const void* mats, const int32_t* aux,
                               void* stream);
 
+/*
+ * cqs_run_passes (basis < 0) or cqs_run_passes_from_basis (basis >= 0) with pass i run by
+ * kernels[i], a kernel compiled for that pass's op sequence instead of the op interpreter:
+ * kernels[i] is the host launch handle of a `__global__ void (float2*, TileProg)` built by
+ * cirq_amd/specialize.py against this library's csrc/tile.hpp (the same device functions,
+ * called in the pass's order with its bit positions as constants).  The program arrays are
+ * checked as for cqs_run_passes.  Same reference as cqs_run_passes
+ * (cirq-core/cirq/sim/simulator_base.py:198-212 and the kernels it reaches).
+ */
+int cqs_run_passes_specialized(void* amps, int32_t n_bits, int64_t basis, const cqs_pass* passes, int32_t n_passes,
+                               const cqs_op* ops, const void* mats, const int32_t* aux,
+                               const void* const* kernels, void* stream);
+
 /* ---------------------------------------------------------- adjoint gradient */
 
 /*
