@@ -262,6 +262,16 @@ def sweep_leg(args):
     }
 
 
+def _specialize_gradient(args, ag):
+    """Compiles the gradient's forward and reverse programs to straight-line kernels (timed; both
+    the adjoint sweep and the parameter-shift comparison then run them), unless --no-specialize."""
+    if args.no_specialize:
+        return None
+    t0 = time.perf_counter()
+    ag.specialize()
+    return time.perf_counter() - t0
+
+
 def _adjoint_roofline(ag, n_bits: int, point) -> dict:
     """Roofline of the reverse-sweep kernel (k_adjoint_pass, the dominant one of a gradient): its
     algorithmic bytes per pass (psi and phi each read and written once: 4 * 8 * 2^n) over its
@@ -301,6 +311,7 @@ def adjoint_leg(args):
     circuit, syms, vals = generators.hardware_efficient_ansatz(args.adjoint_qubits, args.adjoint_layers, seed=0)
     psum = generators.random_pauli_sum(sorted(circuit.all_qubits()), 32, 4, seed=0)
     ag = AdjointGradient(circuit, psum)
+    t_spec = _specialize_gradient(args, ag)
     ag.value_and_grad(vals)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -328,6 +339,7 @@ def adjoint_leg(args):
         "forward_passes": int(ag.prog.fwd.n_passes),
         "reverse_passes": int(ag.prog.rev.n_passes),
         "roofline": _adjoint_roofline(ag, args.adjoint_qubits, vals),
+        "specialize_s": t_spec,
     }
     if not args.no_cpu_baseline:
         # Cirq has no adjoint: its gradient is the shift rule, 2 simulations per symbol
@@ -378,6 +390,7 @@ def qaoa_leg(args):
 
     circuit, syms, vals, ham, edges = generators.qaoa_maxcut(args.qaoa_nodes, args.qaoa_p, seed=0)
     ag = AdjointGradient(circuit, ham)
+    t_spec = _specialize_gradient(args, ag)
     point = dict(zip(syms, vals))
     ag.value_and_grad(point)  # warm-up
     torch.cuda.synchronize()
@@ -406,6 +419,7 @@ def qaoa_leg(args):
         "forward_passes": int(ag.prog.fwd.n_passes),
         "reverse_passes": int(ag.prog.rev.n_passes),
         "roofline": _adjoint_roofline(ag, args.qaoa_nodes, point),
+        "specialize_s": t_spec,
     }
     if not args.no_cpu_baseline:
         per_layer = (len(circuit) - 1) // args.qaoa_p  # moments of one (ZZ layer, Rx layer) round

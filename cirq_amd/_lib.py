@@ -116,6 +116,8 @@ _SIGNATURES = {
     "cqs_norm2": ([_vp, _i32, _vp, _vp, _vp], ctypes.c_int),
     "cqs_inner": ([_vp, _vp, _i32, _vp, _vp, _vp], ctypes.c_int),
     "cqs_run_passes_specialized": ([_vp, _i32, ctypes.c_int64, _vp, _i32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cqs_adjoint_passes_specialized": ([_vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
+                                       ctypes.c_int),
     "cqs_scale": ([_vp, _i32, _f32, _f32, _vp], ctypes.c_int),
     "cqs_collapse": ([_vp, _i32, _u64, _u64, _f32, _f32, _vp], ctypes.c_int),
     "cqs_sample_workspace_bytes": ([_i32], _sz),

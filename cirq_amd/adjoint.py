@@ -12,6 +12,8 @@ ExpectationLayer wraps it in a torch.autograd.Function.
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import sympy
 import torch
@@ -257,6 +259,16 @@ class AdjointGradient:
         self.partial = torch.zeros(max(G, 1) * self.prog.partial_stride, dtype=torch.float64, device=self.device)
         self.gslot = torch.zeros(max(G, 1), dtype=torch.float64, device=self.device)
         self.ev_buf = torch.zeros(4, dtype=torch.float64, device=self.device)
+        self.rev_spec = None  # specialised reverse-sweep kernels (specialize())
+
+    def specialize(self):
+        """Compiles the forward and reverse programs to straight-line kernels (cirq_amd.specialize)
+        and runs them from now on."""
+        from .specialize import specialize
+
+        self.dev_fwd.specialize()
+        self.rev_spec = specialize(self.prog.rev, adjoint=True)
+        return self
 
     def _forward(self):
         check(lib().cqs_init_basis(ptr(self.psi), self.n, 0, engine.stream_handle()), "cqs_init_basis")
@@ -313,14 +325,14 @@ class AdjointGradient:
             ev_t[2].record()
         G = prog.n_slots
         rv = prog.rev
-        check(
-            lib().cqs_adjoint_passes(
-                ptr(self.psi), ptr(self.phi), n, rv.pass_arr.ctypes.data, int(rv.n_passes), ptr(self.dev_rev.ops),
-                ptr(self.dev_rev.mats), ptr(self.dev_rev.aux), ptr(self.partial), prog.partial_stride,
-                engine.stream_handle(),
-            ),
-            "cqs_adjoint_passes",
-        )
+        args = (ptr(self.psi), ptr(self.phi), n, rv.pass_arr.ctypes.data, int(rv.n_passes), ptr(self.dev_rev.ops),
+                ptr(self.dev_rev.mats), ptr(self.dev_rev.aux), ptr(self.partial), prog.partial_stride)
+        if self.rev_spec is not None:
+            check(lib().cqs_adjoint_passes_specialized(
+                *args, ctypes.cast(self.rev_spec.handles, ctypes.c_void_p), engine.stream_handle()),
+                "cqs_adjoint_passes_specialized")
+        else:
+            check(lib().cqs_adjoint_passes(*args, engine.stream_handle()), "cqs_adjoint_passes")
         if ev_t:
             ev_t[3].record()
         check(lib().cqs_grad_reduce(ptr(self.partial), G, prog.partial_stride, ptr(self.gslot), engine.stream_handle()),

@@ -37,9 +37,9 @@ using namespace cqs;
 
 extern "C" {
 
-int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
-                       const cqs_op* ops, const void* mats, const int32_t* aux, double* grad_partial,
-                       int64_t partial_stride, void* stream) {
+static int adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
+                          const cqs_op* ops, const void* mats, const int32_t* aux, double* grad_partial,
+                          int64_t partial_stride, const void* const* kernels, void* stream) {
   CQS_REQUIRE(psi != nullptr && phi != nullptr && psi != phi, "psi/phi must be distinct non-NULL");
   CQS_REQUIRE(n_bits >= 0 && n_bits <= CQS_MAX_BITS, "n_bits=%d out of range", n_bits);
   CQS_REQUIRE(n_passes >= 0 && (n_passes == 0 || passes != nullptr), "bad pass list");
@@ -64,11 +64,36 @@ int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* pas
       CQS_HIP(hipFuncSetAttribute((const void*)kAdjKernels[ps.L], hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shmem));
     const int nt = ps.L <= 12 ? 256 : (1 << ps.L) / 16;
-    hipLaunchKernelGGL(kAdjKernels[ps.L], dim3((unsigned)tiles), dim3(nt), shmem, s, (float2*)psi, (float2*)phi, P,
-                       grad_partial, partial_stride);
+    if (kernels != nullptr) {
+      CQS_REQUIRE(kernels[i] != nullptr, "no specialised kernel for reverse pass %d", i);
+      if (shmem > 64 * 1024)
+        CQS_HIP(hipFuncSetAttribute(kernels[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+      float2* a = (float2*)psi;
+      float2* b = (float2*)phi;
+      void* args[] = {&a, &b, &P, &grad_partial, &partial_stride};
+      CQS_HIP(hipLaunchKernel(kernels[i], dim3((unsigned)tiles), dim3(nt), args, shmem, s));
+    } else {
+      hipLaunchKernelGGL(kAdjKernels[ps.L], dim3((unsigned)tiles), dim3(nt), shmem, s, (float2*)psi, (float2*)phi,
+                         P, grad_partial, partial_stride);
+    }
     CQS_HIP(hipGetLastError());
   }
   return CQS_OK;
+}
+
+int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
+                       const cqs_op* ops, const void* mats, const int32_t* aux, double* grad_partial,
+                       int64_t partial_stride, void* stream) {
+  return adjoint_passes(psi, phi, n_bits, passes, n_passes, ops, mats, aux, grad_partial, partial_stride, nullptr,
+                        stream);
+}
+
+int cqs_adjoint_passes_specialized(void* psi, void* phi, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
+                                   const cqs_op* ops, const void* mats, const int32_t* aux, double* grad_partial,
+                                   int64_t partial_stride, const void* const* kernels, void* stream) {
+  CQS_REQUIRE(kernels != nullptr || n_passes == 0, "kernels is NULL");
+  return adjoint_passes(psi, phi, n_bits, passes, n_passes, ops, mats, aux, grad_partial, partial_stride, kernels,
+                        stream);
 }
 
 int cqs_grad_reduce(const double* grad_partial, int32_t n_slots, int64_t partial_stride, double* out, void* stream) {

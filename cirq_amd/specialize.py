@@ -7,9 +7,8 @@ registers, and every trip of the loop copies all of a thread's amplitudes back i
 the loop keeps them in (64 moves per op at 16 amplitudes per thread, measured in the ISA of
 `k_pass<13>`).  Here each pass becomes its own kernel that calls the same device functions in the
 pass's order, with the kind, the register positions, the matrix offsets and the outer-bit tests
-as constants; the compiler then renames instead of copying and resolves every dispatch.  MFMA
-blocks and the other LDS-phase ops keep their runtime dispatch (their data is in LDS, so it
-costs no copies).
+as constants; the compiler then renames instead of copying and resolves every dispatch, down to
+the MFMA block width and whether its column blocks are whole.
 
 `specialize(packed)` writes the kernels' HIP source, compiles it with hipcc for gfx950 into a
 shared object under the cache directory (reused when the same source was compiled before) and
@@ -83,6 +82,24 @@ def _variant_expr(op) -> str:
     return "(" + " | ".join(parts) + ")"
 
 
+def _mixing_call(op, j: int, L: int, tile: str) -> str:
+    """tile.hpp run_mixing_op for one op with its kind, width and matrix offset resolved: an MFMA
+    block or a VALU dense gate on the LDS tile `tile`."""
+    kind, k, mat = int(op["kind"]) & 0xFF, int(op["k"]), int(op["mat"])
+    var = _variant_expr(op)
+    if kind == _OP_BLOCK:
+        a = f"P.mats + {mat}" + (f" + {var} * {2 << (2 * k)}" if var else "")
+        full = "true" if L - k >= 5 else "false"
+        return (f"tile_block_mfma<{k}, NT, {full}>({tile}, L, ops[{j}], reinterpret_cast<const half8_t*>({a}), "
+                "P.mscale, 1.f / P.mscale);")
+    m = f"scalar_ptr(P.mats) + {mat}" + (f" + {var} * {1 << (2 * k)}" if var else "")
+    if k in (1, 2):
+        return f"tile_dense<{k}, NT>({tile}, L, ops[{j}], {m});"
+    if k == 3:
+        return f"stage_matrix<NT>(mstage, {m}, 64); __syncthreads(); tile_dense<3, NT>({tile}, L, ops[{j}], mstage);"
+    raise ValueError(f"mixing op of kind {kind} on {k} bits has no specialised form")
+
+
 def pass_source(index: int, ps, ops: np.ndarray) -> str:
     """HIP source of one pass as `extern "C" __global__ void k_spec_<index>(float2*, TileProg)`,
     the same steps as tile.hpp pass_tile / run_program for this pass's ops."""
@@ -122,7 +139,9 @@ def pass_source(index: int, ps, ops: np.ndarray) -> str:
         body.append("  regs_to_tile<PER, NT>(v, tile, act);")
         while j < count and _mixing(ops[first + j], nt, per):
             op = ops[first + j]
-            body.append(f"  __syncthreads();\n  {_outer_guard(op)}run_mixing_op<NT>(tile, mstage, P, ops[{j}], base);")
+            guard = _outer_guard(op)
+            call = _mixing_call(op, j, L, "tile")
+            body.append(f"  __syncthreads();\n  {guard}{{ {call} }}" if guard else f"  __syncthreads();\n  {call}")
             j += 1
         body.append("  __syncthreads();\n  tile_to_regs<PER, NT>(v, tile, act);")
     lines = "\n".join(body)
@@ -170,17 +189,167 @@ extern "C" __global__ __launch_bounds__({nt}) __attribute__((amdgpu_waves_per_eu
 """
 
 
-def program_sources(packed) -> list[str]:
-    """The program's kernels, KERNELS_PER_FILE to a translation unit."""
-    head = f'#include "{CSRC / "tile.hpp"}"\nusing namespace cqs;\n'
+def _adj_register_dense(op, nt: int, per: int) -> bool:
+    """adjoint.hpp adj_register_dense: DENSE or GRAD_DENSE, k <= 3 register bits, ascending."""
+    tb, nr = nt.bit_length() - 1, per.bit_length() - 1
+    kind, k = int(op["kind"]) & 0xFF, int(op["k"])
+    if kind not in (_OP_DENSE, _lib.OP_GRAD_DENSE) or k < 1 or k > 3 or k > nr:
+        return False
+    t = [int(x) for x in op["t"][:k]]
+    return t[0] >= tb and all(t[m] > t[m - 1] for m in range(1, k))
+
+
+def _adj_mixing(op, nt: int, per: int) -> bool:
+    kind = int(op["kind"]) & 0xFF
+    if kind == _OP_BLOCK:
+        return True
+    return kind in (_OP_DENSE, _lib.OP_GRAD_DENSE) and not _adj_register_dense(op, nt, per)
+
+
+def _guard_cond(op) -> str:
+    """The op's outer-control test as a C condition ('true' without outer controls)."""
+    mask = int(op["co_mask_lo"]) | (int(op["co_mask_hi"]) << 32)
+    if not mask:
+        return "true"
+    val = int(op["co_val_lo"]) | (int(op["co_val_hi"]) << 32)
+    return f"(base & 0x{mask:x}ull) == 0x{val:x}ull"
+
+
+def adjoint_pass_source(index: int, ps, ops: np.ndarray) -> str:
+    """HIP source of one reverse-sweep pass as `extern "C" __global__ void k_aspec_<index>(float2*
+    psi, float2* phi, TileProg, double* partial, int64_t stride)`, the same steps as adjoint.hpp
+    adjoint_tile for this pass's ops."""
+    L = int(ps["L"])
+    nt, per = _shape(L)
+    tb = nt.bit_length() - 1
+    first, count = int(ps["op_begin"]), int(ps["op_count"])
+    body = []
+    j = 0
+    sel_states = {0: ("va", "vb"), 1: ("va",), 2: ("vb",)}
+    while j < count:
+        op = ops[first + j]
+        kind, sel = int(op["kind"]) & 0xFF, (int(op["kind"]) >> 8) & 3
+        ref = f"ops[{j}]"
+        if not _adj_mixing(op, nt, per):
+            cond = _guard_cond(op)
+            if kind in (_lib.OP_GRAD_DIAG, _lib.OP_GRAD_DENSE):
+                if kind == _lib.OP_GRAD_DIAG:
+                    call = f"grad_reg_diag<PER, NT>(va, vb, P, {ref}, base)"
+                else:
+                    k = int(op["k"])
+                    r = [int(x) - tb for x in op["t"][:k]] + [0, 0]
+                    call = f"grad_reg_dense_k<PER, {k}, {r[0]}, {r[1]}, {r[2]}>(va, vb, scalar_ptr(P.mats) + {int(op['mat'])})"
+                body.append(f"  {{\n    double g = 0.0;\n    if (({cond}) && act) g = {call};\n"
+                            f"    write_wave_partial<NT>(g, partial, stride, tile, {int(op['slot'])});\n  }}")
+                j += 1
+                continue
+            guard = "" if cond == "true" else f"if ({cond}) "
+            states = sel_states[sel]
+            for st in states:
+                if kind == _OP_DENSE:
+                    k = int(op["k"])
+                    r = [int(x) - tb for x in op["t"][:k]] + [0, 0]
+                    var = _variant_expr(op)
+                    mat = f"scalar_ptr(P.mats) + {int(op['mat'])}" + (f" + {var} * {1 << (2 * k)}" if var else "")
+                    ctl = "true" if int(op["cl_mask"]) else "false"
+                    body.append(f"  {guard}reg_dense_k<PER, NT, {k}, {r[0]}, {r[1]}, {r[2]}, {ctl}>({st}, {mat}, "
+                                f"{int(op['cl_mask'])}, {int(op['cl_val'])});")
+                elif kind == _OP_DIAG:
+                    body.append(f"  {guard}ew_diag<PER, NT>({st}, P, {ref}, base);")
+                elif kind == _OP_PAIR:
+                    body.append(f"  {guard}ew_pairs<PER, NT>({st}, P, {ref}, base);")
+                elif kind == _OP_SCALE:
+                    body.append(f"  {guard}ew_scale<PER>({st}, P, {ref});")
+                else:
+                    raise ValueError(f"reverse pass {index}: op kind {kind} has no specialised form")
+            j += 1
+            continue
+        body.append("  regs_to_tile<PER, NT>(va, tA, act);\n  regs_to_tile<PER, NT>(vb, tB, act);")
+        while j < count and _adj_mixing(ops[first + j], nt, per):
+            op = ops[first + j]
+            kind, sel = int(op["kind"]) & 0xFF, (int(op["kind"]) >> 8) & 3
+            ref = f"ops[{j}]"
+            cond = _guard_cond(op)
+            body.append("  __syncthreads();")
+            if kind == _lib.OP_GRAD_DENSE:
+                k = int(op["k"])
+                body.append(f"  {{\n    double g = 0.0;\n    if ({cond}) g = grad_dense<{k}, NT>(tA, tB, L, {ref}, "
+                            f"scalar_ptr(P.mats) + {int(op['mat'])});\n"
+                            f"    write_wave_partial<NT>(g, partial, stride, tile, {int(op['slot'])});\n  }}")
+            else:
+                inner = []
+                if sel != 2:
+                    inner.append(_mixing_call(op, j, L, "tA"))
+                if sel == 0:
+                    inner.append("__syncthreads();")
+                if sel != 1:
+                    inner.append(_mixing_call(op, j, L, "tB"))
+                text = " ".join(inner)
+                body.append(f"  {text}" if cond == "true" else f"  if ({cond}) {{ {text} }}")
+            j += 1
+        body.append("  __syncthreads();\n  tile_to_regs<PER, NT>(va, tA, act);\n  tile_to_regs<PER, NT>(vb, tB, act);")
+    lines = "\n".join(body)
+    return f"""
+extern "C" __global__ __launch_bounds__({nt}) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_aspec_{index}(
+    float2* __restrict__ psi, float2* __restrict__ phi, TileProg P, double* __restrict__ partial, int64_t stride) {{
+  constexpr int L = {L}, NT = {nt}, PER = {per};
+  extern __shared__ float2 lds[];
+  const int64_t tile = blockIdx.x;
+  const uint64_t base = deposit_bounded<31>((uint64_t)blockIdx.x, P.opos, P.nouter);
+  float2* tA = lds;
+  float2* tB = lds + tile_words(L);
+  float2* mstage = lds + 2 * tile_words(L);
+  sop_t* ops = scalar_ptr(P.ops);
+  const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
+  (void)ops;
+  (void)mstage;
+  warm_program<NT>(P);
+  float2 va[PER], vb[PER];
+  {{
+    uint64_t off_lo, off_hi[PER];
+    tile_offsets<L>(P.lpos, off_lo, off_hi);
+    if (act) {{
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {{
+        va[e] = psi[base + off_lo + off_hi[e]];
+        vb[e] = phi[base + off_lo + off_hi[e]];
+      }}
+    }} else {{
+#pragma unroll
+      for (int e = 0; e < PER; ++e) va[e] = vb[e] = make_float2(0.f, 0.f);
+    }}
+  }}
+{lines}
+  {{
+    uint64_t off_lo, off_hi[PER];
+    tile_offsets<L>(P.lpos, off_lo, off_hi);
+    if (act) {{
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {{
+        psi[base + off_lo + off_hi[e]] = va[e];
+        phi[base + off_lo + off_hi[e]] = vb[e];
+      }}
+    }}
+  }}
+}}
+"""
+
+
+def program_sources(packed, adjoint: bool = False) -> list[str]:
+    """The program's kernels (reverse-sweep kernels for an adjoint program), KERNELS_PER_FILE to a
+    translation unit."""
+    head = f'#include "{CSRC / ("adjoint.hpp" if adjoint else "tile.hpp")}"\nusing namespace cqs;\n'
     ops = packed.ops_arr
-    kernels = [pass_source(i, ps, ops) for i, ps in enumerate(packed.pass_arr)]
-    return [head + "".join(kernels[i : i + KERNELS_PER_FILE]) for i in range(0, len(kernels), KERNELS_PER_FILE)]
+    gen = adjoint_pass_source if adjoint else pass_source
+    kernels = [gen(i, ps, ops) for i, ps in enumerate(packed.pass_arr)]
+    per = 1 if adjoint else KERNELS_PER_FILE  # reverse-sweep kernels are several times larger
+    return [head + "".join(kernels[i : i + per]) for i in range(0, len(kernels), per)]
 
 
 def _digest(sources: list[str]) -> str:
     h = hashlib.sha1()
-    for part in [" ".join(_build.COMMON_FLAGS)] + [(CSRC / f).read_text() for f in ("tile.hpp", "common.hpp")] + sources:
+    headers = ("tile.hpp", "common.hpp", "adjoint.hpp")
+    for part in [" ".join(_build.COMMON_FLAGS)] + [(CSRC / f).read_text() for f in headers] + sources:
         h.update(part.encode())
     return h.hexdigest()[:20]
 
@@ -215,10 +384,10 @@ def _compile(sources: list[str], out: Path, jobs: int | None = None) -> None:
 class SpecializedPasses:
     """The launch handles of a program's compiled passes (kept alive with their shared object)."""
 
-    def __init__(self, path: Path, n_passes: int):
+    def __init__(self, path: Path, n_passes: int, prefix: str = "k_spec_"):
         self.path = path
         self.lib = ctypes.CDLL(str(path))
-        handles = [ctypes.cast(getattr(self.lib, f"k_spec_{i}"), ctypes.c_void_p).value for i in range(n_passes)]
+        handles = [ctypes.cast(getattr(self.lib, f"{prefix}{i}"), ctypes.c_void_p).value for i in range(n_passes)]
         self.handles = (ctypes.c_void_p * max(n_passes, 1))(*handles)
         self.n_passes = n_passes
 
@@ -226,9 +395,11 @@ class SpecializedPasses:
 _loaded: dict[str, SpecializedPasses] = {}
 
 
-def specialize(packed, cache_dir: Path | None = None) -> SpecializedPasses:
-    """Compiles (or finds in the cache) the straight-line kernels of a PackedProgram."""
-    sources = program_sources(packed)
+def specialize(packed, cache_dir: Path | None = None, adjoint: bool = False) -> SpecializedPasses:
+    """Compiles (or finds in the cache) the straight-line kernels of a PackedProgram: forward
+    passes for cqs_run_passes_specialized, or (adjoint=True) reverse-sweep passes for
+    cqs_adjoint_passes_specialized."""
+    sources = program_sources(packed, adjoint)
     key = _digest(sources)
     if key in _loaded:
         return _loaded[key]
@@ -237,6 +408,6 @@ def specialize(packed, cache_dir: Path | None = None) -> SpecializedPasses:
     out = cache / f"passes_{key}.so"
     if not out.exists():
         _compile(sources, out)
-    sp = SpecializedPasses(out, int(packed.n_passes))
+    sp = SpecializedPasses(out, int(packed.n_passes), "k_aspec_" if adjoint else "k_spec_")
     _loaded[key] = sp
     return sp

@@ -197,6 +197,17 @@ int cqs_adjoint_passes(void* psi, void* phi, int32_t n_bits, const cqs_pass* pas
                        int64_t partial_stride, void* stream);
 
 /* out[g] = sum over t < partial_stride of grad_partial[g * partial_stride + t], g < n_slots. */
+/*
+ * cqs_adjoint_passes with reverse pass i run by kernels[i], a kernel compiled for that pass's op
+ * sequence by cirq_amd/specialize.py (adjoint=True) against this library's csrc/adjoint.hpp:
+ * the host launch handle of a `__global__ void (float2* psi, float2* phi, TileProg, double*
+ * partial, int64_t stride)`.  Arguments are checked as for cqs_adjoint_passes.  Same reference
+ * (the reverse sweep of Jones & Gacon, arXiv:2009.02823; no Cirq counterpart).
+ */
+int cqs_adjoint_passes_specialized(void* psi, void* phi, int32_t n_bits, const cqs_pass* passes, int32_t n_passes,
+                                   const cqs_op* ops, const void* mats, const int32_t* aux, double* grad_partial,
+                                   int64_t partial_stride, const void* const* kernels, void* stream);
+
 int cqs_grad_reduce(const double* grad_partial, int32_t n_slots, int64_t partial_stride, double* out, void* stream);
 
 /* ---------------------------------------------------------- batched sweeps */

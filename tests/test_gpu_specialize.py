@@ -99,3 +99,26 @@ def test_simulator_option_and_density_matrix(gpu):
     rho = dsim.simulate(noisy).final_density_matrix
     want = O.simulate_density_matrix(to_oracle_ops(noisy, q), 5)
     assert np.max(np.abs(rho - want)) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["hea", "qaoa"])
+def test_specialised_gradients_match_interpreter(gpu, kind):
+    """AdjointGradient.specialize(): the forward and reverse programs as straight-line kernels give
+    the interpreter's value and gradient (register GRAD ops, LDS GRAD ops, diagonals, MFMA blocks
+    of the fixed gates, outer-bit variants)."""
+    from cirq_amd.adjoint import AdjointGradient
+
+    _log(f"specialised gradients {kind}")
+    if kind == "hea":
+        circuit, syms, vals = generators.hardware_efficient_ansatz(14, 4, seed=3)
+        obs = generators.random_pauli_sum(sorted(circuit.all_qubits()), 8, 3, seed=3)
+        kw = dict(tile_bits=10, adjoint_tile_bits=9)
+    else:
+        circuit, syms, vals, obs, edges = generators.qaoa_maxcut(12, 2, seed=1)
+        kw = {}
+    ag = AdjointGradient(circuit, obs, **kw)
+    v0, g0 = ag.value_and_grad(vals)
+    ag.specialize()
+    v1, g1 = ag.value_and_grad(vals)
+    assert abs(v1 - v0) <= 1e-6 * max(1.0, abs(v0))
+    np.testing.assert_allclose(g1, g0, atol=1e-6 * max(1.0, np.abs(g0).max()))

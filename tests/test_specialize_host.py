@@ -21,6 +21,7 @@ def test_sources_follow_the_interpreter_dispatch():
     srcs = specialize.program_sources(packed)
     text = "".join(srcs)
     assert len(srcs) == -(-int(packed.n_passes) // specialize.KERNELS_PER_FILE)
+    assert len(specialize.program_sources(packed, adjoint=False)) == len(srcs)
     for i in range(int(packed.n_passes)):
         assert f"void k_spec_{i}(" in text
     kinds = packed.ops_arr["kind"] & 0xFF
@@ -28,7 +29,7 @@ def test_sources_follow_the_interpreter_dispatch():
     reg = sum(specialize._is_register_dense(op, nt, per) for op in packed.ops_arr[: packed.n_ops])
     mixing = sum(specialize._mixing(op, nt, per) for op in packed.ops_arr[: packed.n_ops])
     assert text.count("reg_dense_k<") == reg
-    assert text.count("run_mixing_op<") == mixing
+    assert text.count("tile_block_mfma<") + text.count("tile_dense<") == mixing
     assert text.count("ew_diag<") == int((kinds[: packed.n_ops] == _lib.OP_DIAG).sum())
     assert text.count("ew_pairs<") == int((kinds[: packed.n_ops] == _lib.OP_PAIRPHASE).sum())
     assert text.count("regs_to_tile_permuted<") == int((kinds[: packed.n_ops] == _lib.OP_PERMUTE).sum())
@@ -65,3 +66,17 @@ def test_small_program_compiles_to_one_handle_per_pass(tmp_path):
     assert all(h for h in list(sp.handles)[: sp.n_passes])
     again = specialize.specialize(packed, cache_dir=tmp_path)
     assert again is sp  # the same source is compiled once per process
+
+
+def test_reverse_sweep_sources_write_one_partial_per_gradient_slot():
+    from cirq_amd.adjoint import AdjointProgram
+
+    circuit, syms, vals, ham, edges = generators.qaoa_maxcut(10, 2, seed=0)
+    prog = AdjointProgram(circuit, ham, tile_bits=8, adjoint_tile_bits=8)
+    text = "".join(specialize.program_sources(prog.rev, adjoint=True))
+    for i in range(int(prog.rev.n_passes)):
+        assert f"void k_aspec_{i}(" in text
+    assert text.count("write_wave_partial<") == prog.n_slots
+    kinds = prog.rev.ops_arr["kind"][: prog.rev.n_ops] & 0xFF
+    grads = int(((kinds == _lib.OP_GRAD_DIAG) | (kinds == _lib.OP_GRAD_DENSE)).sum())
+    assert grads == prog.n_slots
