@@ -18,8 +18,8 @@ __device__ __forceinline__ int op_sel(sop_t& op) { return (op.kind >> 8) & 3; }
 
 // This thread's share (its groups) of 2 Re sum conj(phi) dU psi over the LDS tiles for a dense dU
 // (k <= 3) on tile bits op.t.
-template <int K, int NT>
-__device__ __forceinline__ double grad_dense(const float2* psi, const float2* phi, int L, sop_t& op, smat_t* M) {
+template <int K, int NT, class Op>
+__device__ __forceinline__ double grad_dense(const float2* psi, const float2* phi, int L, Op& op, smat_t* M) {
   constexpr int D = 1 << K;
   int ts[K];
 #pragma unroll
@@ -141,9 +141,9 @@ __device__ __forceinline__ double grad_reg_dense(const float2 (&x)[PER], const f
 }
 
 // 2 Re sum conj(f_j) d(j) x_j over this thread's register elements for a diagonal derivative d.
-template <int PER, int NT, class Prog>
+template <int PER, int NT, class Prog, class Op>
 __device__ __forceinline__ double grad_reg_diag(const float2 (&x)[PER], const float2 (&f)[PER], const Prog& P,
-                                                sop_t& op, uint64_t base) {
+                                                Op& op, uint64_t base) {
   const DiagTable<PER, NT, Prog> D(P, op, base);
   double acc = 0.0;
   if (D.small) {

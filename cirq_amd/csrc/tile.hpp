@@ -99,8 +99,8 @@ __device__ __forceinline__ bool outer_controls_ok(sop_t& op, uint64_t base) {
 // ---------------------------------------------------------------- dense ops on the LDS tile
 // 2^K x 2^K matrix on tile bits op.t[0..K); M row-major, bit m of the matrix index is tile bit
 // op.t[m].  Each thread owns whole groups, so no barrier is needed inside.
-template <int K, int NT, class MPtr>
-__device__ __forceinline__ void tile_dense(float2* tile, int L, sop_t& op, MPtr M) {
+template <int K, int NT, class Op, class MPtr>
+__device__ __forceinline__ void tile_dense(float2* tile, int L, Op& op, MPtr M) {
   constexpr int D = 1 << K;
   int ts[K];
 #pragma unroll
@@ -184,8 +184,8 @@ __device__ __forceinline__ void split_f16(float xr, float xi, float s, half2_t& 
 // index is a per-lane column part + a per-lane part from the lane's half (h) + a part that depends
 // only on compile-time row bits and the op's targets (scalar).
 // kFull: every 32-column block is whole (L - K >= 5), so no column needs a bounds test.
-template <int K, int NT, bool kFull>
-__device__ __forceinline__ void tile_block_mfma(float2* tile, int L, sop_t& op, const half8_t* A,
+template <int K, int NT, bool kFull, class Op>
+__device__ __forceinline__ void tile_block_mfma(float2* tile, int L, Op& op, const half8_t* A,
                                                 float scale, float inv_scale) {
   constexpr int D = 1 << K;
   constexpr int R = 2 * D;
@@ -349,7 +349,8 @@ struct DiagTable {
   float2 d0, d1, d2, d3;
   const float2* Dg;
 
-  __device__ __forceinline__ DiagTable(const Prog& P, sop_t& op, uint64_t base) {
+  template <class Op>
+  __device__ __forceinline__ DiagTable(const Prog& P, Op& op, uint64_t base) {
     const int tid = threadIdx.x;
 #pragma unroll
     for (int r = 0; r < (NR > 0 ? NR : 1); ++r) w[r] = 0;
@@ -398,8 +399,8 @@ struct DiagTable {
   __device__ __forceinline__ float2 at(int e) const { return small ? pick(index(e)) : Dg[index(e)]; }
 };
 
-template <int PER, int NT, class Prog>
-__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, sop_t& op, uint64_t base) {
+template <int PER, int NT, class Prog, class Op>
+__device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, Op& op, uint64_t base) {
   const DiagTable<PER, NT, Prog> D(P, op, base);
   if (!D.any_reg) {
     const float2 d = D.small ? D.pick(D.fixed) : D.Dg[D.fixed];
@@ -418,8 +419,8 @@ __device__ __forceinline__ void ew_diag(float2 (&v)[PER], const Prog& P, sop_t& 
 
 // Products of phases on bit pairs (a CZ layer).  Pairs on thread / outer bits only multiply one
 // per-lane phase; pairs touching register bits multiply the elements whose e has those bits.
-template <int PER, int NT, class Prog>
-__device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const Prog& P, sop_t& op, uint64_t base) {
+template <int PER, int NT, class Prog, class Op>
+__device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const Prog& P, Op& op, uint64_t base) {
   constexpr int TB = __builtin_ctz(NT);
   saux_t* pr = scalar_ptr(P.aux) + op.t[0];
   smat_t* ph = scalar_ptr(P.mats) + op.mat;
@@ -462,8 +463,8 @@ __device__ __forceinline__ void ew_pairs(float2 (&v)[PER], const Prog& P, sop_t&
   for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], cph);
 }
 
-template <int PER, class Prog>
-__device__ __forceinline__ void ew_scale(float2 (&v)[PER], const Prog& P, sop_t& op) {
+template <int PER, class Prog, class Op>
+__device__ __forceinline__ void ew_scale(float2 (&v)[PER], const Prog& P, Op& op) {
   const float2 s = ldm(scalar_ptr(P.mats), op.mat);
 #pragma unroll
   for (int e = 0; e < PER; ++e) v[e] = cmul(v[e], s);

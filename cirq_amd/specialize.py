@@ -82,7 +82,16 @@ def _variant_expr(op) -> str:
     return "(" + " | ".join(parts) + ")"
 
 
-def _mixing_call(op, j: int, L: int, tile: str) -> str:
+def _op_constant(name: str, op) -> str:
+    """The op record as a namespace-scope constant, so that the device functions fold its fields
+    (targets, controls, matrix offset) instead of loading them."""
+    t = ", ".join(str(int(x)) for x in op["t"])
+    return (f"static constexpr cqs_op {name} = {{{int(op['kind'])}, {int(op['k'])}, {int(op['mat'])}, {int(op['slot'])}, "
+            f"{{{t}}}, {int(op['cl_mask'])}, {int(op['cl_val'])}, {int(op['co_mask_lo'])}u, {int(op['co_mask_hi'])}u, "
+            f"{int(op['co_val_lo'])}u, {int(op['co_val_hi'])}u}};\n")
+
+
+def _mixing_call(op, ref: str, L: int, tile: str) -> str:
     """tile.hpp run_mixing_op for one op with its kind, width and matrix offset resolved: an MFMA
     block or a VALU dense gate on the LDS tile `tile`."""
     kind, k, mat = int(op["kind"]) & 0xFF, int(op["k"]), int(op["mat"])
@@ -90,13 +99,13 @@ def _mixing_call(op, j: int, L: int, tile: str) -> str:
     if kind == _OP_BLOCK:
         a = f"P.mats + {mat}" + (f" + {var} * {2 << (2 * k)}" if var else "")
         full = "true" if L - k >= 5 else "false"
-        return (f"tile_block_mfma<{k}, NT, {full}>({tile}, L, ops[{j}], reinterpret_cast<const half8_t*>({a}), "
+        return (f"tile_block_mfma<{k}, NT, {full}>({tile}, L, {ref}, reinterpret_cast<const half8_t*>({a}), "
                 "P.mscale, 1.f / P.mscale);")
     m = f"scalar_ptr(P.mats) + {mat}" + (f" + {var} * {1 << (2 * k)}" if var else "")
     if k in (1, 2):
-        return f"tile_dense<{k}, NT>({tile}, L, ops[{j}], {m});"
+        return f"tile_dense<{k}, NT>({tile}, L, {ref}, {m});"
     if k == 3:
-        return f"stage_matrix<NT>(mstage, {m}, 64); __syncthreads(); tile_dense<3, NT>({tile}, L, ops[{j}], mstage);"
+        return f"stage_matrix<NT>(mstage, {m}, 64); __syncthreads(); tile_dense<3, NT>({tile}, L, {ref}, mstage);"
     raise ValueError(f"mixing op of kind {kind} on {k} bits has no specialised form")
 
 
@@ -107,13 +116,14 @@ def pass_source(index: int, ps, ops: np.ndarray) -> str:
     nt, per = _shape(L)
     tb = nt.bit_length() - 1
     first, count = int(ps["op_begin"]), int(ps["op_count"])
+    consts = "".join(_op_constant(f"kS{index}_{j}", ops[first + j]) for j in range(count))
     body = []
     j = 0
     while j < count:
         op = ops[first + j]
         kind = int(op["kind"]) & 0xFF
         if not _mixing(op, nt, per):
-            ref = f"ops[{j}]"
+            ref = f"kS{index}_{j}"
             guard = _outer_guard(op)
             if kind == _OP_PERMUTE:
                 body.append(f"  __syncthreads();\n  regs_to_tile_permuted<PER, NT>(v, tile, L, scalar_ptr(P.aux) + {int(op['t'][0])}, act);\n"
@@ -140,12 +150,12 @@ def pass_source(index: int, ps, ops: np.ndarray) -> str:
         while j < count and _mixing(ops[first + j], nt, per):
             op = ops[first + j]
             guard = _outer_guard(op)
-            call = _mixing_call(op, j, L, "tile")
+            call = _mixing_call(op, f"kS{index}_{j}", L, "tile")
             body.append(f"  __syncthreads();\n  {guard}{{ {call} }}" if guard else f"  __syncthreads();\n  {call}")
             j += 1
         body.append("  __syncthreads();\n  tile_to_regs<PER, NT>(v, tile, act);")
     lines = "\n".join(body)
-    return f"""
+    return consts + f"""
 extern "C" __global__ __launch_bounds__({nt}) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spec_{index}(
     float2* __restrict__ psi, TileProg P) {{
   constexpr int L = {L}, NT = {nt}, PER = {per};
@@ -223,13 +233,14 @@ def adjoint_pass_source(index: int, ps, ops: np.ndarray) -> str:
     nt, per = _shape(L)
     tb = nt.bit_length() - 1
     first, count = int(ps["op_begin"]), int(ps["op_count"])
+    consts = "".join(_op_constant(f"kA{index}_{j}", ops[first + j]) for j in range(count))
     body = []
     j = 0
     sel_states = {0: ("va", "vb"), 1: ("va",), 2: ("vb",)}
     while j < count:
         op = ops[first + j]
         kind, sel = int(op["kind"]) & 0xFF, (int(op["kind"]) >> 8) & 3
-        ref = f"ops[{j}]"
+        ref = f"kA{index}_{j}"
         if not _adj_mixing(op, nt, per):
             cond = _guard_cond(op)
             if kind in (_lib.OP_GRAD_DIAG, _lib.OP_GRAD_DENSE):
@@ -268,7 +279,7 @@ def adjoint_pass_source(index: int, ps, ops: np.ndarray) -> str:
         while j < count and _adj_mixing(ops[first + j], nt, per):
             op = ops[first + j]
             kind, sel = int(op["kind"]) & 0xFF, (int(op["kind"]) >> 8) & 3
-            ref = f"ops[{j}]"
+            ref = f"kA{index}_{j}"
             cond = _guard_cond(op)
             body.append("  __syncthreads();")
             if kind == _lib.OP_GRAD_DENSE:
@@ -279,17 +290,17 @@ def adjoint_pass_source(index: int, ps, ops: np.ndarray) -> str:
             else:
                 inner = []
                 if sel != 2:
-                    inner.append(_mixing_call(op, j, L, "tA"))
+                    inner.append(_mixing_call(op, ref, L, "tA"))
                 if sel == 0:
                     inner.append("__syncthreads();")
                 if sel != 1:
-                    inner.append(_mixing_call(op, j, L, "tB"))
+                    inner.append(_mixing_call(op, ref, L, "tB"))
                 text = " ".join(inner)
                 body.append(f"  {text}" if cond == "true" else f"  if ({cond}) {{ {text} }}")
             j += 1
         body.append("  __syncthreads();\n  tile_to_regs<PER, NT>(va, tA, act);\n  tile_to_regs<PER, NT>(vb, tB, act);")
     lines = "\n".join(body)
-    return f"""
+    return consts + f"""
 extern "C" __global__ __launch_bounds__({nt}) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_aspec_{index}(
     float2* __restrict__ psi, float2* __restrict__ phi, TileProg P, double* __restrict__ partial, int64_t stride) {{
   constexpr int L = {L}, NT = {nt}, PER = {per};
