@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _progress(msg: str) -> None:
+    """One line per finished leg on stderr (stdout carries only the result line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,8 +95,11 @@ def dm_leg(args):
     n = args.dm_qubits
     circuit = generators.noisy_line_circuit(n, args.dm_layers, seed=0)
     qubits = sorted(circuit.all_qubits())
-    sim = cirq.DensityMatrixSimulator()
+    sim = cirq.DensityMatrixSimulator(specialize_passes=not args.no_specialize)
+    t0 = time.perf_counter()
     comp = sim._compile(circuit, qubits)
+    t_compile = time.perf_counter() - t0
+    _progress(f"density-matrix program compiled in {t_compile:.1f} s")
     qmap = {q: i for i, q in enumerate(qubits)}
     terms = [pauli_masks(t, qmap, n) for t in generators.sum_z(qubits)]
     rho = engine.new_state(2 * n, sim._dev())
@@ -123,6 +131,8 @@ def dm_leg(args):
         "passes_ms": ms,
         "pass_avg_ms": ms / max(n_pass, 1),
         "pass_hbm_gbs": bytes_per_pass / (ms / 1e3 / max(n_pass, 1)) / 1e9,
+        "specialized_passes": comp.dev.specialized is not None,
+        "compile_s": t_compile,
         "tr_rho_sum_z": ev,
     }
     if not args.no_cpu_baseline:
@@ -222,12 +232,14 @@ def sweep_leg(args):
     t0 = time.perf_counter()
     be = BatchedExpectation(circuits, observables)
     t_compile = time.perf_counter() - t0
+    _progress(f"sweep programs compiled in {t_compile:.1f} s")
     be.run(values)  # plan, graph capture, warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     evs, grads = be.run(values)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    _progress(f"batched sweep {t_batch:.2f} s")
     # looped: the first point of each of the first K circuits, one at a time
     K = min(args.sweep_looped_circuits, len(circuits))
     ags = [AdjointGradient(circuits[c], PauliSum.from_pauli_strings(observables[c])) for c in range(K)]
@@ -464,8 +476,10 @@ def dm_adjoint_leg(args):
         return (time.perf_counter() - t0) / reps, r
 
     ga = DMAdjointGradient(circuit, H)
+    t_spec = _specialize_gradient(args, ga)
     t_adj, (v, g) = timed(lambda: ga.value_and_grad(vals), 3)
     gc = DMAdjointGradient(circuit, H, invert_channels=False)
+    _specialize_gradient(args, gc)
     t_ck, (vc, g_ck) = timed(lambda: gc.value_and_grad(vals), 1)
     K = min(8, G)
     base = dict(zip(syms, vals))
@@ -475,7 +489,7 @@ def dm_adjoint_leg(args):
             r = dict(base)
             r[syms[k]] = vals[k] + sgn * np.pi / 2
             resolvers.append(ParamResolver(r))
-    sim = cirq_amd.DensityMatrixSimulator()
+    sim = cirq_amd.DensityMatrixSimulator(specialize_passes=not args.no_specialize)
     t_ps_sample, evs = timed(lambda: sim.simulate_expectation_values_sweep(circuit, H, resolvers), 1)
     ev = np.array([float(np.real(e[0])) for e in evs])
     g_ps = (ev[0::2] - ev[1::2]) / 2
@@ -499,6 +513,7 @@ def dm_adjoint_leg(args):
         "max_abs_grad_diff_vs_param_shift": float(np.abs(g[:K] - g_ps).max()),
         "max_abs_grad_diff_checkpointed": float(np.abs(g - g_ck).max()),
         "value_diff_checkpointed": abs(v - vc),
+        "specialize_s": t_spec,
         "roofline": _sweep_bytes_roofline(
             [(2 * n, int(ga.prog.fwd.n_passes), int(ga.prog.rev.n_passes), 1)], t_adj,
             "whole gradient: forward passes (rho read + written once) and reverse passes (rho and lambda "
@@ -611,6 +626,7 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
     t0 = time.perf_counter()
     comp = sim._compile(circuit, qubits)
     t_compile = time.perf_counter() - t0
+    _progress(f"{n}-qubit grid program compiled and tuned in {t_compile:.1f} s")
     t_spec = None
     if not args.no_specialize:  # the tuned program's passes as straight-line kernels (cirq_amd.specialize)
         t0 = time.perf_counter()
@@ -731,8 +747,10 @@ def main():
         print(json.dumps({args.leg: LEGS[args.leg](args)}), flush=True)
         return
 
+    t0 = time.perf_counter()
     g = grid_workload(args, args.rows, args.cols, args.depth, args.reps, args.steps, args.warmup,
                       autotune=not args.no_autotune)
+    _progress(f"main step done in {time.perf_counter() - t0:.1f} s")
     n, n_ops, zz, comp, bits = g["n"], g["n_ops"], g["zz"], g["comp"], g["bits"]
     wall, pass_ms, n_pass = g["wall"], g["pass_ms"], g["n_pass"]
     avg_launch_s, bytes_per_launch, achieved = g["avg_launch_s"], g["bytes_per_launch"], g["achieved"]
@@ -794,22 +812,18 @@ def main():
             "specialize_s": g["specialize_s"],
         },
     }
-    if not args.no_q32:
-        out["extra"]["q32"] = q32_leg(args)
-    if not args.no_adjoint:
-        out["extra"]["adjoint"] = adjoint_leg(args)
-    if not args.no_qaoa:
-        out["extra"]["qaoa"] = qaoa_leg(args)
-    if not args.no_dm_adjoint:
-        out["extra"]["dm_adjoint"] = dm_adjoint_leg(args)
-    if not args.no_sweep:
-        out["extra"]["sweep"] = sweep_leg(args)
-    if not args.no_trajectories:
-        out["extra"]["trajectories"] = trajectory_leg(args)
-    if not args.no_dm:
-        out["extra"]["density_matrix"] = dm_leg(args)
+    legs = [("q32", args.no_q32), ("adjoint", args.no_adjoint), ("qaoa", args.no_qaoa),
+            ("dm_adjoint", args.no_dm_adjoint), ("sweep", args.no_sweep), ("trajectories", args.no_trajectories),
+            ("density_matrix", args.no_dm)]
+    for name, skip in legs:
+        if not skip:
+            t0 = time.perf_counter()
+            out["extra"][name] = LEGS[name](args)
+            _progress(f"{name} leg done in {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(args, args.rows, args.cols, args.depth, n)
+        _progress(f"cpu baseline done in {time.perf_counter() - t0:.1f} s")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -43,6 +43,9 @@ __device__ __forceinline__ double grad_dense(const float2* psi, const float2* ph
   }
   double acc = 0.0;
   const int G = (1 << L) >> K;
+  // one group at a time: with the op's bits as constants the trip count is known, and unrolling it
+  // would keep several groups' operands live at once (spills at 128 VGPRs)
+#pragma unroll 1
   for (int g = threadIdx.x; g < G; g += NT) {
     int j0 = g;
 #pragma unroll

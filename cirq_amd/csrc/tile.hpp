@@ -135,6 +135,9 @@ __device__ __forceinline__ void tile_dense(float2* tile, int L, Op& op, MPtr M) 
 #pragma unroll
   for (int m = 0; m < D; ++m) ppat[m] = pad(pat[m]);
   if ((int)threadIdx.x >= G) return;  // tiles with fewer groups than threads
+  // one group at a time: with the op's bits as constants the trip count is known, and unrolling it
+  // would keep several groups' operands live at once (spills at 128 VGPRs)
+#pragma unroll 1
   for (int g0 = 0; g0 < G; g0 += NT) {
     int jh = g0;
 #pragma unroll
@@ -236,6 +239,7 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, Op& op, con
   for (int m = 0; m < K; ++m) jl = insert0(jl, ts[m]);
   const int pl = jl + (jl >> kPadShift);
   const int b_in0 = pl + h_in, b_out0 = pl + h_out;
+#pragma unroll 1  // one column block at a time (as for the dense groups above)
   for (int cb = wave; cb < ncb; cb += NW) {
     const int col = cb * 32 + c32;
     const bool valid = kFull || col < ncol;

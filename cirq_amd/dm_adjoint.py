@@ -31,6 +31,7 @@ Cirq has no gradients: its users difference simulate_expectation_values_sweep ca
 
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -249,32 +250,50 @@ class DMAdjointGradient:
         G = self.prog.n_slots
         self.partial = torch.zeros(max(G, 1) * self.prog.partial_stride, dtype=torch.float64, device=self.device)
         self.gslot = torch.zeros(max(G, 1), dtype=torch.float64, device=self.device)
+        self.fwd_spec = self.rev_spec = None  # specialised kernels (specialize())
+
+    def specialize(self):
+        """Compiles the forward and reverse programs to straight-line kernels (cirq_amd.specialize)
+        and runs them from now on."""
+        from .specialize import specialize
+
+        self.fwd_spec = specialize(self.prog.fwd)
+        self.rev_spec = specialize(self.prog.rev, adjoint=True)
+        return self
+
+    @staticmethod
+    def _handles(spec, a: int):
+        """Launch handles of passes a, a+1, ... of a specialised program."""
+        return ctypes.c_void_p(ctypes.addressof(spec.handles) + a * ctypes.sizeof(ctypes.c_void_p))
 
     def _init_rho(self, buf: torch.Tensor):
         check(lib().cqs_init_basis(ptr(buf), self.prog.n_bits, 0, engine.stream_handle()), "cqs_init_basis")
 
     def _run_step(self, j: int, buf: torch.Tensor):
         a, b = self.prog.fwd_range[j]
-        if b > a:
-            check(
-                lib().cqs_run_passes(
-                    ptr(buf), self.prog.n_bits, self.dev_fwd.pass_arr[a:b].ctypes.data, b - a,
-                    ptr(self.dev_fwd.ops), ptr(self.dev_fwd.mats), ptr(self.dev_fwd.aux), engine.stream_handle(),
-                ),
-                "cqs_run_passes",
-            )
+        if b <= a:
+            return
+        d = self.dev_fwd
+        args = (ptr(buf), self.prog.n_bits)
+        rest = (d.pass_arr[a:b].ctypes.data, b - a, ptr(d.ops), ptr(d.mats), ptr(d.aux))
+        if self.fwd_spec is not None:
+            check(lib().cqs_run_passes_specialized(*args, -1, *rest, self._handles(self.fwd_spec, a),
+                                                   engine.stream_handle()), "cqs_run_passes_specialized")
+        else:
+            check(lib().cqs_run_passes(*args, *rest, engine.stream_handle()), "cqs_run_passes")
 
     def _reverse_step(self, j: int, psi: torch.Tensor):
         a, b = self.prog.rev_range[j]
-        if b > a:
-            check(
-                lib().cqs_adjoint_passes(
-                    ptr(psi), ptr(self.lam), self.prog.n_bits, self.dev_rev.pass_arr[a:b].ctypes.data, b - a,
-                    ptr(self.dev_rev.ops), ptr(self.dev_rev.mats), ptr(self.dev_rev.aux), ptr(self.partial),
-                    self.prog.partial_stride, engine.stream_handle(),
-                ),
-                "cqs_adjoint_passes",
-            )
+        if b <= a:
+            return
+        d = self.dev_rev
+        args = (ptr(psi), ptr(self.lam), self.prog.n_bits, d.pass_arr[a:b].ctypes.data, b - a, ptr(d.ops), ptr(d.mats),
+                ptr(d.aux), ptr(self.partial), self.prog.partial_stride)
+        if self.rev_spec is not None:
+            check(lib().cqs_adjoint_passes_specialized(*args, self._handles(self.rev_spec, a), engine.stream_handle()),
+                  "cqs_adjoint_passes_specialized")
+        else:
+            check(lib().cqs_adjoint_passes(*args, engine.stream_handle()), "cqs_adjoint_passes")
 
     def _set_mats(self, values) -> np.ndarray:
         symvals = self.prog.symbol_values(values)

@@ -122,3 +122,20 @@ def test_specialised_gradients_match_interpreter(gpu, kind):
     v1, g1 = ag.value_and_grad(vals)
     assert abs(v1 - v0) <= 1e-6 * max(1.0, abs(v0))
     np.testing.assert_allclose(g1, g0, atol=1e-6 * max(1.0, np.abs(g0).max()))
+
+
+@pytest.mark.parametrize("invert", [True, False])
+def test_specialised_density_matrix_gradients_match_interpreter(gpu, invert):
+    """DMAdjointGradient.specialize(): forward and reverse pass ranges (steps between checkpoints)
+    launched through specialised kernels give the interpreter's value and gradient."""
+    from cirq_amd.dm_adjoint import DMAdjointGradient
+
+    _log(f"specialised density-matrix gradients invert={invert}")
+    circuit, syms, vals = generators.noisy_line_ansatz(5, 3, seed=2, p=0.02, gamma=0.03)
+    H = generators.sum_z(sorted(circuit.all_qubits()))
+    g = DMAdjointGradient(circuit, H, invert_channels=invert)
+    v0, g0 = g.value_and_grad(vals)
+    g.specialize()
+    v1, g1 = g.value_and_grad(vals)
+    assert abs(v1 - v0) <= 1e-6
+    np.testing.assert_allclose(g1, g0, atol=1e-6)
