@@ -664,8 +664,9 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
             pass_events.append((e0, e1))
         u = sim._prng.random_sample(reps)
         idx = sim._sampler.draw(st, n, False, u, axes=axes)
+        ev = engine.expect_terms(st, n, False, terms, sim._ws)  # queued behind the sampling kernels
         bits = sim._sampler.bits(idx, n)
-        vals = engine.expect_terms(st, n, False, terms, sim._ws).cpu().numpy()
+        vals = ev.cpu().numpy()
         return bits, vals
 
     for _ in range(warmup):

@@ -403,73 +403,101 @@ __global__ __launch_bounds__(256) void k_super_fn(const ChunkFn* __restrict__ cf
   }
 }
 
-// Applies map f to exact value S if S lies in binade E and the result stays inside it.  Inside
-// one binade a double is its bit pattern: S in [2^E, 2^(E+1)) exactly when its exponent field is
-// E + 1023 (S is a positive normal double here), the ulp parity is the pattern's lowest bit, and
-// adding d >= 0 ulps (the maps only add probabilities) is adding d to the pattern, exact while the
-// exponent field stays the same.  Integer steps only: the walk that calls this is one serial chain.
-__device__ __forceinline__ bool try_apply(double& S, int E, long long d0, long long d1) {
-  if (!(S > 0.0)) return false;
+// Inside one binade a double is its bit pattern: S in [2^E, 2^(E+1)) exactly when its exponent
+// field is E + 1023 (S is a positive normal double here), the ulp parity is the pattern's lowest
+// bit, and adding d >= 0 ulps (the maps only add probabilities) is adding d to the pattern, exact
+// while the exponent field stays the same.
+//
+// One step of the walk over maps[0 .. avail) (avail <= 64) with one wave.  Lane j composes maps
+// 0 .. j (an inclusive scan of compose across the wave, earlier maps first), so a run of maps that
+// are usable in S's binade and keep the running value inside it advances S in one step; applying
+// the composed map is the same integer arithmetic as applying the maps one after another (the
+// value stays in the binade after every map because the sums only grow).  Returns the number of
+// maps applied (0 .. avail, uniform); `before` of lane j is the value before map j, for every
+// j <= the returned count.
+__device__ __forceinline__ int walk_maps(double& S, const ChunkFn* maps, int avail, double& before) {
   constexpr long long kExp = 0x7ff0000000000000ll;
-  const long long ebits = (long long)(E + 1023) << 52;
+  const int lane = threadIdx.x & 63;
+  long long d0 = 0, d1 = 0;
+  int E = 0, ok = 0;
+  if (lane < avail) {
+    const ChunkFn f = maps[lane];
+    d0 = f.d0;
+    d1 = f.d1;
+    E = f.E;
+    ok = f.ok;
+  }
   const long long b = __double_as_longlong(S);
-  if ((b & kExp) != ebits) return false;
+  const long long ebits = b & kExp;
+  const bool usable = S > 0.0 && lane < avail && ok && ((long long)(E + 1023) << 52) == ebits;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    long long p0 = __shfl_up(d0, o, 64), p1 = __shfl_up(d1, o, 64);
+    if (lane >= o) {
+      compose(p0, p1, d0, d1);  // maps before this lane's window first
+      d0 = p0;
+      d1 = p1;
+    }
+  }
   const long long r = b + ((b & 1) ? d1 : d0);
-  if ((r & kExp) != ebits) return false;
-  S = __longlong_as_double(r);
-  return true;
+  const unsigned long long stays = __ballot(usable && (r & kExp) == ebits);
+  const int n = ~stays ? __builtin_ctzll(~stays) : 64;  // first lane whose map does not apply
+  const long long rp = __shfl_up(r, 1, 64);
+  before = lane == 0 ? S : __longlong_as_double(rp);
+  if (n > 0) S = __longlong_as_double(__shfl(r, n - 1, 64));
+  return n;
 }
 
-// Reads a map with all its fields in one round trip: the empty asm uses every field, so the loads
-// are issued together instead of one after each test of the walk.
-__device__ __forceinline__ ChunkFn load_map(const ChunkFn* p) {
-  ChunkFn f = *p;
-  asm volatile("" ::"v"(f.d0), "v"(f.d1), "v"(f.E), "v"(f.ok));
-  return f;
-}
-
-// One workgroup of 64 lanes walks the superchunks in order.  All lanes keep identical copies
-// of S; lanes cooperate only to stage the elements of a chunk that must be summed one by one.
+// One workgroup of 64 lanes walks the superchunks in order, up to 64 maps per step (walk_maps).
+// All lanes keep identical copies of S.  A superchunk whose composed map does not apply is walked
+// by its chunk maps; a chunk whose map does not apply (it crosses a binade edge, or S is still 0)
+// is summed element by element from staged values.
 __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, uint64_t NSC, uint64_t SCl,
                                                     uint64_t Cl, const ChunkFn* __restrict__ cfn,
                                                     const ChunkFn* __restrict__ scfn,
                                                     double* __restrict__ scstart, int* __restrict__ scmode,
                                                     double* __restrict__ cstart) {
-  constexpr int kStage = 512;  // superchunk maps staged in LDS, so the serial walk reads LDS only
+  constexpr int kStage = 512;  // superchunk maps staged in LDS, so the walk reads LDS only
   __shared__ double xs[kChunk];
   __shared__ ChunkFn sf[kStage];
   __shared__ ChunkFn cf[kSuper];  // the chunk maps of a superchunk walked chunk by chunk
   const float T32 = hdr->T32;
   const int t = threadIdx.x;
+  const int ncs = (int)SCl;  // chunks per superchunk (<= kSuper)
   double S = 0.0;
   int nseq = 0;
-  for (uint64_t sc = 0; sc < NSC; ++sc) {
-    if (sc % kStage == 0) {
-      __syncthreads();
-      for (uint64_t i = t; i < kStage && sc + i < NSC; i += 64) sf[i] = scfn[sc + i];
-      __syncthreads();
-    }
-    if (t == 0) scstart[sc] = S;
-    const ChunkFn F = load_map(sf + sc % kStage);
-    if (F.ok && try_apply(S, F.E, F.d0, F.d1)) {
-      if (t == 0) scmode[sc] = 1;
-      continue;
-    }
-    if (t == 0) scmode[sc] = 0;
-    // one batch of loads for the superchunk's maps instead of a dependent load per chunk
+  for (uint64_t s0 = 0; s0 < NSC; s0 += kStage) {
+    const int ns = (int)(NSC - s0 < (uint64_t)kStage ? NSC - s0 : (uint64_t)kStage);
     __syncthreads();
-    for (uint64_t i = t; i < SCl; i += 64) cf[i] = cfn[sc * SCl + i];
+    for (int i = t; i < ns; i += 64) sf[i] = scfn[s0 + i];
     __syncthreads();
-    for (uint64_t k = 0; k < SCl; ++k) {
-      const uint64_t c = sc * SCl + k;
-      if (t == 0) cstart[c] = S;
-      const ChunkFn f = load_map(cf + k);
-      if (f.ok && try_apply(S, f.E, f.d0, f.d1)) continue;
+    for (int k = 0; k < ns;) {
+      double before;
+      const int adv = walk_maps(S, sf + k, ns - k < 64 ? ns - k : 64, before);
+      if (t <= adv && k + t < ns) scstart[s0 + k + t] = before;  // lane adv: the next superchunk's start
+      if (t < adv) scmode[s0 + k + t] = 1;
+      k += adv;
+      if (adv == 64 || k >= ns) continue;
+      const uint64_t sc = s0 + k;  // walked by its chunk maps
+      if (t == 0) scmode[sc] = 0;
       __syncthreads();
-      for (uint64_t i = t; i < Cl; i += 64) xs[i] = px(ld, T32, c * Cl + i);
+      for (int i = t; i < ncs; i += 64) cf[i] = cfn[sc * SCl + i];
       __syncthreads();
-      for (uint64_t i = 0; i < Cl; ++i) S = S + xs[i];
-      ++nseq;
+      for (int c = 0; c < ncs;) {
+        double cb;
+        const int a = walk_maps(S, cf + c, ncs - c < 64 ? ncs - c : 64, cb);
+        if (t <= a && c + t < ncs) cstart[sc * SCl + c + t] = cb;
+        c += a;
+        if (a == 64 || c >= ncs) continue;
+        const uint64_t ch = sc * SCl + c;  // summed element by element
+        __syncthreads();
+        for (uint64_t i = t; i < Cl; i += 64) xs[i] = px(ld, T32, ch * Cl + i);
+        __syncthreads();
+        for (uint64_t i = 0; i < Cl; ++i) S = S + xs[i];
+        ++nseq;
+        ++c;
+      }
+      ++k;
     }
   }
   if (t == 0) {

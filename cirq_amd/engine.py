@@ -429,7 +429,9 @@ class Sampler:
             lib().cqs_sample_prepare(ptr(src), ptr(probs), n_qubits, int(is_dm), ptr(ax), ptr(w), stream_handle()),
             "cqs_sample_prepare",
         )
-        u = torch.from_numpy(np.ascontiguousarray(uniforms, dtype=np.float64)).to(self.device)
+        # staged through page-locked memory and queued: the host does not wait for the state's passes
+        u = torch.from_numpy(np.ascontiguousarray(uniforms, dtype=np.float64)).pin_memory()
+        u = u.to(self.device, non_blocking=True)
         idx = torch.empty(max(reps, 1), dtype=torch.int64, device=self.device)
         check(
             lib().cqs_sample_draw(
@@ -453,4 +455,9 @@ class Sampler:
         reps = int(idx.shape[0])
         out = torch.empty(max(reps * m, 1), dtype=torch.uint8, device=self.device)
         check(lib().cqs_indices_to_bits(ptr(idx), reps, m, ptr(out), stream_handle()), "cqs_indices_to_bits")
-        return out[: reps * m].cpu().numpy().reshape(reps, m)
+        # one DMA copy into page-locked memory (torch's host allocator hands the block out again
+        # once the returned array is dropped) instead of a staged copy into fresh pageable memory
+        host = torch.empty(out.shape, dtype=torch.uint8, pin_memory=True)
+        host.copy_(out, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        return host[: reps * m].numpy().reshape(reps, m)
