@@ -232,22 +232,78 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, Op& op, con
 #pragma unroll
     for (int f = 0; f < KS * RB * 2; ++f) ahold[f] = A[f * 64 + lane];
   }
-  // Inserting zeros at the targets and pad() are both additive over disjoint bits: the lane's
-  // column bits (c32) are placed once, the column block's (cb * 32) per block on the scalar unit.
-  int jl = c32;
+  // Which non-target tile bits carry the lane's column bits (c32) and which the column block's.
+  // Any split of the non-target bits between the two visits every column once, and pad() is
+  // additive over disjoint bits, so the lane part is placed once and the block part per block on
+  // the scalar unit.  LDS serves an 8-byte access 16 lanes per cycle over 32 four-byte banks: the
+  // 16 lanes of one cycle (c32 bits 0-3) need padded word offsets that differ modulo 16.
+  // pad(1 << b) is 2^b modulo 16 for b < 4 and 2^(b-5) for 5 <= b < 9, so (whole column blocks)
+  // c32 bit j takes tile bit j, or bit j + 5 when j is a target; c32 bit 4, and a bit j whose two
+  // choices are both targets, take the lowest free bits; the block index takes the rest in order.
+  // Partial column blocks (L - K < 5) keep the five lowest non-target bits for the lane.
+  int jl = 0;
+  uint32_t gmask = 0;  // whole column blocks: the tile bits of the column block index
+  if constexpr (kFull) {
+    uint32_t used = 0;
 #pragma unroll
-  for (int m = 0; m < K; ++m) jl = insert0(jl, ts[m]);
+    for (int m = 0; m < K; ++m) used |= 1u << ts[m];
+    int lp[5];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      lp[j] = !((used >> j) & 1) ? j : !((used >> (j + 5)) & 1) ? j + 5 : -1;
+      if (lp[j] >= 0) used |= 1u << lp[j];
+    }
+    lp[4] = -1;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      if (lp[j] < 0) {
+        lp[j] = __builtin_ctz(~used);
+        used |= 1u << lp[j];
+      }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) jl |= ((c32 >> j) & 1) << lp[j];
+    gmask = ~used & ((1u << L) - 1);
+  } else {
+    jl = c32;
+#pragma unroll
+    for (int m = 0; m < K; ++m) jl = insert0(jl, ts[m]);
+  }
   const int pl = jl + (jl >> kPadShift);
   const int b_in0 = pl + h_in, b_out0 = pl + h_out;
-#pragma unroll 1  // one column block at a time (as for the dense groups above)
-  for (int cb = wave; cb < ncb; cb += NW) {
-    const int col = cb * 32 + c32;
-    const bool valid = kFull || col < ncol;
-    int jh = cb * 32;
+  // A column block's padded input / output offsets and its LDS writes.
+  auto offsets = [&](int cb, int& b_in, int& b_out) {
+    int jh = 0;
+    if constexpr (kFull) {
+      uint32_t g = gmask;
+      for (int i = 0; g != 0; ++i) {  // deposit cb's bits at the block bits, lowest first
+        const int b = __builtin_ctz(g);
+        g &= g - 1;
+        jh |= ((cb >> i) & 1) << b;
+      }
+    } else {
+      jh = cb * 32;
 #pragma unroll
-    for (int m = 0; m < K; ++m) jh = insert0(jh, ts[m]);
+      for (int m = 0; m < K; ++m) jh = insert0(jh, ts[m]);
+    }
     const int ph = jh + (jh >> kPadShift);
-    const int b_in = b_in0 + ph, b_out = b_out0 + ph;
+    b_in = b_in0 + ph;
+    b_out = b_out0 + ph;
+  };
+  auto store = [&](int b_out, bool valid, const floatx16_t (&acc)[RB]) {
+    if (valid) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int a = 0; a < 16; a += 2)  // a even: real part, a + 1: imaginary part of one row
+          tile[b_out + prow(rb * 16 + 4 * (a >> 2) + ((a >> 1) & 1))] =
+              make_float2(acc[rb][a] * inv_scale, acc[rb][a + 1] * inv_scale);
+    }
+  };
+#pragma unroll 1  // one column block at a time, each K-slice split as it is read
+  for (int cb = wave; cb < ncb; cb += NW) {
+    int b_in, b_out;
+    offsets(cb, b_in, b_out);
+    const bool valid = kFull || cb * 32 + c32 < ncol;
     floatx16_t acc[RB];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
@@ -258,11 +314,11 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, Op& op, con
         split_f16(v.x, v.y, scale, hh[e], ll[e]);
       }
       const half8_t bhi = __builtin_shufflevector(__builtin_shufflevector(hh[0], hh[1], 0, 1, 2, 3),
-                                                  __builtin_shufflevector(hh[2], hh[3], 0, 1, 2, 3), 0, 1, 2, 3, 4,
-                                                  5, 6, 7);
+                                                  __builtin_shufflevector(hh[2], hh[3], 0, 1, 2, 3), 0, 1, 2, 3,
+                                                  4, 5, 6, 7);
       const half8_t blo = __builtin_shufflevector(__builtin_shufflevector(ll[0], ll[1], 0, 1, 2, 3),
-                                                  __builtin_shufflevector(ll[2], ll[3], 0, 1, 2, 3), 0, 1, 2, 3, 4,
-                                                  5, 6, 7);
+                                                  __builtin_shufflevector(ll[2], ll[3], 0, 1, 2, 3), 0, 1, 2, 3,
+                                                  4, 5, 6, 7);
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) {
         half8_t ahi, alo;
@@ -279,14 +335,7 @@ __device__ __forceinline__ void tile_block_mfma(float2* tile, int L, Op& op, con
         acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, blo, acc[rb], 0, 0, 0);
       }
     }
-    if (valid) {
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int a = 0; a < 16; a += 2)  // a even: real part, a + 1: imaginary part of one row
-          tile[b_out + prow(rb * 16 + 4 * (a >> 2) + ((a >> 1) & 1))] =
-              make_float2(acc[rb][a] * inv_scale, acc[rb][a + 1] * inv_scale);
-    }
+    store(b_out, valid, acc);
   }
 }
 

@@ -91,16 +91,18 @@ def _op_constant(name: str, op) -> str:
             f"{int(op['co_val_lo'])}u, {int(op['co_val_hi'])}u}};\n")
 
 
-def _mixing_call(op, ref: str, L: int, tile: str) -> str:
+def _mixing_call(op, ref: str, L: int, tile: str, scaled: bool = False) -> str:
     """tile.hpp run_mixing_op for one op with its kind, width and matrix offset resolved: an MFMA
-    block or a VALU dense gate on the LDS tile `tile`."""
+    block or a VALU dense gate on the LDS tile `tile`.  scaled: the tile already holds amplitudes
+    times P.mscale, so an MFMA block splits and stores them as they are."""
     kind, k, mat = int(op["kind"]) & 0xFF, int(op["k"]), int(op["mat"])
     var = _variant_expr(op)
     if kind == _OP_BLOCK:
         a = f"P.mats + {mat}" + (f" + {var} * {2 << (2 * k)}" if var else "")
         full = "true" if L - k >= 5 else "false"
+        scale = "1.f, 1.f" if scaled else "P.mscale, 1.f / P.mscale"
         return (f"tile_block_mfma<{k}, NT, {full}>({tile}, L, {ref}, reinterpret_cast<const half8_t*>({a}), "
-                "P.mscale, 1.f / P.mscale);")
+                f"{scale});")
     m = f"scalar_ptr(P.mats) + {mat}" + (f" + {var} * {1 << (2 * k)}" if var else "")
     if k in (1, 2):
         return f"tile_dense<{k}, NT>({tile}, L, {ref}, {m});"
@@ -117,6 +119,10 @@ def pass_source(index: int, ps, ops: np.ndarray) -> str:
     tb = nt.bit_length() - 1
     first, count = int(ps["op_begin"]), int(ps["op_count"])
     consts = "".join(_op_constant(f"kS{index}_{j}", ops[first + j]) for j in range(count))
+    # Every op of a pass is linear, and scaling by the power of two P.mscale is exact, so a pass
+    # with MFMA blocks scales its amplitudes once after the load and once before the store instead
+    # of around every block (same results: FP32 rounding does not depend on a power-of-two scale).
+    scaled = any((int(ops[first + j]["kind"]) & 0xFF) == _OP_BLOCK for j in range(count))
     body = []
     j = 0
     while j < count:
@@ -150,11 +156,18 @@ def pass_source(index: int, ps, ops: np.ndarray) -> str:
         while j < count and _mixing(ops[first + j], nt, per):
             op = ops[first + j]
             guard = _outer_guard(op)
-            call = _mixing_call(op, f"kS{index}_{j}", L, "tile")
+            call = _mixing_call(op, f"kS{index}_{j}", L, "tile", scaled)
             body.append(f"  __syncthreads();\n  {guard}{{ {call} }}" if guard else f"  __syncthreads();\n  {call}")
             j += 1
         body.append("  __syncthreads();\n  tile_to_regs<PER, NT>(v, tile, act);")
     lines = "\n".join(body)
+    one = "P.mscale" if scaled else "1.f"
+    load = "v[e] = psi[base + off_lo + off_hi[e]];"
+    if scaled:
+        load = ("{ const float2 a = psi[base + off_lo + off_hi[e]]; "
+                "v[e] = make_float2(a.x * P.mscale, a.y * P.mscale); }")
+    store = "make_float2(v[e].x * inv, v[e].y * inv)" if scaled else "v[e]"
+    inv = "\n    const float inv = 1.f / P.mscale;" if scaled else ""
     return consts + f"""
 extern "C" __global__ __launch_bounds__({nt}) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spec_{index}(
     float2* __restrict__ psi, TileProg P) {{
@@ -179,20 +192,20 @@ extern "C" __global__ __launch_bounds__({nt}) __attribute__((amdgpu_waves_per_eu
       if (P.basis >= 0) {{
 #pragma unroll
         for (int e = 0; e < PER; ++e)
-          v[e] = make_float2(base + off_lo + off_hi[e] == (uint64_t)P.basis ? 1.f : 0.f, 0.f);
+          v[e] = make_float2(base + off_lo + off_hi[e] == (uint64_t)P.basis ? {one} : 0.f, 0.f);
       }} else {{
 #pragma unroll
-        for (int e = 0; e < PER; ++e) v[e] = psi[base + off_lo + off_hi[e]];
+        for (int e = 0; e < PER; ++e) {load}
       }}
     }}
   }}
 {lines}
   {{
     uint64_t off_lo, off_hi[PER];
-    tile_offsets<L>(P.lpos_out, off_lo, off_hi);
+    tile_offsets<L>(P.lpos_out, off_lo, off_hi);{inv}
     if (act) {{
 #pragma unroll
-      for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = v[e];
+      for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = {store};
     }}
   }}
 }}
