@@ -134,28 +134,48 @@ __global__ __launch_bounds__(64) void k_expect_final(const double* __restrict__ 
   }
 }
 
-// Terms without X or Y (x mask 0): sum_i (-1)^popcount(i & z) |psi_i|^2.  Each thread takes
-// 16 consecutive amplitudes i = 16g + j, so the sign splits into (-1)^popcount(g & z>>4), the same
-// for the 16, times (-1)^popcount(j & z&15), the same for every thread: one FP64 FMA per
-// amplitude and distinct low mask, then one add per term.  |psi_i|^2 is exact in FP64 up to its
-// final rounding.
-constexpr int kZTerms = 32;
-constexpr int kZBlocks = 2048;  // fixed grid => deterministic reduction order
+// Terms without X or Y (x mask 0): sum_i (-1)^popcount(i & z) |psi_i|^2, up to 64 terms per read
+// of the state.  Each thread takes 16 consecutive amplitudes i = 16g + j, so the sign splits into
+// (-1)^popcount(g & z>>4), the same for the 16, times (-1)^popcount(j & z&15), the same for every
+// thread: one FP64 FMA per amplitude and distinct low mask, then one add per term.  |psi_i|^2 is
+// exact in FP64 up to its final rounding.  Which thread sums which groups does not depend on the
+// batch, so every term's reduction order is fixed.
+constexpr int kZTerms = 64;       // terms per launch (FP64 accumulators in registers)
+constexpr int kZBatch = kZTerms;
+constexpr int kZBlocks = 2048;    // fixed grid => deterministic reduction order
 struct ZBatch {
   int count;
   int pad;
-  uint64_t z[kZTerms];
+  uint64_t newrun;  // bit t: term t's low 4 z bits differ from term t - 1's (bit 0 set)
+  uint64_t z[kZBatch];
 };
 
 __global__ __launch_bounds__(256) void k_expect_z(const float2* __restrict__ amps, int n, ZBatch zb,
                                                   double* __restrict__ partials) {
   __shared__ double red[4][kZTerms];
+  __shared__ int zls[kZTerms];  // low 4 z bits per term, read inside the loop (not kept in registers)
+  const int tid = threadIdx.x;
+  const int cnt = zb.count;
+  if (tid < kZTerms) zls[tid] = tid < cnt ? (int)(zb.z[tid] & 15) : 0;
+  __syncthreads();
   double acc[kZTerms];
 #pragma unroll
   for (int t = 0; t < kZTerms; ++t) acc[t] = 0.0;
   const uint64_t G = 1ull << (n - 4);
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < G; g += stride) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;  // 2^19 whenever G >= 2^19 (full grid)
+  const uint64_t lo = (uint64_t)blockIdx.x * 256 + tid;
+  // The group sign of term t at g = lo + k * stride is parity(lo & z_t>>4), fixed per thread and
+  // applied once after the loop (negation is exact, so the sum is the same), times
+  // parity(k * stride & z_t>>4), the same for every thread of an iteration: lane t evaluates it
+  // for term t and a ballot hands the bits to the whole wave.  Every thread runs the same
+  // iterations, so the ballot sees all lanes.
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t zh_lane = lane < cnt ? zb.z[lane] >> 4 : 0;
+  const uint64_t iters = G > stride ? G / stride : 1;
+  for (uint64_t k = 0; k < iters; ++k) {
+    const uint64_t kpar = __ballot(__popcll((k * stride) & zh_lane) & 1);
+    const uint64_t g = lo + k * stride;
+    if (g >= G) continue;
     const float4* src = reinterpret_cast<const float4*>(amps + 16 * g);
     double p[16];
 #pragma unroll
@@ -165,56 +185,51 @@ __global__ __launch_bounds__(256) void k_expect_z(const float2* __restrict__ amp
       p[2 * q + 1] = fma((double)v.z, (double)v.z, (double)v.w * (double)v.w);
     }
     // The host orders a batch's terms by their low 4 z bits, so the signed 16-term sum is formed
-    // once per run of equal low masks (a uniform branch) and shared by the run's terms.
-    int zl_done = -1;
+    // once per run of equal low masks (a uniform branch) and shared by the run's terms.  The run's
+    // mask is read from LDS there (a barrier to code motion keeps 64 masks out of the registers).
+    asm volatile("" ::: "memory");
     double w = 0.0;
 #pragma unroll
     for (int t = 0; t < kZTerms; ++t) {
-      if (t < zb.count) {
-        const uint64_t z = zb.z[t];
-        const int zl = (int)(z & 15);
-        if (zl != zl_done) {
+      if (t < cnt) {
+        if ((zb.newrun >> t) & 1) {
+          const int zl = __builtin_amdgcn_readfirstlane(zls[t]);
           w = 0.0;
 #pragma unroll
           for (int j = 0; j < 16; ++j) w = fma((__popc(j & zl) & 1) ? -1.0 : 1.0, p[j], w);
-          zl_done = zl;
         }
-        acc[t] += (__popcll(g & (z >> 4)) & 1) ? -w : w;
+        acc[t] += ((kpar >> t) & 1) ? -w : w;
       }
     }
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int t = 0; t < kZTerms; ++t) {
-    if (t < zb.count) {
-      const double sv = wave_sum(acc[t]);
+    if (t < cnt) {
+      const double sv = wave_sum((__popcll(lo & (zb.z[t] >> 4)) & 1) ? -acc[t] : acc[t]);
       if (lane == 0) red[wv][t] = sv;
     }
   }
   __syncthreads();
-  if ((int)threadIdx.x < zb.count) {
-    const int t = threadIdx.x;
-    partials[(size_t)blockIdx.x * kZTerms + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-  }
+  if (tid < cnt) partials[(size_t)blockIdx.x * kZBatch + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
 }
 
 struct ZFinal {
   int count;
   int nblocks;
-  int out_idx[kZTerms];
-  double cr[kZTerms], ci[kZTerms];
+  int out_idx[kZBatch];
+  double cr[kZBatch], ci[kZBatch];
 };
 
+// One wave per term (workgroup k sums term k's block partials in a fixed order).
 __global__ __launch_bounds__(64) void k_expect_z_final(const double* __restrict__ partials, ZFinal fb,
                                                        double* __restrict__ out) {
-  for (int k = 0; k < fb.count; ++k) {
-    double sv = 0.0;
-    for (int b = threadIdx.x; b < fb.nblocks; b += 64) sv += partials[(size_t)b * kZTerms + k];
-    sv = wave_sum(sv);
-    if (threadIdx.x == 0) {
-      out[2 * fb.out_idx[k] + 0] = fb.cr[k] * sv;
-      out[2 * fb.out_idx[k] + 1] = fb.ci[k] * sv;
-    }
+  const int k = blockIdx.x;
+  double sv = 0.0;
+  for (int b = threadIdx.x; b < fb.nblocks; b += 64) sv += partials[(size_t)b * kZBatch + k];
+  sv = wave_sum(sv);
+  if (threadIdx.x == 0) {
+    out[2 * fb.out_idx[k] + 0] = fb.cr[k] * sv;
+    out[2 * fb.out_idx[k] + 1] = fb.ci[k] * sv;
   }
 }
 
@@ -265,7 +280,7 @@ static int check_terms(int32_t n_qubits, int32_t is_dm, int32_t n_terms, const u
 }
 
 // Launches over the terms in `order` (identity when NULL): runs of terms with one X mask share a
-// k_expect launch (up to 16 terms), runs of Z-only terms a k_expect_z launch (up to 32).  Every
+// k_expect launch (up to 16 terms), runs of Z-only terms a k_expect_z launch (up to 64).  Every
 // term's sum has a fixed reduction order whatever batch it lands in, so any order of the terms
 // gives the same bits.  `partials` holds one launch's per-block partial sums.
 static int expect_batches(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms, const int* order,
@@ -282,8 +297,10 @@ static int expect_batches(const void* amps, int32_t n_qubits, int32_t is_dm, int
       ZFinal zf;
       zb.count = 0;
       zb.pad = 0;
-      while (i < n_terms && zb.count < kZTerms && xmask[term(i)] == 0) {
+      zb.newrun = 0;
+      while (i < n_terms && zb.count < kZBatch && xmask[term(i)] == 0) {
         const int t = term(i);
+        if (zb.count == 0 || (zmask[t] & 15) != (zb.z[zb.count - 1] & 15)) zb.newrun |= 1ull << zb.count;
         zb.z[zb.count] = zmask[t];
         zf.out_idx[zb.count] = t;
         zf.cr[zb.count] = coef[2 * t];
@@ -291,12 +308,12 @@ static int expect_batches(const void* amps, int32_t n_qubits, int32_t is_dm, int
         ++zb.count;
         ++i;
       }
-      for (int k = zb.count; k < kZTerms; ++k) zb.z[k] = 0;
+      for (int k = zb.count; k < kZBatch; ++k) zb.z[k] = 0;
       zf.count = zb.count;
       zf.nblocks = zgrid;
       hipLaunchKernelGGL(k_expect_z, dim3(zgrid), dim3(256), 0, s, (const float2*)amps, (int)n_qubits, zb,
                          partials);
-      hipLaunchKernelGGL(k_expect_z_final, dim3(1), dim3(64), 0, s, (const double*)partials, zf, out);
+      hipLaunchKernelGGL(k_expect_z_final, dim3(zf.count), dim3(64), 0, s, (const double*)partials, zf, out);
       continue;
     }
     TermBatch tb;
@@ -325,11 +342,11 @@ static int expect_batches(const void* amps, int32_t n_qubits, int32_t is_dm, int
 }
 
 // Static scratch for cqs_expect_paulis_noalloc: one launch's partial sums, the largest any batch
-// writes (2048 blocks x 32 Z terms, or 512 blocks x 16 terms x 2 doubles): 512 KiB per GPU, held
+// writes (2048 blocks x 64 Z terms, or 512 blocks x 16 terms x 2 doubles): 1 MiB per GPU, held
 // until the process ends.  In host code `scratch` is only a handle for hipGetSymbolAddress; the
 // kernels get the device address that call returns, kept per device.
-static __device__ __align__(256) double scratch[kZTerms * kZBlocks > 2 * kTermsPerLaunch * kRedBlocks
-                                                    ? kZTerms * kZBlocks
+static __device__ __align__(256) double scratch[kZBatch * kZBlocks > 2 * kTermsPerLaunch * kRedBlocks
+                                                    ? kZBatch * kZBlocks
                                                     : 2 * kTermsPerLaunch * kRedBlocks];
 constexpr int kMaxDevices = 64;
 static std::atomic<double*> g_scratch_ptr[kMaxDevices];
@@ -360,7 +377,7 @@ extern "C" {
 
 size_t cqs_expect_workspace_bytes(int32_t n_terms) {
   (void)n_terms;
-  return sizeof(double) * std::max(2 * kTermsPerLaunch * kRedBlocks, kZTerms * kZBlocks);
+  return sizeof(double) * std::max(2 * kTermsPerLaunch * kRedBlocks, kZBatch * kZBlocks);
 }
 
 int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t n_terms,

@@ -581,11 +581,28 @@ __global__ __launch_bounds__(256) void k_draw(Loader ld, const SampleHdr* hdr, u
   const uint64_t c = lo;
   double s = cstart[c];
   int64_t idx = (int64_t)(c * Cl + Cl - 1);
-  for (uint64_t i = 0; i < Cl; ++i) {
-    s = s + px(ld, T32, c * Cl + i);
-    if (s / T > u) {
-      idx = (int64_t)(c * Cl + i);
-      break;
+  if (Cl % 4 == 0) {  // four probabilities per load, added one at a time as NumPy's cumsum does
+    for (uint64_t i = 0; i < Cl; i += 4) {
+      const float4 q = ld.q4(c * Cl + i);
+      const float pq[4] = {q.x, q.y, q.z, q.w};
+      int hit = -1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s = s + (double)__fdiv_rn(pq[k], T32);
+        if (hit < 0 && s / T > u) hit = k;
+      }
+      if (hit >= 0) {
+        idx = (int64_t)(c * Cl + i + hit);
+        break;
+      }
+    }
+  } else {
+    for (uint64_t i = 0; i < Cl; ++i) {
+      s = s + px(ld, T32, c * Cl + i);
+      if (s / T > u) {
+        idx = (int64_t)(c * Cl + i);
+        break;
+      }
     }
   }
   out[r] = idx;
@@ -703,7 +720,7 @@ __global__ __launch_bounds__(256) void k_marginal(Loader ld, int n, AxisBits ab,
 // out[j] = p(map(j)) where output bit b is input flat bit perm[b] and p = |amp|^2 as NumPy rounds
 // it (state vector) or Re rho[k, k] (density matrix).  One workgroup per tile of 2^T output
 // indices over the output bits S[0..T) (ascending); loads walk the tile with the bits that feed
-// input bits 0..3 lowest (contiguous reads), stores with output bits 0..3 lowest (contiguous
+// input bits 0..4 lowest (contiguous reads), stores with output bits 0..4 lowest (contiguous
 // writes), and LDS holds the tile in between.
 struct PermTile {
   int n, T, nouter, is_dm;
@@ -938,7 +955,8 @@ int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
   pt.is_dm = is_dm ? 1 : 0;
   for (int b = 0; b < n; ++b) pt.perm[b] = perm[b];
   const int T = n < kPermT ? n : kPermT;
-  // tile bits: output bits 0..3, the output bits feeding input bits 0..3, then the lowest others
+  // tile bits: output bits 0..4, the output bits feeding input bits 0..4, then the lowest others
+  // (loads and stores in runs of at least 32 elements)
   bool in_s[64] = {false};
   int inv[64];
   for (int b = 0; b < n; ++b) inv[perm[b]] = b;
@@ -949,8 +967,8 @@ int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
       ++cnt;
     }
   };
-  for (int b = 0; b < 4 && b < n; ++b) take(b);
-  for (int b = 0; b < 4 && b < n; ++b) take(inv[b]);
+  for (int b = 0; b < 5 && b < n; ++b) take(b);
+  for (int b = 0; b < 5 && b < n; ++b) take(inv[b]);
   for (int b = 0; b < n; ++b) take(b);
   int k = 0;
   int tile_of[64];
@@ -965,7 +983,7 @@ int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
   k = 0;
   for (int q = 0; q < n && k < T; ++q) {
     const int b = inv[q];
-    if (in_s[b] && !used[tile_of[b]] && q < 4) {
+    if (in_s[b] && !used[tile_of[b]] && q < 5) {
       used[tile_of[b]] = true;
       pt.A[k++] = tile_of[b];
     }

@@ -312,7 +312,7 @@ int cqs_expect_paulis(const void* amps, int32_t n_qubits, int32_t is_dm, int32_t
 /*
  * cqs_expect_paulis without a workspace argument and without any allocation (no hipMalloc,
  * hipFree or allocating library call, host or device): the per-block partial sums live in a
- * static 512 KiB device array per GPU, located once per device with hipGetSymbolAddress and kept
+ * static 1 MiB device array per GPU, located once per device with hipGetSymbolAddress and kept
  * until the process ends.  No counterpart in the reference: it is for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).
  * Terms are batched in the caller's order (runs of one X mask share a launch); the values are
