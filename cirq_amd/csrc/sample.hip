@@ -124,7 +124,9 @@ __device__ __forceinline__ double ldexp_d(double x, int e) { return ldexp(x, e);
 // ---------------------------------------------------------------- NumPy float32 sum
 // Pairwise sum of one buffer of len (power of two <= 8192) exactly as NumPy's
 // FLOAT_pairwise_sum: blocks of 128 with 8 strided accumulators, halves combined recursively.
-__global__ __launch_bounds__(256) void k_sum_buffers(Loader ld, uint64_t len, float* __restrict__ out) {
+// craw (optional, chunks of Cl elements): each chunk's sum of q in FP64, before normalisation.
+__global__ __launch_bounds__(256) void k_sum_buffers(Loader ld, uint64_t len, float* __restrict__ out,
+                                                     uint64_t Cl = 0, double* __restrict__ craw = nullptr) {
   __shared__ float sq[kNpBuf + kNpBuf / 128];
   const uint64_t off = (uint64_t)blockIdx.x * len;
   if (len % 4 == 0) {
@@ -140,6 +142,22 @@ __global__ __launch_bounds__(256) void k_sum_buffers(Loader ld, uint64_t len, fl
     for (uint64_t i = threadIdx.x; i < len; i += 256) sq[i + (i >> 7)] = ld.q(off + i);
   }
   __syncthreads();
+  if (craw != nullptr && len >= Cl) {
+    // the buffer's chunks (len / Cl of them, at most 32): 8 threads per chunk, strided, then a
+    // shuffle tree; the order does not matter here (the chunk maps verify every use)
+    const int nch = (int)(len / Cl);
+    for (int c = threadIdx.x >> 3; c < nch; c += 32) {
+      double a = 0.0;
+      for (uint64_t i = threadIdx.x & 7; i < Cl; i += 8) {
+        const uint64_t k = (uint64_t)c * Cl + i;
+        a += (double)sq[k + (k >> 7)];
+      }
+      a += __shfl_xor(a, 1, 8);
+      a += __shfl_xor(a, 2, 8);
+      a += __shfl_xor(a, 4, 8);
+      if ((threadIdx.x & 7) == 0) craw[off / Cl + c] = a;
+    }
+  }
   if (threadIdx.x >= 64) return;
   const int t = threadIdx.x;
   if (len < 128) {
@@ -210,47 +228,27 @@ __device__ __forceinline__ double px(const Loader& ld, float T32, uint64_t i) {
   return (double)__fdiv_rn(ld.q(i), T32);
 }
 
-// ---------------------------------------------------------------- approximate chunk sums
-__global__ __launch_bounds__(256) void k_chunk_approx(Loader ld, const SampleHdr* hdr, uint64_t NC,
-                                                      uint64_t Cl, double* __restrict__ capx) {
-  __shared__ double part[256];
-  const float T32 = hdr->T32;
-  const uint64_t c0 = (uint64_t)blockIdx.x * kChunksPerWG;
-  // 8 threads per chunk, each summing a strided slice (order is irrelevant here)
-  const int lc = threadIdx.x >> 3, sub = threadIdx.x & 7;
-  const uint64_t c = c0 + lc;
-  double s = 0.0;
-  if (c < NC) {
-    if (Cl % 4 == 0) {
-      for (uint64_t i = 4 * sub; i < Cl; i += 32) {
-        const float4 v = ld.q4(c * Cl + i);
-        s += (double)__fdiv_rn(v.x, T32) + (double)__fdiv_rn(v.y, T32) + (double)__fdiv_rn(v.z, T32) +
-             (double)__fdiv_rn(v.w, T32);
-      }
-    } else {
-      for (uint64_t i = sub; i < Cl; i += 8) s += px(ld, T32, c * Cl + i);
-    }
-  }
-  part[threadIdx.x] = s;
-  __syncthreads();
-  if (sub == 0 && c < NC) {
-    double t = 0.0;
-    for (int k = 0; k < 8; ++k) t += part[threadIdx.x + k];
-    capx[c] = t;
-  }
-}
-
-// Exclusive scan of approximate chunk sums inside each superchunk; superchunk totals out.
-__global__ __launch_bounds__(256) void k_scan_local(const double* __restrict__ capx, uint64_t SCl,
+// Approximate chunk sums: the FP64 chunk sums of q from k_sum_buffers divided by NumPy's float32
+// total (normalised here, in place).  Against the sum of the float32 quotients p_i the chunks use,
+// the relative error is at most 2^-24 (each quotient is one rounding); the chunk maps take a binade
+// from these only when the chunk lies 2^-22 (relative) inside it, and every map is verified against
+// the exact running sum before use.  Then the exclusive scan inside each superchunk; superchunk
+// totals out.
+__global__ __launch_bounds__(256) void k_scan_local(double* __restrict__ capx, const SampleHdr* hdr, uint64_t SCl,
                                                     double* __restrict__ cin, double* __restrict__ scsum) {
   __shared__ double sh[256];
   const uint64_t base = (uint64_t)blockIdx.x * SCl;
   const int per = (int)((SCl + 255) / 256);
   const int t = threadIdx.x;
+  const double T32 = (double)hdr->T32;
   double loc = 0.0;
   for (int k = 0; k < per; ++k) {
     const uint64_t i = (uint64_t)t * per + k;
-    if (i < SCl) loc += capx[base + i];
+    if (i < SCl) {
+      const double v = capx[base + i] / T32;
+      capx[base + i] = v;
+      loc += v;
+    }
   }
   sh[t] = loc;
   __syncthreads();
@@ -326,7 +324,7 @@ __global__ __launch_bounds__(256) void k_chunk_fn(Loader ld, const SampleHdr* hd
   if (t < 2 * kChunksPerWG && c < NC) {
     const double a = cin[c] + scpre[c / SCl];
     const double b = a + capx[c];
-    const double eps = 0x1p-40;
+    const double eps = 0x1p-22;  // 4x the approximate sums' relative error
     if (a > 0.0) {
       E = binade_of(a);
       const double lo = ldexp_d(1.0, E), hi = ldexp_d(1.0, E + 1);
@@ -902,11 +900,12 @@ int cqs_sample_prepare(const void* amps, const float* probs_in, int32_t n_qubits
   const Loader ld = make_loader(amps, probs_in, n_qubits, is_dm, qubit_axes ? perm : nullptr);
   hipStream_t s = as_stream(stream);
   const uint64_t buflen = l.N / l.NB;
-  hipLaunchKernelGGL(k_sum_buffers, dim3((unsigned)l.NB), dim3(256), 0, s, ld, buflen, csum32);
+  CQS_REQUIRE(buflen >= l.Cl && buflen % l.Cl == 0, "buffer of %llu elements, chunks of %llu",
+              (unsigned long long)buflen, (unsigned long long)l.Cl);
+  hipLaunchKernelGGL(k_sum_buffers, dim3((unsigned)l.NB), dim3(256), 0, s, ld, buflen, csum32, l.Cl, capx);
   hipLaunchKernelGGL(k_total32, dim3(1), dim3(256), 0, s, csum32, l.NB, hdr);
   const unsigned gch = (unsigned)((l.NC + kChunksPerWG - 1) / kChunksPerWG);
-  hipLaunchKernelGGL(k_chunk_approx, dim3(gch), dim3(256), 0, s, ld, (const SampleHdr*)hdr, l.NC, l.Cl, capx);
-  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)l.NSC), dim3(256), 0, s, (const double*)capx, l.SCl, cin,
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)l.NSC), dim3(256), 0, s, capx, (const SampleHdr*)hdr, l.SCl, cin,
                      scsum);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(64), 0, s, (const double*)scsum, l.NSC, scpre);
   hipLaunchKernelGGL(k_chunk_fn, dim3(gch), dim3(256), 0, s, ld, (const SampleHdr*)hdr, l.NC, l.Cl, l.SCl,
