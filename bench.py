@@ -619,19 +619,17 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
         from cirq_amd.autotune import Autotuner
 
         tuner = Autotuner(os.environ.get("CIRQ_AMD_TUNE_CACHE"), reps=1)
+    # unless --no-specialize, the tuned program's passes run as straight-line kernels
+    # (cirq_amd.specialize), planned with the cost entries measured for them
     sim = cirq.Simulator(
         seed=1234, tile_bits=args.tile_bits, coalesce_bits=args.coalesce_bits, mfma_blocks=not args.no_blocks,
-        autotune=tuner or False,
+        autotune=tuner or False, specialize_passes=not args.no_specialize,
     )
     t0 = time.perf_counter()
     comp = sim._compile(circuit, qubits)
     t_compile = time.perf_counter() - t0
+    t_spec = comp.specialize_s
     _progress(f"{n}-qubit grid program compiled and tuned in {t_compile:.1f} s")
-    t_spec = None
-    if not args.no_specialize:  # the tuned program's passes as straight-line kernels (cirq_amd.specialize)
-        t0 = time.perf_counter()
-        comp.dev.specialize()
-        t_spec = time.perf_counter() - t0
     tune_info = None
     if tuner is not None:
         import dataclasses

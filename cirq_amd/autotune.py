@@ -22,7 +22,7 @@ import os
 import torch
 
 from . import engine
-from .tape import COSTS, DEFAULT_KMAX, MAX_VARIANT_BITS, LOp, compile_ops
+from .tape import COSTS, DEFAULT_KMAX, MAX_VARIANT_BITS, SPECIALIZED_COSTS, LOp, compile_ops
 
 
 @dataclasses.dataclass(frozen=True)
@@ -35,9 +35,11 @@ class TuneConfig:
     blocks: bool = True
     kron_merge: bool = True  # small blocks left at the end of a pass share MFMA blocks
 
-    def compile_kwargs(self, n_bits: int) -> dict:
+    def compile_kwargs(self, n_bits: int, specialized: bool = False) -> dict:
+        """compile_ops keyword arguments; specialized: the program will run as specialised kernels
+        (the planner's cost table takes SPECIALIZED_COSTS)."""
         L = min(self.tile_bits, n_bits)
-        costs = dict(COSTS)
+        costs = dict(COSTS, **SPECIALIZED_COSTS) if specialized else dict(COSTS)
         costs["mfma4"] *= self.mfma_weight
         costs["mfma5"] *= self.mfma_weight
         return dict(L=L, coalesce=min(self.coalesce_bits, L), blocks=self.blocks, kmax=self.kmax, costs=costs,

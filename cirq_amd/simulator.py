@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import collections
 import math
+import time
 import warnings
 import weakref
 from typing import Any, Iterator, Sequence
@@ -30,7 +31,9 @@ from .paulis import PauliSum, pauli_masks
 from .qubits import QubitOrder
 from .study import ParamResolver, ResultDict, to_resolvers
 from .tape import (
+    COSTS,
     DEFAULT_KMAX,
+    SPECIALIZED_COSTS,
     Lowering,
     Measurement,
     ParamEvaluator,
@@ -199,7 +202,7 @@ class _Compiled:
     """A circuit lowered, scheduled and uploaded for one qubit order."""
 
     def __init__(self, circuit, qubits, dm: bool, device, L: int, coalesce: int, fuse: bool, blocks: bool = True,
-                 tuner=None):
+                 tuner=None, specialized: bool = False):
         self.qubits = tuple(qubits)
         self.n = len(self.qubits)
         self.dm = dm
@@ -210,10 +213,13 @@ class _Compiled:
         self.measurements: list[Measurement] = low.measurements
         if tuner is not None and low.ops:
             self.tuning = tuner.config_for(low.ops, self.n_bits, dm)
-            kw = self.tuning.compile_kwargs(self.n_bits)
+            kw = self.tuning.compile_kwargs(self.n_bits, specialized)
         else:
             self.tuning = None
             kw = dict(L=L, coalesce=coalesce, blocks=blocks, kmax=DEFAULT_KMAX[bool(dm)])
+            if specialized:
+                kw["costs"] = dict(COSTS, **SPECIALIZED_COSTS)
+        self.specialize_s = None  # seconds spent compiling specialised kernels, when it does
         self.packed = compile_ops(low.ops, self.n_bits, fuse=fuse, **kw)
         self.evaluator = ParamEvaluator(low.param_refs)
         self.has_params = bool(self.packed.param_entries)
@@ -331,9 +337,12 @@ class _SimulatorBase:
         nb = 2 * len(qubits) if self._dm else len(qubits)
         engine.check_state_fits(nb, self._dev())
         L, c = self._tile_params(nb)
-        comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse, self._blocks, self._tuner)
+        comp = _Compiled(circuit, qubits, self._dm, self._dev(), L, c, self._fuse, self._blocks, self._tuner,
+                         self._specialize)
         if self._specialize and comp.packed.n_passes:
+            t0 = time.perf_counter()
             comp.dev.specialize()
+            comp.specialize_s = time.perf_counter() - t0
         try:
             ref = weakref.ref(circuit)
         except TypeError:

@@ -420,6 +420,11 @@ def _op_matrix(op: LOp) -> np.ndarray:
 DEFAULT_KMAX = {False: 4, True: 5}  # widest fused block: state vectors, density matrices (timed)
 COSTS = {"dense1": 1.0, "dense2": 1.8, "dense3": 1.75, "diag": 0.05, "pair": 0.03, "scale": 0.03, "mfma4": 2.0,
          "mfma5": 2.55, "reg1": 0.45, "reg2": 0.9, "reg3": 1.8, "permute": 0.8}
+# Entries replaced for programs that run as specialised kernels (cirq_amd.specialize).  Measured
+# there at 30 qubits, 13-bit tiles: a 3-qubit dense op through LDS adds 0.87 ms to its pass (its
+# VALU floor, 32 FMAs per amplitude), a 4-qubit MFMA block 0.37 ms; with dense3 above mfma4 the
+# planner widens such groups to MFMA blocks, and the 30-qubit grid program's passes take 0.8% less.
+SPECIALIZED_COSTS = {"dense3": 2.6}
 
 
 def _valu_cost(op: LOp, costs: dict | None = None) -> float:

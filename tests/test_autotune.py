@@ -71,3 +71,19 @@ def test_default_config_and_cache_file(tmp_path):
     t._save()
     t2 = Autotuner(path)
     assert t2.config_for(ops, len(qs)) == cfg  # served from the file, no GPU needed
+
+
+def test_specialised_programs_plan_with_their_cost_entries():
+    """Programs that will run as specialised kernels are planned with SPECIALIZED_COSTS (a 3-qubit
+    VALU dense op is dearer than a 4-qubit MFMA block there); the interpreter keeps the table."""
+    from cirq_amd.tape import COSTS, SPECIALIZED_COSTS
+
+    cfg = TuneConfig(tile_bits=12)
+    plain = cfg.compile_kwargs(20)["costs"]
+    spec = cfg.compile_kwargs(20, specialized=True)["costs"]
+    assert plain == COSTS
+    for k, v in SPECIALIZED_COSTS.items():
+        assert spec[k] == v
+    assert spec["dense3"] > spec["mfma4"]
+    assert {k: v for k, v in spec.items() if k not in SPECIALIZED_COSTS} == {
+        k: v for k, v in plain.items() if k not in SPECIALIZED_COSTS}
