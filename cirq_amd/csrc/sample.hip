@@ -42,7 +42,7 @@ struct SampleHdr {
 struct Layout {
   uint64_t N, NC, Cl, NSC, SCl, NB;
   size_t o_hdr, o_csum32, o_capx, o_cin, o_cfn, o_scsum, o_scpre, o_scfn, o_scstart, o_scmode,
-      o_cstart, bytes;
+      o_cstart, o_t32sum, o_t32map, bytes;
 };
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
@@ -67,6 +67,9 @@ static Layout layout_for(int n) {
   l.o_scstart = o; o += al(l.NSC * sizeof(double));
   l.o_scmode = o; o += al(l.NSC * sizeof(int));
   l.o_cstart = o; o += al(l.NC * sizeof(double));
+  const uint64_t nch32 = (l.NB + 63) / 64;  // chunks of the float32 total's exact walk
+  l.o_t32sum = o; o += al(nch32 * sizeof(double));
+  l.o_t32map = o; o += al(nch32 * 16);
   l.bytes = o;
   return l;
 }
@@ -272,10 +275,29 @@ __global__ __launch_bounds__(256) void k_scan_local(double* __restrict__ capx, c
   }
 }
 
-__global__ void k_scan_top(const double* __restrict__ scsum, uint64_t NSC, double* __restrict__ scpre) {
-  if (threadIdx.x != 0) return;
-  double run = 0.0;
-  for (uint64_t i = 0; i < NSC; ++i) {
+// Exclusive prefix of the superchunk sums (approximate values: any summation order will do), one
+// workgroup, each thread over a contiguous run.
+__global__ __launch_bounds__(1024) void k_scan_top(const double* __restrict__ scsum, uint64_t NSC,
+                                                   double* __restrict__ scpre) {
+  __shared__ double tot[1024];
+  const int t = threadIdx.x;
+  const uint64_t per = (NSC + 1023) / 1024;
+  const uint64_t b = (uint64_t)t * per, e = b + per < NSC ? b + per : NSC;
+  double a = 0.0;
+  for (uint64_t i = b; i < e; ++i) a += scsum[i];
+  tot[t] = a;
+  __syncthreads();
+  if (t == 0) {
+    double run = 0.0;
+    for (int k = 0; k < 1024; ++k) {
+      const double v = tot[k];
+      tot[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  double run = tot[t];
+  for (uint64_t i = b; i < e; ++i) {
     scpre[i] = run;
     run += scsum[i];
   }
@@ -502,6 +524,156 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
     hdr->total = S;
     hdr->n_seq_chunks = nseq;
   }
+}
+
+// ---------------------------------------------------------------- NumPy's float32 total, many buffers
+// The same exact-walk idea in float32 for the sequential float32 additions of the buffer sums
+// (k_total32 adds them one at a time in one thread): a float32 in [2^E, 2^(E+1)) (E >= -126) is a
+// multiple of u = 2^(E-23), and rounding s + x depends on s only through the parity of s/u.  The
+// buffer sums are cut into chunks of kT32Chunk; a chunk whose approximate start and end (FP64 prefix
+// of FP64 chunk sums) lie well inside one binade gets a map (ulps added for each incoming parity),
+// computed by adding its elements to 2^E + parity*u in float32; one wave walks the maps up to 64 at
+// a time (the float32 twin of walk_maps) and adds the other chunks element by element.
+constexpr int kT32Chunk = 64;
+constexpr int kT32MinBuffers = 1024;  // below this the one-thread loop is as fast
+constexpr int kT32Stage = 1024;       // maps staged in LDS per round of the walk
+struct Map32 {
+  int d0, d1;  // ulps for incoming parity 0 / 1
+  int E;       // binade exponent
+  int ok;      // 1 = usable map
+};
+static_assert(sizeof(Map32) == 16, "the workspace layout reserves 16 bytes per map");
+
+__global__ __launch_bounds__(256) void k_t32_chunk_sums(const float* __restrict__ bsum, uint64_t nb,
+                                                        double* __restrict__ csum) {
+  const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t nch = (nb + kT32Chunk - 1) / kT32Chunk;
+  if (c >= nch) return;
+  const uint64_t e = (c + 1) * kT32Chunk < nb ? (c + 1) * kT32Chunk : nb;
+  double a = 0.0;
+  for (uint64_t i = c * kT32Chunk; i < e; ++i) a += (double)bsum[i];
+  csum[c] = a;
+}
+
+// Exclusive prefix of the chunk sums in place (one workgroup; each thread scans a contiguous run).
+__global__ __launch_bounds__(1024) void k_t32_prefix(double* __restrict__ csum, uint64_t nch) {
+  __shared__ double tot[1024];
+  const int t = threadIdx.x;
+  const uint64_t per = (nch + 1023) / 1024;
+  const uint64_t b = (uint64_t)t * per, e = b + per < nch ? b + per : nch;
+  double a = 0.0;
+  for (uint64_t i = b; i < e; ++i) a += csum[i];
+  tot[t] = a;
+  __syncthreads();
+  if (t == 0) {
+    double run = 0.0;
+    for (int k = 0; k < 1024; ++k) {
+      const double v = tot[k];
+      tot[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  double run = tot[t];
+  for (uint64_t i = b; i < e; ++i) {
+    const double v = csum[i];
+    csum[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_t32_maps(const float* __restrict__ bsum, uint64_t nb,
+                                                  const double* __restrict__ cpre, Map32* __restrict__ maps) {
+  const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t nch = (nb + kT32Chunk - 1) / kT32Chunk;
+  if (c >= nch) return;
+  const uint64_t b0 = c * kT32Chunk, e = b0 + kT32Chunk < nb ? b0 + kT32Chunk : nb;
+  const double a = cpre[c];
+  const double b = c + 1 < nch ? cpre[c + 1] : a;  // the next chunk's start (the last chunk: unused)
+  Map32 m = {0, 0, 0, 0};
+  if (a > 0.0 && c + 1 < nch) {
+    int ex;
+    frexp(a, &ex);
+    const int E = ex - 1;
+    // The FP64 prefix tracks the exact sum, not the float32 running sum (which drifts from it by
+    // float32 roundings): a wrong binade only makes the map unusable at its verification.
+    const double eps = 0x1p-30;
+    if (E >= -126 && E <= 126 && a * (1.0 - eps) >= ldexp(1.0, E) && b * (1.0 + eps) < ldexp(1.0, E + 1)) {
+      const float lo = ldexpf(1.f, E), hi = ldexpf(1.f, E + 1), u = ldexpf(1.f, E - 23);
+      int d[2], over = 0;
+      for (int p = 0; p < 2; ++p) {
+        const float s0 = lo + (p ? u : 0.f);
+        float s = s0;
+        for (uint64_t i = b0; i < e; ++i) s = s + bsum[i];
+        over |= !(s < hi);
+        d[p] = __float_as_int(s) - __float_as_int(s0);
+      }
+      m.d0 = d[0];
+      m.d1 = d[1];
+      m.E = E;
+      m.ok = !over;
+    }
+  }
+  maps[c] = m;
+}
+
+// walk_maps in float32: applies up to `avail` (<= 64) maps to S with one wave; returns how many.
+__device__ __forceinline__ int walk_maps32(float& S, const Map32* maps, int avail) {
+  constexpr int kExp = 0x7f800000;
+  const int lane = threadIdx.x & 63;
+  long long d0 = 0, d1 = 0;
+  int E = 0, ok = 0;
+  if (lane < avail) {
+    const Map32 f = maps[lane];
+    d0 = f.d0;
+    d1 = f.d1;
+    E = f.E;
+    ok = f.ok;
+  }
+  const int bits = __float_as_int(S);
+  const int ebits = bits & kExp;
+  const bool usable = S > 0.f && lane < avail && ok && ((E + 127) << 23) == ebits;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    long long p0 = __shfl_up(d0, o, 64), p1 = __shfl_up(d1, o, 64);
+    if (lane >= o) {
+      compose(p0, p1, d0, d1);
+      d0 = p0;
+      d1 = p1;
+    }
+  }
+  const long long r = (long long)bits + ((bits & 1) ? d1 : d0);
+  const unsigned long long stays = __ballot(usable && r < 0x7f800000ll && ((int)r & kExp) == ebits);
+  const int n = ~stays ? __builtin_ctzll(~stays) : 64;
+  if (n > 0) S = __int_as_float((int)__shfl(r, n - 1, 64));
+  return n;
+}
+
+__global__ __launch_bounds__(64) void k_t32_walk(const float* __restrict__ bsum, uint64_t nb,
+                                                 const Map32* __restrict__ maps, SampleHdr* hdr) {
+  __shared__ Map32 sm[kT32Stage];
+  const int t = threadIdx.x;
+  const uint64_t nch = (nb + kT32Chunk - 1) / kT32Chunk;
+  float S = 0.f;
+  for (uint64_t s0 = 0; s0 < nch; s0 += kT32Stage) {
+    const int ns = (int)(nch - s0 < (uint64_t)kT32Stage ? nch - s0 : (uint64_t)kT32Stage);
+    __syncthreads();
+    for (int i = t; i < ns; i += 64) sm[i] = maps[s0 + i];
+    __syncthreads();
+    for (int k = 0; k < ns;) {
+      const int adv = walk_maps32(S, sm + k, ns - k < 64 ? ns - k : 64);
+      k += adv;
+      if (adv == 64 || k >= ns) continue;
+      // chunk s0 + k added element by element: one load per lane, then the lanes' values in order
+      // (every lane adds the same values, so S stays uniform)
+      const uint64_t i0 = (s0 + k) * kT32Chunk;
+      const int cnt = (int)(nb - i0 < (uint64_t)kT32Chunk ? nb - i0 : (uint64_t)kT32Chunk);
+      const int v = t < cnt ? __float_as_int(bsum[i0 + t]) : 0;
+      for (int i = 0; i < cnt; ++i) S = S + __int_as_float(__builtin_amdgcn_readlane(v, i));
+      ++k;
+    }
+  }
+  if (t == 0) hdr->T32 = S;
 }
 
 // Chunk start values inside superchunks that were advanced by one composed map.
@@ -903,11 +1075,23 @@ int cqs_sample_prepare(const void* amps, const float* probs_in, int32_t n_qubits
   CQS_REQUIRE(buflen >= l.Cl && buflen % l.Cl == 0, "buffer of %llu elements, chunks of %llu",
               (unsigned long long)buflen, (unsigned long long)l.Cl);
   hipLaunchKernelGGL(k_sum_buffers, dim3((unsigned)l.NB), dim3(256), 0, s, ld, buflen, csum32, l.Cl, capx);
-  hipLaunchKernelGGL(k_total32, dim3(1), dim3(256), 0, s, csum32, l.NB, hdr);
+  if (l.NB >= (uint64_t)kT32MinBuffers) {
+    const uint64_t nch = (l.NB + kT32Chunk - 1) / kT32Chunk;
+    double* t32sum = (double*)(ws + l.o_t32sum);
+    Map32* t32map = (Map32*)(ws + l.o_t32map);
+    const unsigned g = (unsigned)((nch + 255) / 256);
+    hipLaunchKernelGGL(k_t32_chunk_sums, dim3(g), dim3(256), 0, s, (const float*)csum32, l.NB, t32sum);
+    hipLaunchKernelGGL(k_t32_prefix, dim3(1), dim3(1024), 0, s, t32sum, nch);
+    hipLaunchKernelGGL(k_t32_maps, dim3(g), dim3(256), 0, s, (const float*)csum32, l.NB, (const double*)t32sum,
+                       t32map);
+    hipLaunchKernelGGL(k_t32_walk, dim3(1), dim3(64), 0, s, (const float*)csum32, l.NB, (const Map32*)t32map, hdr);
+  } else {
+    hipLaunchKernelGGL(k_total32, dim3(1), dim3(256), 0, s, csum32, l.NB, hdr);
+  }
   const unsigned gch = (unsigned)((l.NC + kChunksPerWG - 1) / kChunksPerWG);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)l.NSC), dim3(256), 0, s, capx, (const SampleHdr*)hdr, l.SCl, cin,
                      scsum);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(64), 0, s, (const double*)scsum, l.NSC, scpre);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, (const double*)scsum, l.NSC, scpre);
   hipLaunchKernelGGL(k_chunk_fn, dim3(gch), dim3(256), 0, s, ld, (const SampleHdr*)hdr, l.NC, l.Cl, l.SCl,
                      (const double*)capx, (const double*)cin, (const double*)scpre, cfn);
   hipLaunchKernelGGL(k_super_fn, dim3((unsigned)l.NSC), dim3(256), 0, s, (const ChunkFn*)cfn, l.SCl, scfn);
