@@ -120,6 +120,41 @@ struct Loader {
     }
     return make_float4(q(i), q(i + 1), q(i + 2), q(i + 3));
   }
+  // Uniform: q4 at every multiple of 4 is plain 16-byte loads (no permutation, aligned base).
+  __device__ __forceinline__ bool vec4() const {
+    if (permute) return false;
+    if (mode == 2) return (reinterpret_cast<uintptr_t>(probs) & 15) == 0;
+    if (mode == 0) return (reinterpret_cast<uintptr_t>(amps) & 15) == 0;
+    return false;
+  }
+  // R consecutive groups of 4 at i0, i0 + stride, ... (vec4() holds): every load issued before
+  // any is used, so their latencies overlap.
+  template <int R>
+  __device__ __forceinline__ void q4_batch(uint64_t i0, uint64_t stride, float4 (&out)[R]) const {
+    if (mode == 2) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) out[r] = *reinterpret_cast<const float4*>(probs + i0 + r * stride);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float4 v = out[r];
+        out[r] = make_float4(v.x < 0.f ? 0.f : v.x, v.y < 0.f ? 0.f : v.y, v.z < 0.f ? 0.f : v.z,
+                             v.w < 0.f ? 0.f : v.w);
+      }
+    } else {
+      float4 a[R], b[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        a[r] = *reinterpret_cast<const float4*>(amps + i0 + r * stride);
+        b[r] = *reinterpret_cast<const float4*>(amps + i0 + r * stride + 2);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float p0 = fmaf(a[r].x, a[r].x, a[r].y * a[r].y), p1 = fmaf(a[r].z, a[r].z, a[r].w * a[r].w);
+        const float p2 = fmaf(b[r].x, b[r].x, b[r].y * b[r].y), p3 = fmaf(b[r].z, b[r].z, b[r].w * b[r].w);
+        out[r] = make_float4(p0 < 0.f ? 0.f : p0, p1 < 0.f ? 0.f : p1, p2 < 0.f ? 0.f : p2, p3 < 0.f ? 0.f : p3);
+      }
+    }
+  }
 };
 
 __device__ __forceinline__ double ldexp_d(double x, int e) { return ldexp(x, e); }
@@ -132,7 +167,20 @@ __global__ __launch_bounds__(256) void k_sum_buffers(Loader ld, uint64_t len, fl
                                                      uint64_t Cl = 0, double* __restrict__ craw = nullptr) {
   __shared__ float sq[kNpBuf + kNpBuf / 128];
   const uint64_t off = (uint64_t)blockIdx.x * len;
-  if (len % 4 == 0) {
+  if (len == (uint64_t)kNpBuf && ld.vec4()) {  // full buffers: 8 groups of 4 per thread, loads batched
+    constexpr int R = kNpBuf / 1024;
+    float4 v[R];
+    ld.q4_batch<R>(off + 4 * threadIdx.x, 1024, v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t i = 4 * threadIdx.x + (uint64_t)r * 1024;
+      float* d = sq + i + (i >> 7);
+      d[0] = v[r].x;
+      d[1] = v[r].y;
+      d[2] = v[r].z;
+      d[3] = v[r].w;
+    }
+  } else if (len % 4 == 0) {
     for (uint64_t i = 4 * threadIdx.x; i < len; i += 1024) {
       const float4 v = ld.q4(off + i);  // i .. i + 3 lie in one 128-element block
       float* d = sq + i + (i >> 7);
@@ -319,7 +367,20 @@ __global__ __launch_bounds__(256) void k_chunk_fn(Loader ld, const SampleHdr* hd
   const float T32 = hdr->T32;
   const uint64_t c0 = (uint64_t)blockIdx.x * kChunksPerWG;
   const uint64_t nloc = (NC - c0) < (uint64_t)kChunksPerWG ? (NC - c0) : (uint64_t)kChunksPerWG;
-  if (Cl % 4 == 0) {
+  if (Cl == (uint64_t)kChunk && nloc == (uint64_t)kChunksPerWG && ld.vec4()) {  // loads batched
+    constexpr int R = kChunksPerWG * kChunk / 1024;
+    float4 v[R];
+    ld.q4_batch<R>(c0 * Cl + 4 * threadIdx.x, 1024, v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t i = 4 * threadIdx.x + (uint64_t)r * 1024;
+      float* d = sp + (i / kChunk) * (kChunk + 1) + (i % kChunk);
+      d[0] = __fdiv_rn(v[r].x, T32);
+      d[1] = __fdiv_rn(v[r].y, T32);
+      d[2] = __fdiv_rn(v[r].z, T32);
+      d[3] = __fdiv_rn(v[r].w, T32);
+    }
+  } else if (Cl % 4 == 0) {
     for (uint64_t i = 4 * threadIdx.x; i < nloc * Cl; i += 1024) {
       const uint64_t lc = i / Cl, k = i % Cl;  // k .. k + 3 lie in one chunk
       const float4 v = ld.q4(c0 * Cl + i);
