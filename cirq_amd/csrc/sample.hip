@@ -550,7 +550,11 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
   for (uint64_t s0 = 0; s0 < NSC; s0 += kStage) {
     const int ns = (int)(NSC - s0 < (uint64_t)kStage ? NSC - s0 : (uint64_t)kStage);
     __syncthreads();
-    for (int i = t; i < ns; i += 64) sf[i] = scfn[s0 + i];
+#pragma unroll  // every lane's loads issued before any LDS store (one latency, not eight)
+    for (int r = 0; r < kStage / 64; ++r) {
+      const int i = t + 64 * r;
+      if (i < ns) sf[i] = scfn[s0 + i];
+    }
     __syncthreads();
     for (int k = 0; k < ns;) {
       double before;
@@ -562,7 +566,11 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
       const uint64_t sc = s0 + k;  // walked by its chunk maps
       if (t == 0) scmode[sc] = 0;
       __syncthreads();
-      for (int i = t; i < ncs; i += 64) cf[i] = cfn[sc * SCl + i];
+#pragma unroll
+      for (int r = 0; r < kSuper / 64; ++r) {
+        const int i = t + 64 * r;
+        if (i < ncs) cf[i] = cfn[sc * SCl + i];
+      }
       __syncthreads();
       for (int c = 0; c < ncs;) {
         double cb;
@@ -572,7 +580,11 @@ __global__ __launch_bounds__(64) void k_scan_serial(Loader ld, SampleHdr* hdr, u
         if (a == 64 || c >= ncs) continue;
         const uint64_t ch = sc * SCl + c;  // summed element by element
         __syncthreads();
-        for (uint64_t i = t; i < Cl; i += 64) xs[i] = px(ld, T32, ch * Cl + i);
+#pragma unroll
+        for (int r = 0; r < kChunk / 64; ++r) {
+          const uint64_t i = t + 64 * r;
+          if (i < Cl) xs[i] = px(ld, T32, ch * Cl + i);
+        }
         __syncthreads();
         for (uint64_t i = 0; i < Cl; ++i) S = S + xs[i];
         ++nseq;
