@@ -963,7 +963,7 @@ __global__ __launch_bounds__(256) void k_marginal(Loader ld, int n, AxisBits ab,
 // out[j] = p(map(j)) where output bit b is input flat bit perm[b] and p = |amp|^2 as NumPy rounds
 // it (state vector) or Re rho[k, k] (density matrix).  One workgroup per tile of 2^T output
 // indices over the output bits S[0..T) (ascending); loads walk the tile with the bits that feed
-// input bits 0..4 lowest (contiguous reads), stores with output bits 0..4 lowest (contiguous
+// input bits 0..5 lowest (contiguous reads), stores with output bits 0..5 lowest (contiguous
 // writes), and LDS holds the tile in between.
 struct PermTile {
   int n, T, nouter, is_dm;
@@ -974,6 +974,7 @@ struct PermTile {
 };
 
 constexpr int kPermT = 12;
+constexpr int kPermRun = 6;  // contiguous run bits on each side (2 x 6 <= kPermT)
 
 __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ amps, float* __restrict__ out,
                                                     PermTile pt) {
@@ -1211,8 +1212,8 @@ int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
   pt.is_dm = is_dm ? 1 : 0;
   for (int b = 0; b < n; ++b) pt.perm[b] = perm[b];
   const int T = n < kPermT ? n : kPermT;
-  // tile bits: output bits 0..4, the output bits feeding input bits 0..4, then the lowest others
-  // (loads and stores in runs of at least 32 elements)
+  // tile bits: output bits 0..5, the output bits feeding input bits 0..5, then the lowest others
+  // (loads and stores in runs of at least 64 elements; 12 tile bits hold both sets)
   bool in_s[64] = {false};
   int inv[64];
   for (int b = 0; b < n; ++b) inv[perm[b]] = b;
@@ -1223,8 +1224,8 @@ int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
       ++cnt;
     }
   };
-  for (int b = 0; b < 5 && b < n; ++b) take(b);
-  for (int b = 0; b < 5 && b < n; ++b) take(inv[b]);
+  for (int b = 0; b < kPermRun && b < n; ++b) take(b);
+  for (int b = 0; b < kPermRun && b < n; ++b) take(inv[b]);
   for (int b = 0; b < n; ++b) take(b);
   int k = 0;
   int tile_of[64];
@@ -1239,7 +1240,7 @@ int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
   k = 0;
   for (int q = 0; q < n && k < T; ++q) {
     const int b = inv[q];
-    if (in_s[b] && !used[tile_of[b]] && q < 5) {
+    if (in_s[b] && !used[tile_of[b]] && q < kPermRun) {
       used[tile_of[b]] = true;
       pt.A[k++] = tile_of[b];
     }
