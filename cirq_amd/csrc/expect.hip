@@ -4,6 +4,8 @@
 #include <numeric>
 #include <vector>
 
+#include <string.h>
+
 #include "common.hpp"
 
 namespace cqs {
@@ -137,41 +139,45 @@ __global__ __launch_bounds__(64) void k_expect_final(const double* __restrict__ 
 // Terms without X or Y (x mask 0): sum_i (-1)^popcount(i & z) |psi_i|^2, up to 64 terms per read
 // of the state.  Each thread takes 16 consecutive amplitudes i = 16g + j, so the sign splits into
 // (-1)^popcount(g & z>>4), the same for the 16, times (-1)^popcount(j & z&15), the same for every
-// thread: one FP64 FMA per amplitude and distinct low mask, then one add per term.  |psi_i|^2 is
-// exact in FP64 up to its final rounding.  Which thread sums which groups does not depend on the
-// batch, so every term's reduction order is fixed.
-constexpr int kZTerms = 64;       // terms per launch (FP64 accumulators in registers)
+// thread: one FP64 FMA per amplitude and distinct low mask.  A thread's groups are
+// g = lo + k * stride (lo < stride, stride = 2^sbits the grid's threads), so the group sign is
+// parity(lo & z>>4), fixed per thread, times parity(k * stride & z>>4), which depends only on the
+// z bits at 4 + sbits and above.  Terms that agree in their low 4 bits and in those high bits form
+// one class: a thread accumulates one FP64 sum per class (lane j evaluates the iteration sign of
+// class j and a ballot shares it) and gives each term its class sum with the term's fixed sign
+// (negation is exact, so every term's sum is the same as with the sign applied per group).
+// |psi_i|^2 is exact in FP64 up to its final rounding.  Which thread sums which groups does not
+// depend on the batch, so every term's reduction order is fixed.
+constexpr int kZTerms = 64;       // terms per launch
 constexpr int kZBatch = kZTerms;
+constexpr int kZClasses = 24;     // classes per launch (FP64 accumulators in registers)
 constexpr int kZBlocks = 2048;    // fixed grid => deterministic reduction order
 struct ZBatch {
-  int count;
-  int pad;
-  uint64_t newrun;  // bit t: term t's low 4 z bits differ from term t - 1's (bit 0 set)
-  uint64_t z[kZBatch];
+  int count;                      // terms
+  int nclass;                     // classes
+  uint64_t newrun;                // bit j: class j's low 4 bits differ from class j - 1's (bit 0 set)
+  uint64_t z[kZBatch];            // the terms' z masks
+  uint64_t zc[kZClasses];         // a z mask of each class (its low 4 bits and high bits)
+  uint8_t cls[kZBatch];           // term -> class
 };
 
 __global__ __launch_bounds__(256) void k_expect_z(const float2* __restrict__ amps, int n, ZBatch zb,
                                                   double* __restrict__ partials) {
   __shared__ double red[4][kZTerms];
-  __shared__ int zls[kZTerms];  // low 4 z bits per term, read inside the loop (not kept in registers)
+  __shared__ int zls[kZClasses];  // low 4 z bits per class, read inside the loop (not kept in registers)
   const int tid = threadIdx.x;
-  const int cnt = zb.count;
-  if (tid < kZTerms) zls[tid] = tid < cnt ? (int)(zb.z[tid] & 15) : 0;
+  const int cnt = zb.count, ncl = zb.nclass;
+  if (tid < kZClasses) zls[tid] = tid < ncl ? (int)(zb.zc[tid] & 15) : 0;
   __syncthreads();
-  double acc[kZTerms];
+  double acc[kZClasses];
 #pragma unroll
-  for (int t = 0; t < kZTerms; ++t) acc[t] = 0.0;
+  for (int j = 0; j < kZClasses; ++j) acc[j] = 0.0;
   const uint64_t G = 1ull << (n - 4);
   const uint64_t stride = (uint64_t)gridDim.x * 256;  // 2^19 whenever G >= 2^19 (full grid)
   const uint64_t lo = (uint64_t)blockIdx.x * 256 + tid;
-  // The group sign of term t at g = lo + k * stride is parity(lo & z_t>>4), fixed per thread and
-  // applied once after the loop (negation is exact, so the sum is the same), times
-  // parity(k * stride & z_t>>4), the same for every thread of an iteration: lane t evaluates it
-  // for term t and a ballot hands the bits to the whole wave.  Every thread runs the same
-  // iterations, so the ballot sees all lanes.
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const uint64_t zh_lane = lane < cnt ? zb.z[lane] >> 4 : 0;
-  const uint64_t iters = G > stride ? G / stride : 1;
+  const uint64_t zh_lane = lane < ncl ? zb.zc[lane] >> 4 : 0;
+  const uint64_t iters = G > stride ? G / stride : 1;  // every thread runs them all: the ballot sees all lanes
   for (uint64_t k = 0; k < iters; ++k) {
     const uint64_t kpar = __ballot(__popcll((k * stride) & zh_lane) & 1);
     const uint64_t g = lo + k * stride;
@@ -184,30 +190,32 @@ __global__ __launch_bounds__(256) void k_expect_z(const float2* __restrict__ amp
       p[2 * q] = fma((double)v.x, (double)v.x, (double)v.y * (double)v.y);
       p[2 * q + 1] = fma((double)v.z, (double)v.z, (double)v.w * (double)v.w);
     }
-    // The host orders a batch's terms by their low 4 z bits, so the signed 16-term sum is formed
-    // once per run of equal low masks (a uniform branch) and shared by the run's terms.  The run's
-    // mask is read from LDS there (a barrier to code motion keeps 64 masks out of the registers).
+    // The host orders classes by their low 4 z bits, so the signed 16-term sum is formed once per
+    // run of equal low masks (a uniform branch) and shared by the run's classes.  The run's mask is
+    // read from LDS there (a barrier to code motion keeps the masks out of the registers).
     asm volatile("" ::: "memory");
     double w = 0.0;
 #pragma unroll
-    for (int t = 0; t < kZTerms; ++t) {
-      if (t < cnt) {
-        if ((zb.newrun >> t) & 1) {
-          const int zl = __builtin_amdgcn_readfirstlane(zls[t]);
+    for (int j = 0; j < kZClasses; ++j) {
+      if (j < ncl) {
+        if ((zb.newrun >> j) & 1) {
+          const int zl = __builtin_amdgcn_readfirstlane(zls[j]);
           w = 0.0;
 #pragma unroll
-          for (int j = 0; j < 16; ++j) w = fma((__popc(j & zl) & 1) ? -1.0 : 1.0, p[j], w);
+          for (int q = 0; q < 16; ++q) w = fma((__popc(q & zl) & 1) ? -1.0 : 1.0, p[q], w);
         }
-        acc[t] += ((kpar >> t) & 1) ? -w : w;
+        acc[j] += ((kpar >> j) & 1) ? -w : w;
       }
     }
   }
+  // each term: its class's sum with the term's fixed sign parity(lo & z>>4)
+  for (int t = 0; t < cnt; ++t) {
+    const int c = zb.cls[t];
+    double v = acc[0];
 #pragma unroll
-  for (int t = 0; t < kZTerms; ++t) {
-    if (t < cnt) {
-      const double sv = wave_sum((__popcll(lo & (zb.z[t] >> 4)) & 1) ? -acc[t] : acc[t]);
-      if (lane == 0) red[wv][t] = sv;
-    }
+    for (int j = 1; j < kZClasses; ++j) v = c == j ? acc[j] : v;
+    const double sv = wave_sum((__popcll(lo & (zb.z[t] >> 4)) & 1) ? -v : v);
+    if (lane == 0) red[wv][t] = sv;
   }
   __syncthreads();
   if (tid < cnt) partials[(size_t)blockIdx.x * kZBatch + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
@@ -295,12 +303,25 @@ static int expect_batches(const void* amps, int32_t n_qubits, int32_t is_dm, int
     if (!is_dm && n_qubits >= 4 && xmask[term(i)] == 0) {
       ZBatch zb;
       ZFinal zf;
-      zb.count = 0;
-      zb.pad = 0;
-      zb.newrun = 0;
+      memset(&zb, 0, sizeof(zb));
+      // class key: the low 4 z bits and the z bits that the iteration index reaches (from bit
+      // 4 + sbits up; none when the grid covers every group in one iteration)
+      const uint64_t G = N >> 4, stride = (uint64_t)zgrid * 256;
+      const int sbits = __builtin_ctzll(stride);
+      const uint64_t hmask = G > stride ? ~0ull << (4 + sbits) : 0;
+      auto key = [&](uint64_t z) { return (z & 15) | (z & hmask); };
       while (i < n_terms && zb.count < kZBatch && xmask[term(i)] == 0) {
         const int t = term(i);
-        if (zb.count == 0 || (zmask[t] & 15) != (zb.z[zb.count - 1] & 15)) zb.newrun |= 1ull << zb.count;
+        const uint64_t kz = key(zmask[t]);
+        int c = 0;
+        while (c < zb.nclass && key(zb.zc[c]) != kz) ++c;
+        if (c == zb.nclass) {
+          if (zb.nclass == kZClasses) break;  // the next launch takes it
+          if (c == 0 || (kz & 15) != (zb.zc[c - 1] & 15)) zb.newrun |= 1ull << c;
+          zb.zc[c] = zmask[t];
+          ++zb.nclass;
+        }
+        zb.cls[zb.count] = (uint8_t)c;
         zb.z[zb.count] = zmask[t];
         zf.out_idx[zb.count] = t;
         zf.cr[zb.count] = coef[2 * t];
@@ -308,7 +329,6 @@ static int expect_batches(const void* amps, int32_t n_qubits, int32_t is_dm, int
         ++zb.count;
         ++i;
       }
-      for (int k = zb.count; k < kZBatch; ++k) zb.z[k] = 0;
       zf.count = zb.count;
       zf.nblocks = zgrid;
       hipLaunchKernelGGL(k_expect_z, dim3(zgrid), dim3(256), 0, s, (const float2*)amps, (int)n_qubits, zb,
