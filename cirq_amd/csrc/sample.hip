@@ -968,9 +968,10 @@ __global__ __launch_bounds__(256) void k_marginal(Loader ld, int n, AxisBits ab,
 struct PermTile {
   int n, T, nouter, is_dm;
   int S[16];
-  int A[16];  // A[k]: tile bit (index into S) of load-order bit k
+  int A[16];   // A[k]: tile bit (index into S) of load-order bit k
+  int Ain[16]; // perm[S[A[k]]]: the input bit of load-order bit k (resolved on the host)
   int O[CQS_MAX_BITS];
-  int perm[CQS_MAX_BITS];
+  int Oin[CQS_MAX_BITS];  // perm[O[b]]: the input bit of outer bit b
 };
 
 constexpr int kPermT = 12;
@@ -984,7 +985,7 @@ __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ a
   for (int b = 0; b < pt.nouter; ++b)
     if ((blockIdx.x >> b) & 1) {
       base_out |= 1ull << pt.O[b];
-      base_in |= 1ull << pt.perm[pt.O[b]];
+      base_in |= 1ull << pt.Oin[b];
     }
   const int T = pt.T, cnt = 1 << T;
   // thread part of the load-order index: bits 0..7 (or fewer when T < 8)
@@ -994,7 +995,7 @@ __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ a
   for (int k = 0; k < tb; ++k)
     if ((threadIdx.x >> k) & 1) {
       lt |= 1 << pt.A[k];
-      it |= 1ull << pt.perm[pt.S[pt.A[k]]];
+      it |= 1ull << pt.Ain[k];
     }
   const bool act = (int)threadIdx.x < cnt;
   // element loops unrolled to their compile-time bound (2^(kPermT - 8)) so that the per-element
@@ -1011,7 +1012,7 @@ __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ a
       const int k = tb + kk;
       if (k < T && ((e >> kk) & 1)) {
         le |= 1 << pt.A[k];
-        ie |= 1ull << pt.perm[pt.S[pt.A[k]]];
+        ie |= 1ull << pt.Ain[k];
       }
     }
     if (act) {
@@ -1210,7 +1211,6 @@ int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
   memset(&pt, 0, sizeof(pt));
   pt.n = n;
   pt.is_dm = is_dm ? 1 : 0;
-  for (int b = 0; b < n; ++b) pt.perm[b] = perm[b];
   const int T = n < kPermT ? n : kPermT;
   // tile bits: output bits 0..5, the output bits feeding input bits 0..5, then the lowest others
   // (loads and stores in runs of at least 64 elements; 12 tile bits hold both sets)
@@ -1247,10 +1247,14 @@ int cqs_permuted_probs(const void* amps, int32_t n_qubits, int32_t is_dm, const 
   }
   for (int t = 0; t < T; ++t)
     if (!used[t]) pt.A[k++] = t;
+  for (int t = 0; t < T; ++t) pt.Ain[t] = perm[pt.S[pt.A[t]]];
   pt.nouter = n - T;
   k = 0;
   for (int b = 0; b < n; ++b)
-    if (!in_s[b]) pt.O[k++] = b;
+    if (!in_s[b]) {
+      pt.Oin[k] = perm[b];
+      pt.O[k++] = b;
+    }
   const uint64_t grid = 1ull << pt.nouter;
   CQS_REQUIRE(grid < (1ull << 31), "too many tiles");
   hipLaunchKernelGGL(k_perm_probs, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), (const float2*)amps, out,
