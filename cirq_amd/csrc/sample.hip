@@ -998,34 +998,41 @@ __global__ __launch_bounds__(256) void k_perm_probs(const float2* __restrict__ a
       it |= 1ull << pt.Ain[k];
     }
   const bool act = (int)threadIdx.x < cnt;
-  // element loops unrolled to their compile-time bound (2^(kPermT - 8)) so that the per-element
-  // offsets (uniform) and the loads issue together
+  // element loops unrolled to their compile-time bound (2^(kPermT - 8)): every element's offsets
+  // (uniform) are formed and its load issued before any value is used
   constexpr int kE = 1 << (kPermT - 8);
   const int ne = cnt >> tb;
+  int le[kE];
+  uint64_t ie[kE];
 #pragma unroll
   for (int e = 0; e < kE; ++e) {
-    if (e >= ne) break;
-    int le = 0;
-    uint64_t ie = 0;
+    le[e] = 0;
+    ie[e] = 0;
 #pragma unroll
     for (int kk = 0; kk < kPermT - 8; ++kk) {
       const int k = tb + kk;
       if (k < T && ((e >> kk) & 1)) {
-        le |= 1 << pt.A[k];
-        ie |= 1ull << pt.Ain[k];
+        le[e] |= 1 << pt.A[k];
+        ie[e] |= 1ull << pt.Ain[k];
       }
     }
-    if (act) {
-      const uint64_t i = base_in | it | ie;
-      float v;
-      if (pt.is_dm) {
-        v = amps[i * (N + 1)].x;
-      } else {
-        const float2 a = amps[i];
-        v = fmaf(a.x, a.x, a.y * a.y);
-      }
-      const int w = lt | le;
-      tile[w + (w >> 5)] = v;
+  }
+  float v[kE];
+  if (pt.is_dm) {
+#pragma unroll
+    for (int e = 0; e < kE; ++e) v[e] = act && e < ne ? amps[(base_in | it | ie[e]) * (N + 1)].x : 0.f;
+  } else {
+    float2 a[kE];
+#pragma unroll
+    for (int e = 0; e < kE; ++e) a[e] = act && e < ne ? amps[base_in | it | ie[e]] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int e = 0; e < kE; ++e) v[e] = fmaf(a[e].x, a[e].x, a[e].y * a[e].y);
+  }
+#pragma unroll
+  for (int e = 0; e < kE; ++e) {
+    if (act && e < ne) {
+      const int w = lt | le[e];
+      tile[w + (w >> 5)] = v[e];
     }
   }
   __syncthreads();
