@@ -732,6 +732,27 @@ __device__ __forceinline__ void warm_program(const Prog& P) {
   }
 }
 
+// Whether the tile at outer offset `base` (local bits lpos) holds the basis index.
+template <int L>
+__device__ __forceinline__ bool basis_in_tile(int64_t basis, const int32_t* lpos, uint64_t base) {
+  uint64_t lm = 0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) lm |= 1ull << lpos[i];
+  return ((uint64_t)basis & ~lm) == base;
+}
+
+// Zeros at the tile's amplitudes (layout lpos_out).
+template <int L, int PER, int NT>
+__device__ __forceinline__ void store_zero_tile(float2* __restrict__ psi, const int32_t* lpos_out, uint64_t base,
+                                                bool act) {
+  uint64_t off_lo, off_hi[PER];
+  tile_offsets<L>(lpos_out, off_lo, off_hi);
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) psi[base + off_lo + off_hi[e]] = make_float2(0.f, 0.f);
+  }
+}
+
 // One workgroup's whole pass: load the tile from HBM (layout lpos) into registers, run the
 // program, store it (layout lpos_out).  `base` is the flat offset of the tile's outer bits.  With
 // basis >= 0 nothing is read: the tile starts as its part of the basis state |basis>.
@@ -743,6 +764,12 @@ __device__ __forceinline__ void pass_tile(float2* __restrict__ psi, const Prog& 
   float2* tile = lds;
   float2* mstage = lds + tile_words(L);
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
+  if (basis >= 0 && !basis_in_tile<L>(basis, lpos, base)) {
+    // every op of a pass acts within its tile and maps zero to zero: a tile that holds no part of
+    // |basis> stays zero (uniform across the workgroup, so no barrier is skipped by part of it)
+    store_zero_tile<L, PER, NT>(psi, lpos_out, base, act);
+    return;
+  }
   warm_program<NT>(P);
   float2 v[PER];
   {
