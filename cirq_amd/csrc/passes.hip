@@ -55,7 +55,7 @@ __global__ __launch_bounds__(TileShape<L>::NT) __attribute__((amdgpu_waves_per_e
   const uint32_t G = gridDim.x;
   if ((G & 7) == 0) b = (b & 7) * (G >> 3) + (b >> 3);
   const uint64_t base = deposit_bounded<31>((uint64_t)b, P.opos, P.nouter);
-  pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds, P.basis);
+  pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds, P.basis, P.zmask, P.zval);
 }
 
 // amps[i] = ((i & mask) == value) ? amps[i] * s : 0
@@ -121,6 +121,8 @@ bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const 
   P.mscale = 32768.f;
   P.warm = P.nouter <= kWarmOuterBits;
   P.basis = -1;
+  P.zmask = 0;
+  P.zval = 0;
   P.ops = ops + ps.op_begin;
   P.mats = mats;
   P.aux = aux;
@@ -387,6 +389,10 @@ static int run_passes(void* amps, int32_t n_bits, int64_t basis, const cqs_pass*
               (long long)basis);
   hipStream_t s = as_stream(stream);
   int64_t start = basis;  // consumed by the first pass that runs
+  // From a basis state, the bits that no pass so far had local still hold the basis's values:
+  // the state is zero elsewhere, so tiles that disagree on them are skipped (written as zeros).
+  const uint64_t all = n_bits >= 64 ? ~0ull : (1ull << n_bits) - 1;
+  uint64_t fixed = basis >= 0 ? all : 0;
   for (int i = 0; i < n_passes; ++i) {
     const cqs_pass& ps = passes[i];
     if (ps.op_count <= 0) continue;
@@ -395,6 +401,11 @@ static int run_passes(void* amps, int32_t n_bits, int64_t basis, const cqs_pass*
     if (!make_prog(P, n_bits, ps, ops, (const float2*)mats, aux)) return CQS_ERR_ARG;
     P.basis = start;
     start = -1;
+    uint64_t local = 0;
+    for (int b = 0; b < ps.L; ++b) local |= 1ull << ps.lpos[b];
+    P.zmask = fixed & ~local;
+    P.zval = (uint64_t)basis & P.zmask;
+    fixed &= ~local;
     const uint64_t tiles = 1ull << P.nouter;
     CQS_REQUIRE(tiles <= 0x7fffffffull, "too many tiles (n=%d, L=%d)", n_bits, ps.L);
     const int mk = ps.max_k;

@@ -30,6 +30,10 @@ struct TileProg {
   const float2* mats;
   const int32_t* aux;
   int64_t basis;  // >= 0: the pass starts from basis state |basis> instead of reading psi
+  // The state is zero outside {x : x & zmask == zval} (a chain of passes from a basis state: the
+  // bits no earlier pass had local still hold the basis's values), restricted here to the pass's
+  // outer bits: a tile whose outer bits disagree holds only zeros.  zmask = 0: no such knowledge.
+  uint64_t zmask, zval;
 };
 
 // The part of a pass program the tile interpreter reads (TileProg carries the same fields).
@@ -732,15 +736,6 @@ __device__ __forceinline__ void warm_program(const Prog& P) {
   }
 }
 
-// Whether the tile at outer offset `base` (local bits lpos) holds the basis index.
-template <int L>
-__device__ __forceinline__ bool basis_in_tile(int64_t basis, const int32_t* lpos, uint64_t base) {
-  uint64_t lm = 0;
-#pragma unroll
-  for (int i = 0; i < L; ++i) lm |= 1ull << lpos[i];
-  return ((uint64_t)basis & ~lm) == base;
-}
-
 // Zeros at the tile's amplitudes (layout lpos_out).
 template <int L, int PER, int NT>
 __device__ __forceinline__ void store_zero_tile(float2* __restrict__ psi, const int32_t* lpos_out, uint64_t base,
@@ -755,18 +750,19 @@ __device__ __forceinline__ void store_zero_tile(float2* __restrict__ psi, const 
 
 // One workgroup's whole pass: load the tile from HBM (layout lpos) into registers, run the
 // program, store it (layout lpos_out).  `base` is the flat offset of the tile's outer bits.  With
-// basis >= 0 nothing is read: the tile starts as its part of the basis state |basis>.
+// basis >= 0 nothing is read: the tile starts as its part of the basis state |basis>.  Tiles whose
+// outer bits disagree with (zmask, zval) hold only zeros and store zeros.
 template <int L, class Prog>
 __device__ __forceinline__ void pass_tile(float2* __restrict__ psi, const Prog& P, const int32_t* lpos,
                                           const int32_t* lpos_out, uint64_t base, float2* lds,
-                                          int64_t basis = -1) {
+                                          int64_t basis = -1, uint64_t zmask = 0, uint64_t zval = 0) {
   constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
   float2* tile = lds;
   float2* mstage = lds + tile_words(L);
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
-  if (basis >= 0 && !basis_in_tile<L>(basis, lpos, base)) {
-    // every op of a pass acts within its tile and maps zero to zero: a tile that holds no part of
-    // |basis> stays zero (uniform across the workgroup, so no barrier is skipped by part of it)
+  if ((base & zmask) != zval) {
+    // every op of a pass acts within its tile and maps zero to zero: a tile that holds only zeros
+    // stays zero, unread (uniform across the workgroup, so no barrier is skipped by part of it)
     store_zero_tile<L, PER, NT>(psi, lpos_out, base, act);
     return;
   }
