@@ -26,6 +26,24 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _chain_bytes(packed, n_bits: int) -> float:
+    """Algorithmic HBM bytes of one run of a program from a basis state: every pass writes the
+    state once; the first builds it in registers and reads nothing, and a later pass reads only the
+    tiles the state can reach yet (those whose outer bits that no earlier pass had local equal the
+    basis state's; the others hold zeros and are written without being read)."""
+    full = 8.0 * 2**n_bits
+    fixed = (1 << n_bits) - 1
+    total = 0.0
+    for i, ps in enumerate(packed.pass_arr[: int(packed.n_passes)]):
+        local = 0
+        for b in range(int(ps["L"])):
+            local |= 1 << int(ps["lpos"][b])
+        zmask = fixed & ~local
+        total += full + (0.0 if i == 0 else full / 2 ** bin(zmask).count("1"))
+        fixed &= ~local
+    return total
+
+
 def _progress(msg: str) -> None:
     """One line per finished leg on stderr (stdout carries only the result line)."""
     print(f"bench: {msg}", file=sys.stderr, flush=True)
@@ -120,7 +138,7 @@ def dm_leg(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     n_pass = int(comp.packed.n_passes)
-    bytes_per_pass = (2 * n_pass - 1) * 8 * 4**n / max(n_pass, 1)  # the first pass only writes
+    bytes_per_pass = _chain_bytes(comp.packed, 2 * n) / max(n_pass, 1)  # from |0..0><0..0|
     del rho
     torch.cuda.empty_cache()
     out = {
@@ -679,9 +697,9 @@ def grid_workload(args, rows: int, cols: int, depth: int, reps: int, steps: int,
     pass_ms = sum(a.elapsed_time(b) for a, b in pass_events) / len(pass_events)
     n_pass = int(comp.packed.n_passes)
     avg_launch_s = pass_ms / 1e3 / n_pass
-    # every pass reads and writes the state once, except the first, which builds |0..0> and only
-    # writes: the average algorithmic bytes per launch
-    bytes_per_launch = (2 * n_pass - 1) * 8 * (2**n) / n_pass
+    # every pass writes the state once and reads what the state can reach so far (the first builds
+    # |0..0> and reads nothing): the average algorithmic bytes per launch
+    bytes_per_launch = _chain_bytes(comp.packed, n) / n_pass
     achieved = bytes_per_launch / avg_launch_s / 1e9
     ms_per_step = wall / steps * 1e3
     layers_per_s = depth * steps / wall
