@@ -27,20 +27,27 @@ HBM_PEAK_GBS = 8000.0
 
 
 def _chain_bytes(packed, n_bits: int) -> float:
-    """Algorithmic HBM bytes of one run of a program from a basis state: every pass writes the
-    state once; the first builds it in registers and reads nothing, and a later pass reads only the
-    tiles the state can reach yet (those whose outer bits that no earlier pass had local equal the
-    basis state's; the others hold zeros and are written without being read)."""
+    """Algorithmic HBM bytes of one run of a program from a basis state.  A pass reads only the
+    tiles the state can reach by then (those whose bits that no earlier pass had local equal the
+    basis state's; the first pass builds the state in registers and reads nothing), and it writes
+    the tiles the next pass reads plus the zeros of the rest, except where the next pass writes
+    those zeros itself: it writes what the next pass reads, and the last pass writes everything."""
     full = 8.0 * 2**n_bits
     fixed = (1 << n_bits) - 1
-    total = 0.0
-    for i, ps in enumerate(packed.pass_arr[: int(packed.n_passes)]):
+    zmasks = []
+    for ps in packed.pass_arr[: int(packed.n_passes)]:
+        if int(ps["op_count"]) <= 0:
+            continue
         local = 0
         for b in range(int(ps["L"])):
             local |= 1 << int(ps["lpos"][b])
-        zmask = fixed & ~local
-        total += full + (0.0 if i == 0 else full / 2 ** bin(zmask).count("1"))
+        zmasks.append(fixed & ~local)
         fixed &= ~local
+    total = 0.0
+    for i, z in enumerate(zmasks):
+        read = 0.0 if i == 0 else full / 2 ** bin(z).count("1")
+        write = full / 2 ** bin(zmasks[i + 1]).count("1") if i + 1 < len(zmasks) else full
+        total += read + write
     return total
 
 

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(TileShape<L>::NT) __attribute__((amdgpu_waves_per_e
   const uint32_t G = gridDim.x;
   if ((G & 7) == 0) b = (b & 7) * (G >> 3) + (b >> 3);
   const uint64_t base = deposit_bounded<31>((uint64_t)b, P.opos, P.nouter);
-  pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds, P.basis, P.zmask, P.zval);
+  pass_tile<L>(psi, P, P.lpos, P.lpos_out, base, lds, P.basis, P.zmask, P.zval, P.wmask, P.wval);
 }
 
 // amps[i] = ((i & mask) == value) ? amps[i] * s : 0
@@ -123,6 +123,8 @@ bool make_prog(TileProg& P, int n, const cqs_pass& ps, const cqs_op* ops, const 
   P.basis = -1;
   P.zmask = 0;
   P.zval = 0;
+  P.wmask = 0;
+  P.wval = 0;
   P.ops = ops + ps.op_begin;
   P.mats = mats;
   P.aux = aux;
@@ -390,7 +392,8 @@ static int run_passes(void* amps, int32_t n_bits, int64_t basis, const cqs_pass*
   hipStream_t s = as_stream(stream);
   int64_t start = basis;  // consumed by the first pass that runs
   // From a basis state, the bits that no pass so far had local still hold the basis's values:
-  // the state is zero elsewhere, so tiles that disagree on them are skipped (written as zeros).
+  // the state is zero elsewhere, so tiles that disagree on them are skipped (written as zeros, and
+  // not written at all where the next pass does not read them either).
   const uint64_t all = n_bits >= 64 ? ~0ull : (1ull << n_bits) - 1;
   uint64_t fixed = basis >= 0 ? all : 0;
   for (int i = 0; i < n_passes; ++i) {
@@ -406,6 +409,22 @@ static int run_passes(void* amps, int32_t n_bits, int64_t basis, const cqs_pass*
     P.zmask = fixed & ~local;
     P.zval = (uint64_t)basis & P.zmask;
     fixed &= ~local;
+    // The next pass that runs reads only the tiles its own (zmask, zval) keeps and stores zeros
+    // over the rest: this pass leaves the rest unwritten.  Its mask lies within this pass's outer
+    // bits (fixed no longer holds this pass's local bits), so whole tiles are kept or left.
+    for (int j = i + 1; fixed != 0 && j < n_passes; ++j) {
+      if (passes[j].op_count <= 0) continue;
+      CQS_REQUIRE(passes[j].L >= 0 && passes[j].L <= CQS_MAX_LOCAL, "pass %d: L=%d out of range", j, passes[j].L);
+      uint64_t next = 0;
+      for (int b = 0; b < passes[j].L; ++b) {
+        CQS_REQUIRE(passes[j].lpos[b] >= 0 && passes[j].lpos[b] < n_bits, "pass %d: lpos[%d]=%d out of range", j,
+                    b, passes[j].lpos[b]);
+        next |= 1ull << passes[j].lpos[b];
+      }
+      P.wmask = fixed & ~next;
+      P.wval = (uint64_t)basis & P.wmask;
+      break;
+    }
     const uint64_t tiles = 1ull << P.nouter;
     CQS_REQUIRE(tiles <= 0x7fffffffull, "too many tiles (n=%d, L=%d)", n_bits, ps.L);
     const int mk = ps.max_k;

@@ -34,6 +34,9 @@ struct TileProg {
   // bits no earlier pass had local still hold the basis's values), restricted here to the pass's
   // outer bits: a tile whose outer bits disagree holds only zeros.  zmask = 0: no such knowledge.
   uint64_t zmask, zval;
+  // Tiles whose outer bits disagree with (wmask, wval) are not stored at all: the next pass of the
+  // same run reads none of their elements (it stores them itself, as zeros).  wmask = 0: store all.
+  uint64_t wmask, wval;
 };
 
 // The part of a pass program the tile interpreter reads (TileProg carries the same fields).
@@ -751,15 +754,18 @@ __device__ __forceinline__ void store_zero_tile(float2* __restrict__ psi, const 
 // One workgroup's whole pass: load the tile from HBM (layout lpos) into registers, run the
 // program, store it (layout lpos_out).  `base` is the flat offset of the tile's outer bits.  With
 // basis >= 0 nothing is read: the tile starts as its part of the basis state |basis>.  Tiles whose
-// outer bits disagree with (zmask, zval) hold only zeros and store zeros.
+// outer bits disagree with (zmask, zval) hold only zeros and store zeros; tiles whose outer bits
+// disagree with (wmask, wval) store nothing.
 template <int L, class Prog>
 __device__ __forceinline__ void pass_tile(float2* __restrict__ psi, const Prog& P, const int32_t* lpos,
                                           const int32_t* lpos_out, uint64_t base, float2* lds,
-                                          int64_t basis = -1, uint64_t zmask = 0, uint64_t zval = 0) {
+                                          int64_t basis = -1, uint64_t zmask = 0, uint64_t zval = 0,
+                                          uint64_t wmask = 0, uint64_t wval = 0) {
   constexpr int NT = TileShape<L>::NT, PER = TileShape<L>::PER;
   float2* tile = lds;
   float2* mstage = lds + tile_words(L);
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
+  if ((base & wmask) != wval) return;  // nobody reads this tile before the next pass stores it
   if ((base & zmask) != zval) {
     // every op of a pass acts within its tile and maps zero to zero: a tile that holds only zeros
     // stays zero, unread (uniform across the workgroup, so no barrier is skipped by part of it)

@@ -183,6 +183,7 @@ extern "C" __global__ __launch_bounds__({nt}) __attribute__((amdgpu_waves_per_eu
   const bool act = L >= 8 || (int)threadIdx.x < (1 << L);
   (void)ops;
   (void)mstage;
+  if ((base & P.wmask) != P.wval) return;  // nobody reads this tile before the next pass stores it
   if ((base & P.zmask) != P.zval) {{  // a tile of zeros stays zero (ops map zero to zero)
     store_zero_tile<L, PER, NT>(psi, P.lpos_out, base, act);
     return;

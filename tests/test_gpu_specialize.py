@@ -141,25 +141,29 @@ def test_specialised_density_matrix_gradients_match_interpreter(gpu, invert):
     np.testing.assert_allclose(g1, g0, atol=1e-6)
 
 
-@pytest.mark.parametrize("basis", [0, 0b1011001110100101, 2**16 - 1])
-def test_pass_chain_from_basis_skips_zero_tiles(gpu, basis):
+@pytest.mark.parametrize("rows,cols,L,basis", [(4, 4, 11, 0), (4, 4, 11, 0b1011001110100101), (4, 4, 11, 2**16 - 1),
+                                                (4, 5, 8, 0), (4, 5, 8, 0b10110011101001011010)])
+def test_pass_chain_from_basis_skips_zero_tiles(gpu, rows, cols, L, basis):
     """From a basis state the passes skip the tiles the state cannot reach yet (written as zeros,
-    not read); the result equals the oracle's from the same basis state, through the interpreter
-    and through specialised kernels."""
-    _log(f"pass chain from basis {basis}")
-    c = generators.grid_circuit(4, 4, 12, seed=4)
+    not read) and leave unwritten the tiles the next pass does not read; on buffers filled with NaN
+    the result equals the oracle's from the same basis state, through the interpreter and through
+    specialised kernels (with 8-bit tiles at 20 qubits the first pass leaves most tiles unwritten)."""
+    n = rows * cols
+    _log(f"pass chain from basis {basis}, {n} qubits, L={L}")
+    c = generators.grid_circuit(rows, cols, 12, seed=4)
     qs = sorted(c.all_qubits())
     low = tape.Lowering(qs, False)
     low.lower_circuit(c)
-    packed = tape.compile_ops(low.ops, 16, L=11, coalesce=3, blocks=True, kmax=4, variant_bits=2)
-    want = O.simulate_state_vector(to_oracle_ops(c, qs), 16, initial_state=basis)
+    packed = tape.compile_ops(low.ops, n, L=L, coalesce=3, blocks=True, kmax=4, variant_bits=2)
+    want = O.simulate_state_vector(to_oracle_ops(c, qs), n, initial_state=basis)
     dp = engine.DeviceProgram(packed, gpu)
-    a = engine.new_state(16, gpu)
+    a = torch.full((1 << n,), float("nan"), dtype=torch.complex64, device=gpu)
     dp.run(a, basis=basis)
     dp.specialize()
-    b = engine.new_state(16, gpu)
+    b = torch.full((1 << n,), float("nan"), dtype=torch.complex64, device=gpu)
     dp.run(b, basis=basis)
     torch.cuda.synchronize()
     for got in (a.cpu().numpy(), b.cpu().numpy()):
+        assert not np.isnan(got).any()
         assert np.max(np.abs(got - want)) <= 1e-5
         assert fidelity(got, want) >= 1 - 1e-6
