@@ -161,7 +161,9 @@ int cqs_program_validate(int32_t n_bits, const cqs_pass* passes, int32_t n_passe
 /*
  * cqs_init_basis(amps, n_bits, basis) followed by cqs_run_passes, with the initial state never
  * written or read: the first pass that runs builds its tiles of |basis> in registers (amps may
- * hold anything on entry).  With no pass to run it is cqs_init_basis.  The same initial states as
+ * hold anything on entry).  Tiles the state cannot reach yet are not read, and a pass leaves
+ * unwritten the tiles the next pass does not read; every element is written by the time the call's
+ * work completes on the stream.  With no pass to run it is cqs_init_basis.  The same initial states as
  * cirq-core/cirq/sim/state_vector_simulation_state.py:49-76 and
  * cirq-core/cirq/sim/density_matrix_simulation_state.py:41-76, one HBM write and read fewer.
  */
