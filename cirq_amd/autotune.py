@@ -101,15 +101,18 @@ class Autotuner:
             json.dump({k: dataclasses.asdict(v) for k, v in self.cache.items()}, f, indent=1)
         os.replace(tmp, self.path)
 
-    def time_program(self, ops: list[LOp], n_bits: int, cfg: TuneConfig, state: torch.Tensor) -> float:
-        """Average ms of one run of the program compiled with cfg (HIP events on torch's stream)."""
+    def time_program(self, ops: list[LOp], n_bits: int, cfg: TuneConfig, state: torch.Tensor,
+                     basis: int | None = 0) -> float:
+        """Average ms of one run of the program compiled with cfg (HIP events on torch's stream),
+        from |basis> as simulations from a computational basis state run (the passes then skip the
+        tiles the state cannot reach yet); basis None: from the state in the buffer."""
         prog = engine.DeviceProgram(compile_ops(ops, n_bits, **cfg.compile_kwargs(n_bits)), state.device)
-        prog.run(state)
+        prog.run(state, basis=basis)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(self.reps):
-            prog.run(state)
+            prog.run(state, basis=basis)
         e1.record()
         e1.synchronize()
         return e0.elapsed_time(e1) / self.reps
